@@ -1,0 +1,164 @@
+/*
+ * C ABI of the MI355X (gfx950) approximate KNN-graph builder.
+ *
+ * The drop-in boundary is the reference's
+ *   static void SolutionKmeans::constructResult(float points[][112], uint32_t numPoints,
+ *                                               std::vector<std::vector<uint32_t>>& result)
+ * (src/SolutionKmeans.hpp:558, called from src/main.cpp:59). knng_build() replaces it; the C++
+ * header knng_solution.hpp wraps it back into that exact signature.
+ *
+ * All pointers are plain host or device pointers with sizes; no library-specific types cross the
+ * boundary except the opaque handle and the parameter/statistics structs below. Every function
+ * returns KNNG_OK (0) or a negative status; knng_last_error() describes the last failure on the
+ * calling thread. A handle is not reentrant: calls on one handle must not overlap.
+ */
+#ifndef KNNG_H
+#define KNNG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KNNG_OK 0
+#define KNNG_ERR_INVALID (-1)     /* bad argument */
+#define KNNG_ERR_HIP (-2)         /* HIP runtime error (message in knng_last_error) */
+#define KNNG_ERR_UNSUPPORTED (-3) /* shape outside what the kernels handle */
+#define KNNG_ERR_NOMEM (-4)       /* device allocation failed */
+#define KNNG_ERR_OVERFLOW (-5)    /* a bounded buffer overflowed (exact k-NN candidates) */
+
+#define KNNG_PREC_FP32 0 /* exact FP32 lists: fp16 screen with a rigorous margin, FP32 re-rank */
+#define KNNG_PREC_FP16 1 /* fp16 MFMA screen, no margin, FP32 re-rank of the survivors */
+#define KNNG_PREC_BF16 2 /* bf16 MFMA screen, no margin, FP32 re-rank of the survivors */
+
+/* Build parameters. The reference hard-codes k = dims = 100 (src/Constants.hpp:23-24),
+ * maxGroupSize 200/400 alternating (src/SolutionKmeans.hpp:582-587), knnIterations = 1 (:591),
+ * the 3000-point small-branch threshold (:156) and a wall-clock round budget (:562). */
+typedef struct knng_params {
+    uint32_t k;            /* neighbours per point, 1..128 */
+    uint32_t leaf_max;     /* largest leaf bucket, inclusive (reference maxGroupSize - 1) */
+    uint32_t leaf_max_odd; /* leaf_max on odd rounds; 0 = leaf_max */
+    uint32_t small_max;    /* nodes below this size take the small branch, <= 4096 */
+    uint32_t iterations;   /* Lloyd iterations per split */
+    uint32_t rounds;       /* split rounds */
+    uint32_t non_passes;   /* neighbours-of-neighbours passes after the rounds */
+    uint32_t precision;    /* KNNG_PREC_* */
+    uint32_t rerank;       /* reserved: per-point re-rank cap of the approximate screens */
+    uint32_t flags;        /* reserved, 0 */
+    uint64_t seed;
+} knng_params;
+
+/* Per-build timings (milliseconds, HIP events on the build stream) and counters. */
+typedef struct knng_stats {
+    double split_ms;       /* all split levels of all rounds */
+    double side_ms;        /* of which: the full-range side test kernels */
+    double leaf_ms;        /* leaf Gram + top-k merge */
+    double non_ms;         /* neighbours-of-neighbours */
+    double total_ms;       /* whole build on device, input resident, output written */
+    uint64_t side_launches;
+    uint64_t side_rows;    /* rows read by the side test */
+    uint64_t leaf_launches;
+    uint64_t leaves;
+    uint64_t leaf_pairs;   /* sum over leaves of b*(b-1)/2 */
+    uint64_t survivors;    /* screened pairs re-ranked in FP32 */
+    uint64_t inserts;      /* list insertions */
+    uint64_t split_failures;
+    uint64_t split_fallbacks;
+} knng_stats;
+
+typedef struct knng_handle knng_handle;
+
+const char* knng_last_error(void);
+void knng_default_params(knng_params* p);
+
+int knng_create(const knng_params* params, knng_handle** out);
+void knng_destroy(knng_handle* h);
+int knng_set_params(knng_handle* h, const knng_params* params);
+int knng_get_stats(const knng_handle* h, knng_stats* out);
+
+/* Replaces SolutionKmeans::constructResult (src/SolutionKmeans.hpp:558-616). points: host rows of
+ * row_stride floats, the first d used (the reference passes stride 112, d 100). out_ids: n*k host,
+ * each row ascending by (distance, id), padded with id 1 (padResult, src/KnnSets.hpp:334-344).
+ * out_dists (nullable): n*k FP32 distances, FLT_MAX for padding. */
+int knng_build(knng_handle* h, const float* points, uint32_t n, uint32_t d, size_t row_stride,
+               uint32_t* out_ids, float* out_dists);
+
+/* Same with device-resident input and output (row_stride a multiple of 4, 16-byte aligned).
+ * Returns after the build completed on the device. */
+int knng_build_device(knng_handle* h, const float* d_points, uint32_t n, uint32_t d,
+                      size_t row_stride, uint32_t* d_out_ids, float* d_out_dists);
+
+/* ---- stage entry points (host buffers), each one reference routine ------------------------- */
+
+/* One round of splitKmeansBinaryProcess from the identity permutation
+ * (src/SolutionKmeans.hpp:142-311, :589-591). perm_out: n; leaf_start/leaf_size: capacity n,
+ * leaves ascending by start; *n_leaves receives the count. */
+int knng_split_round(knng_handle* h, const float* points, uint32_t n, uint32_t d,
+                     size_t row_stride, uint32_t round, uint32_t* perm_out, uint32_t* leaf_start,
+                     uint32_t* leaf_size, uint32_t* n_leaves);
+
+/* Side test and stable partition of node ranges of perm with given hyperplanes
+ * (`dot(coefs, pt) >= offset` first, src/SolutionKmeans.hpp:189-197). coefs: n_nodes*d. */
+int knng_partition_injected(knng_handle* h, const float* points, uint32_t n, uint32_t d,
+                            size_t row_stride, uint32_t* perm, uint32_t n_nodes,
+                            const uint32_t* node_start, const uint32_t* node_size,
+                            const float* coefs, const float* offsets, uint32_t* lo_out);
+
+/* addCandidatesLessThan over every leaf (src/KnnSets.hpp:131-160) on a state of n*k ids/dists,
+ * each row sorted by (dist, id) and padded (FLT_MAX, 0xFFFFFFFF). ids/dists are updated in place;
+ * ids == NULL starts from empty sets (then ids/dists must still be writable n*k buffers). */
+int knng_leaf_stage(knng_handle* h, const float* points, uint32_t n, uint32_t d,
+                    size_t row_stride, const uint32_t* perm, const uint32_t* leaf_start,
+                    const uint32_t* leaf_size, uint32_t n_leaves, uint32_t* ids, float* dists,
+                    int start_empty);
+
+/* topUpSingle (src/SolutionKmeans.hpp:480-555) on a state as above, in place. */
+int knng_non(knng_handle* h, const float* points, uint32_t n, uint32_t d, size_t row_stride,
+             uint32_t* ids, float* dists);
+
+/* ---- exact FP64 k-NN (recall measurement; the baseline grade.cpp compares against) -------- */
+
+/* Exact k nearest (self excluded) of each query id by FP64 squared L2, ascending (dist, id).
+ * Device pointers. upper (nullable, nq): a known upper bound on each query's k-th distance (only
+ * points within it are kept); cap: candidate slots per query. Returns KNNG_ERR_OVERFLOW when a
+ * query had more than cap candidates within its bound. */
+int knng_exact_knn_device(knng_handle* h, const float* d_points, uint32_t n, uint32_t d,
+                          size_t row_stride, const uint32_t* d_queries, uint32_t nq, uint32_t k,
+                          const double* d_upper, uint32_t cap, uint32_t* d_out_ids,
+                          double* d_out_dists);
+
+/* FP64 upper bounds for knng_exact_knn_device from candidate lists (nq*k device ids): the FP64
+ * distance of the k-th distinct valid entry, +inf if fewer. */
+int knng_exact_upper_from_lists(knng_handle* h, const float* d_points, uint32_t n, uint32_t d,
+                                size_t row_stride, const uint32_t* d_queries, uint32_t nq,
+                                const uint32_t* d_lists, uint32_t k, double* d_upper);
+
+/* Recall as grade.cpp:61-95 computes it: mean over rows of |truth_row ∩ result_row|, divided by
+ * k. Host arrays of rows*k ids. */
+double knng_recall(const uint32_t* result_ids, const uint32_t* truth_ids, uint32_t rows,
+                   uint32_t k);
+
+/* ---- synthetic data and device memory (for callers without their own GPU runtime) --------- */
+
+#define KNNG_GEN_LOWRANK 0  /* z W + eps, 24 latent dims, sigma_i = 1/sqrt(1+i), unit norm */
+#define KNNG_GEN_MIXTURE 1  /* 500 centres uniform in [-10,10]^d, isotropic noise 1.0 */
+#define KNNG_GEN_ZIPF 2     /* 20000 centres, Zipf(1.1) weights, sigma ~ U[0.3, 2.0] */
+
+/* Fills n rows of row_stride floats at d_points (device). */
+int knng_generate_device(knng_handle* h, int kind, uint32_t n, uint32_t d, size_t row_stride,
+                         uint64_t seed, float* d_points);
+
+int knng_device_alloc(size_t bytes, void** out);
+int knng_device_free(void* p);
+int knng_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int knng_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int knng_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int knng_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KNNG_H */

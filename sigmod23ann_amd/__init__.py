@@ -1,0 +1,345 @@
+"""MI355X (gfx950) approximate KNN-graph builder: Python view of ``libknng.so``.
+
+The product is the C ABI in ``include/knng.h`` (HIP kernels + C++ host driver in ``csrc/``).
+This module loads it with ctypes and mirrors the reference's entry point::
+
+    construct_result(points, k=100, ...)  ->  (n, k) uint32 ids
+
+which replaces ``SolutionKmeans::constructResult`` (src/SolutionKmeans.hpp:558-616). There is no
+CPU fallback: every call goes to the GPU library, and importing fails loudly when the library has
+not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libknng.so")
+CSRC = os.path.join(_HERE, "csrc")
+
+OK = 0
+ERR_INVALID = -1
+ERR_HIP = -2
+ERR_UNSUPPORTED = -3
+ERR_NOMEM = -4
+ERR_OVERFLOW = -5
+
+PREC_FP32 = 0
+PREC_FP16 = 1
+PREC_BF16 = 2
+
+GEN_LOWRANK = 0
+GEN_MIXTURE = 1
+GEN_ZIPF = 2
+
+# Every symbol include/knng.h declares.
+EXPORTS = [
+    "knng_last_error", "knng_default_params", "knng_create", "knng_destroy", "knng_set_params",
+    "knng_get_stats", "knng_build", "knng_build_device", "knng_split_round",
+    "knng_partition_injected", "knng_leaf_stage", "knng_non", "knng_exact_knn_device",
+    "knng_exact_upper_from_lists", "knng_recall", "knng_generate_device", "knng_device_alloc",
+    "knng_device_free", "knng_memcpy_h2d", "knng_memcpy_d2h", "knng_memcpy_d2d",
+    "knng_synchronize",
+]
+
+
+def build(force: bool = True, jobs: int = 8) -> str:
+    """Compile libknng.so for gfx950 with hipcc (csrc/Makefile)."""
+    args = ["make", "-C", CSRC, f"-j{jobs}"]
+    if force:
+        args.append("-B")
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("k", C.c_uint32),
+        ("leaf_max", C.c_uint32),
+        ("leaf_max_odd", C.c_uint32),
+        ("small_max", C.c_uint32),
+        ("iterations", C.c_uint32),
+        ("rounds", C.c_uint32),
+        ("non_passes", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("rerank", C.c_uint32),
+        ("flags", C.c_uint32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("split_ms", C.c_double),
+        ("side_ms", C.c_double),
+        ("leaf_ms", C.c_double),
+        ("non_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("side_launches", C.c_uint64),
+        ("side_rows", C.c_uint64),
+        ("leaf_launches", C.c_uint64),
+        ("leaves", C.c_uint64),
+        ("leaf_pairs", C.c_uint64),
+        ("survivors", C.c_uint64),
+        ("inserts", C.c_uint64),
+        ("split_failures", C.c_uint64),
+        ("split_fallbacks", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class KnngError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"knng error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_P = C.POINTER
+_u32p = _P(C.c_uint32)
+_f32p = _P(C.c_float)
+_f64p = _P(C.c_double)
+_vp = C.c_void_p
+_sz = C.c_size_t
+_u32 = C.c_uint32
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with sigmod23ann_amd.build()")
+        L = C.CDLL(LIB_PATH)
+        L.knng_last_error.restype = C.c_char_p
+        L.knng_default_params.argtypes = [_P(Params)]
+        L.knng_create.argtypes = [_P(Params), _P(_vp)]
+        L.knng_destroy.argtypes = [_vp]
+        L.knng_set_params.argtypes = [_vp, _P(Params)]
+        L.knng_get_stats.argtypes = [_vp, _P(Stats)]
+        L.knng_build.argtypes = [_vp, _f32p, _u32, _u32, _sz, _u32p, _f32p]
+        L.knng_build_device.argtypes = [_vp, _vp, _u32, _u32, _sz, _vp, _vp]
+        L.knng_split_round.argtypes = [_vp, _f32p, _u32, _u32, _sz, _u32, _u32p, _u32p, _u32p,
+                                       _u32p]
+        L.knng_partition_injected.argtypes = [_vp, _f32p, _u32, _u32, _sz, _u32p, _u32, _u32p,
+                                              _u32p, _f32p, _f32p, _u32p]
+        L.knng_leaf_stage.argtypes = [_vp, _f32p, _u32, _u32, _sz, _u32p, _u32p, _u32p, _u32,
+                                      _u32p, _f32p, C.c_int]
+        L.knng_non.argtypes = [_vp, _f32p, _u32, _u32, _sz, _u32p, _f32p]
+        L.knng_exact_knn_device.argtypes = [_vp, _vp, _u32, _u32, _sz, _vp, _u32, _u32, _vp,
+                                            _u32, _vp, _vp]
+        L.knng_exact_upper_from_lists.argtypes = [_vp, _vp, _u32, _u32, _sz, _vp, _u32, _vp,
+                                                  _u32, _vp]
+        L.knng_recall.argtypes = [_u32p, _u32p, _u32, _u32]
+        L.knng_recall.restype = C.c_double
+        L.knng_generate_device.argtypes = [_vp, C.c_int, _u32, _u32, _sz, C.c_uint64, _vp]
+        L.knng_device_alloc.argtypes = [_sz, _P(_vp)]
+        L.knng_device_free.argtypes = [_vp]
+        L.knng_memcpy_h2d.argtypes = [_vp, _vp, _sz]
+        L.knng_memcpy_d2h.argtypes = [_vp, _vp, _sz]
+        L.knng_memcpy_d2d.argtypes = [_vp, _vp, _sz]
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != OK:
+        raise KnngError(code, lib().knng_last_error().decode(errors="replace"))
+
+
+def params(k=100, leaf_max=400, leaf_max_odd=0, small_max=3000, iterations=1, rounds=8,
+           non_passes=1, precision=PREC_FP32, rerank=160, seed=12345) -> Params:
+    return Params(k, leaf_max, leaf_max_odd, small_max, iterations, rounds, non_passes, precision,
+                  rerank, 0, seed)
+
+
+def _f(a):
+    return a.ctypes.data_as(_f32p)
+
+
+def _u(a):
+    return a.ctypes.data_as(_u32p)
+
+
+def _points(X):
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    if X.ndim != 2:
+        raise ValueError("points must be 2-D (n, d)")
+    return X, X.shape[0], X.shape[1]
+
+
+class DeviceArray:
+    """Device allocation owned by Python (plain hipMalloc through the library)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = _vp()
+        _check(lib().knng_device_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().knng_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def upload(self, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(lib().knng_memcpy_h2d(self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def download(self, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().knng_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+
+class Builder:
+    """One handle: device buffers and a HIP stream reused across calls (not reentrant)."""
+
+    def __init__(self, prm: Params | None = None, **kw):
+        self.prm = prm if prm is not None else params(**kw)
+        h = _vp()
+        _check(lib().knng_create(C.byref(self.prm), C.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().knng_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, prm: Params):
+        _check(lib().knng_set_params(self.h, C.byref(prm)))
+        self.prm = prm
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().knng_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    # -- whole build (constructResult) --------------------------------------------------------
+    def build(self, X, return_dists=False):
+        X, n, d = _points(X)
+        k = self.prm.k
+        ids = np.empty((n, k), np.uint32)
+        ds = np.empty((n, k), np.float32) if return_dists else None
+        _check(lib().knng_build(self.h, _f(X), n, d, d, _u(ids), _f(ds) if ds is not None else None))
+        return (ids, ds) if return_dists else ids
+
+    def build_device(self, d_points: int, n: int, d: int, row_stride: int, d_out_ids: int,
+                     d_out_dists: int | None = None):
+        _check(lib().knng_build_device(self.h, d_points, n, d, row_stride, d_out_ids,
+                                       d_out_dists))
+
+    # -- stage entry points -------------------------------------------------------------------
+    def split_round(self, X, round_idx: int):
+        X, n, d = _points(X)
+        perm = np.empty(n, np.uint32)
+        ls = np.empty(max(n, 1), np.uint32)
+        lz = np.empty(max(n, 1), np.uint32)
+        nl = C.c_uint32(0)
+        _check(lib().knng_split_round(self.h, _f(X), n, d, d, round_idx, _u(perm), _u(ls), _u(lz),
+                                      C.byref(nl)))
+        return perm, ls[: nl.value].copy(), lz[: nl.value].copy()
+
+    def partition_injected(self, X, perm, node_start, node_size, coefs, offsets):
+        X, n, d = _points(X)
+        perm = np.ascontiguousarray(perm, np.uint32).copy()
+        ns = np.ascontiguousarray(node_start, np.uint32)
+        nz = np.ascontiguousarray(node_size, np.uint32)
+        cf = np.ascontiguousarray(coefs, np.float32)
+        of = np.ascontiguousarray(offsets, np.float32)
+        lo = np.empty(ns.shape[0], np.uint32)
+        _check(lib().knng_partition_injected(self.h, _f(X), n, d, d, _u(perm), ns.shape[0], _u(ns),
+                                             _u(nz), _f(cf), _f(of), _u(lo)))
+        return perm, lo
+
+    def leaf_stage(self, X, perm, leaf_start, leaf_size, state=None):
+        """state = (ids, dists) each (n, k), rows sorted by (dist, id); None = empty sets."""
+        X, n, d = _points(X)
+        k = self.prm.k
+        if state is None:
+            ids = np.empty((n, k), np.uint32)
+            ds = np.empty((n, k), np.float32)
+            empty = 1
+        else:
+            ids = np.ascontiguousarray(state[0], np.uint32).copy()
+            ds = np.ascontiguousarray(state[1], np.float32).copy()
+            empty = 0
+        perm = np.ascontiguousarray(perm, np.uint32)
+        ls = np.ascontiguousarray(leaf_start, np.uint32)
+        lz = np.ascontiguousarray(leaf_size, np.uint32)
+        _check(lib().knng_leaf_stage(self.h, _f(X), n, d, d, _u(perm), _u(ls), _u(lz), ls.shape[0],
+                                     _u(ids), _f(ds), empty))
+        return ids, ds
+
+    def non(self, X, state):
+        X, n, d = _points(X)
+        ids = np.ascontiguousarray(state[0], np.uint32).copy()
+        ds = np.ascontiguousarray(state[1], np.float32).copy()
+        _check(lib().knng_non(self.h, _f(X), n, d, d, _u(ids), _f(ds)))
+        return ids, ds
+
+    # -- recall tooling -------------------------------------------------------------------------
+    def exact_knn_device(self, d_points, n, d, row_stride, d_queries, nq, k, d_upper, cap,
+                         d_out_ids, d_out_dists):
+        _check(lib().knng_exact_knn_device(self.h, d_points, n, d, row_stride, d_queries, nq, k,
+                                           d_upper, cap, d_out_ids, d_out_dists))
+
+    def exact_upper_from_lists(self, d_points, n, d, row_stride, d_queries, nq, d_lists, k,
+                               d_upper):
+        _check(lib().knng_exact_upper_from_lists(self.h, d_points, n, d, row_stride, d_queries,
+                                                 nq, d_lists, k, d_upper))
+
+    def exact_knn(self, X, queries, k, cap=None):
+        """Host convenience: exact FP64 k-NN of query ids (uploads X)."""
+        X, n, d = _points(X)
+        q = np.ascontiguousarray(queries, np.uint32)
+        cap = cap or max(n, k)
+        dX = DeviceArray(X.nbytes)
+        dX.upload(X)
+        dq = DeviceArray(q.nbytes)
+        dq.upload(q)
+        di = DeviceArray(q.shape[0] * k * 4)
+        dd = DeviceArray(q.shape[0] * k * 8)
+        self.exact_knn_device(dX.ptr, n, d, d, dq.ptr, q.shape[0], k, None, cap, di.ptr, dd.ptr)
+        return di.download((q.shape[0], k), np.uint32), dd.download((q.shape[0], k), np.float64)
+
+    def generate_device(self, kind: int, n: int, d: int, row_stride: int, seed: int,
+                        d_points: int):
+        _check(lib().knng_generate_device(self.h, kind, n, d, row_stride, seed, d_points))
+
+
+def recall(result_ids, truth_ids) -> float:
+    """grade.cpp:61-95: mean |truth ∩ result| per row, divided by k."""
+    r = np.ascontiguousarray(result_ids, np.uint32)
+    t = np.ascontiguousarray(truth_ids, np.uint32)
+    return lib().knng_recall(_u(r), _u(t), t.shape[0], t.shape[1])
+
+
+def construct_result(points, k=100, **kw):
+    """constructResult (src/SolutionKmeans.hpp:558): (n, d) float32 -> (n, k) uint32 ids, each row
+    ascending by distance, short rows padded with id 1."""
+    b = Builder(params(k=k, **kw))
+    try:
+        return b.build(points)
+    finally:
+        b.close()

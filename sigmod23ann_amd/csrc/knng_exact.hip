@@ -1,0 +1,199 @@
+// Exact FP64 k-nearest neighbours of sampled query points (the recall reference; the role of the
+// externally built exact baseline that grade.cpp:42-98 compares against).
+//
+// Distances are n_q + n_r - 2 q.r in FP64 (FP32 inputs, products exact in FP64). Every database
+// point within the query's upper bound ub_q (+inf allowed) is appended to that query's candidate
+// buffer; the k smallest (dist, id) of the buffer are the answer. A caller that knows a valid
+// upper bound (for instance the FP64 distance of the k-th entry of any k distinct points) keeps the
+// buffers small; an overflowing query is reported, never silently truncated.
+#include "knng_device.hpp"
+#include "knng_exact.hpp"
+
+namespace knng {
+
+constexpr uint32_t kExQ = 64;     // queries per block
+constexpr uint32_t kExR = 64;     // database rows per step
+constexpr uint32_t kExThreads = 256;
+
+__global__ void k_norms64(const float* __restrict__ X, uint32_t ld, uint32_t d, uint32_t n,
+                          double* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* x = X + (size_t)i * ld;
+    double s = 0.0;
+    for (uint32_t j = 0; j < d; ++j) s += (double)x[j] * (double)x[j];
+    out[i] = s;
+}
+
+__global__ __launch_bounds__(kExThreads) void k_exact_scan(ExactScanArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t d = a.d, ld = a.ld;
+    const uint32_t dp = d + 1;  // padded row stride (doubles) in LDS
+    double* sq = reinterpret_cast<double*>(lds);
+    double* sr = sq + kExQ * dp;
+    const uint32_t q0 = blockIdx.x * kExQ;
+    const uint32_t tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    for (uint32_t t = tid; t < kExQ * d; t += kExThreads) {
+        const uint32_t qi = t / d, j = t - qi * d;
+        const uint32_t q = q0 + qi;
+        sq[qi * dp + j] = q < a.nq ? (double)a.X[(size_t)a.queries[q] * ld + j] : 0.0;
+    }
+    double qn[4], ub[4];
+    uint32_t qid[4];
+    bool qok[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t q = q0 + ty + 16 * i;
+        qok[i] = q < a.nq;
+        qid[i] = qok[i] ? a.queries[q] : kNoId;
+        qn[i] = qok[i] ? a.norms[qid[i]] : 0.0;
+        ub[i] = qok[i] ? a.ub[q] : 0.0;
+    }
+    const uint64_t per = ((uint64_t)a.n + gridDim.y - 1) / gridDim.y;
+    const uint32_t r_begin = (uint32_t)min<uint64_t>((uint64_t)blockIdx.y * per, a.n);
+    const uint32_t r_end = (uint32_t)min<uint64_t>((uint64_t)r_begin + per, a.n);
+    for (uint32_t r0 = r_begin; r0 < r_end; r0 += kExR) {
+        __syncthreads();
+        for (uint32_t t = tid; t < kExR * d; t += kExThreads) {
+            const uint32_t ri = t / d, j = t - ri * d;
+            const uint32_t r = r0 + ri;
+            sr[ri * dp + j] = r < r_end ? (double)a.X[(size_t)r * ld + j] : 0.0;
+        }
+        __syncthreads();
+        double acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+        for (uint32_t kk = 0; kk < d; ++kk) {
+            double qv[4], rv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) qv[i] = sq[(ty + 16 * i) * dp + kk];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rv[j] = sr[(tx + 16 * j) * dp + kk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_fma(qv[i], rv[j], acc[i][j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t r = r0 + tx + 16 * j;
+            if (r >= r_end) continue;
+            const double rn = a.norms[r];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (!qok[i] || r == qid[i]) continue;
+                const double dd = qn[i] + rn - 2.0 * acc[i][j];
+                if (dd <= ub[i]) {
+                    const uint32_t q = q0 + ty + 16 * i;
+                    const uint32_t slot = atomicAdd(&a.count[q], 1u);
+                    if (slot < a.cap) {
+                        a.cand_d[(size_t)q * a.cap + slot] = dd;
+                        a.cand_id[(size_t)q * a.cap + slot] = r;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// One wave per query: the k smallest (dist, id) of its candidates by repeated wave minimum.
+__global__ void k_exact_select(ExactScanArgs a, uint32_t k, uint32_t* __restrict__ out_ids,
+                               double* __restrict__ out_d, uint32_t* __restrict__ overflow) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t q = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (q >= a.nq) return;
+    const uint32_t cnt = a.count[q];
+    if (cnt > a.cap && lane == 0) atomicAdd(overflow, 1u);
+    const uint32_t m = min(cnt, a.cap);
+    const double* cd = a.cand_d + (size_t)q * a.cap;
+    uint32_t* ci = a.cand_id + (size_t)q * a.cap;
+    double last_d = -1.0;
+    uint32_t last_i = 0;
+    bool have_last = false;
+    for (uint32_t out = 0; out < k; ++out) {
+        double bd = __builtin_inf();
+        uint32_t bi = kNoId;
+        for (uint32_t e = lane; e < m; e += 64) {
+            const double dv = cd[e];
+            const uint32_t iv = ci[e];
+            const bool after = !have_last || dv > last_d || (dv == last_d && iv > last_i);
+            if (after && (dv < bd || (dv == bd && iv < bi))) {
+                bd = dv;
+                bi = iv;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double od = __shfl_xor(bd, o);
+            const uint32_t oi = __shfl_xor(bi, o);
+            if (od < bd || (od == bd && oi < bi)) {
+                bd = od;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            out_ids[(size_t)q * k + out] = bi;
+            out_d[(size_t)q * k + out] = bd;
+        }
+        last_d = bd;
+        last_i = bi;
+        have_last = bi != kNoId;
+        if (!have_last) last_d = __builtin_inf();
+    }
+}
+
+// Upper bound per query from any list of candidate ids (FP64 distance of its k-th distinct valid
+// entry); +inf when the list has fewer than k distinct valid ids.
+__global__ void k_exact_ub(ExactScanArgs a, const uint32_t* __restrict__ lists, uint32_t k,
+                           double* __restrict__ ub) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.nq) return;
+    const uint32_t qid = a.queries[q];
+    const uint32_t* L = lists + (size_t)q * k;
+    const float* xq = a.X + (size_t)qid * a.ld;
+    double worst = 0.0;
+    uint32_t valid = 0;
+    for (uint32_t s = 0; s < k; ++s) {
+        const uint32_t r = L[s];
+        if (r >= a.n || r == qid) continue;
+        bool dup = false;
+        for (uint32_t t = 0; t < s; ++t) dup |= L[t] == r;
+        if (dup) continue;
+        const float* xr = a.X + (size_t)r * a.ld;
+        double dot = 0.0;
+        for (uint32_t j = 0; j < a.d; ++j) dot = __builtin_fma((double)xq[j], (double)xr[j], dot);
+        const double dd = a.norms[qid] + a.norms[r] - 2.0 * dot;
+        worst = fmax(worst, dd);
+        ++valid;
+    }
+    // slack of a few ulps covers the different summation order of the scan kernel
+    ub[q] = valid >= k ? worst * (1.0 + 1e-12) + 1e-300 : __builtin_inf();
+}
+
+void launch_norms64(const float* X, uint32_t ld, uint32_t d, uint32_t n, double* out,
+                    hipStream_t st) {
+    if (n) k_norms64<<<(n + 255) / 256, 256, 0, st>>>(X, ld, d, n, out);
+}
+
+size_t exact_scan_lds(uint32_t d) { return (size_t)(kExQ + kExR) * (d + 1) * sizeof(double); }
+
+void launch_exact_scan(const ExactScanArgs& a, uint32_t slices, hipStream_t st) {
+    if (!a.nq) return;
+    dim3 grid((a.nq + kExQ - 1) / kExQ, slices);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_exact_scan),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)exact_scan_lds(a.d));
+    k_exact_scan<<<grid, kExThreads, exact_scan_lds(a.d), st>>>(a);
+}
+
+void launch_exact_select(const ExactScanArgs& a, uint32_t k, uint32_t* out_ids, double* out_d,
+                         uint32_t* overflow, hipStream_t st) {
+    if (a.nq) k_exact_select<<<(a.nq + 3) / 4, 256, 0, st>>>(a, k, out_ids, out_d, overflow);
+}
+
+void launch_exact_ub(const ExactScanArgs& a, const uint32_t* lists, uint32_t k, double* ub,
+                     hipStream_t st) {
+    if (a.nq) k_exact_ub<<<(a.nq + 255) / 256, 256, 0, st>>>(a, lists, k, ub);
+}
+
+}  // namespace knng

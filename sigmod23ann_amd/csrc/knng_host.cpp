@@ -1,0 +1,879 @@
+// Host driver and C ABI of the gfx950 KNN-graph builder (constructResult,
+// SolutionKmeans.hpp:558-616, re-designed around device-resident state).
+//
+// A build keeps the points, the permutation, the split level tables and the per-point sorted
+// top-k lists on the device for its whole life. Per round the host walks the split tree one level
+// at a time: it uploads the level's node/chunk/tile tables, launches the level kernels, reads back
+// each node's lo size and status, and derives the next level (children below leaf_max become
+// leaves). Then one leaf launch covers every bucket of the round. Sample sizes are computed on the
+// host with the C library's log/pow (SolutionKmeans.hpp:50-58, :70, :105), so they match a CPU
+// run of the same formulas exactly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/knng.h"
+#include "knng_device.hpp"
+#include "knng_exact.hpp"
+#include "knng_kernels.hpp"
+
+namespace knng {
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define KNNG_HIP(expr)                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? KNNG_ERR_NOMEM : KNNG_ERR_HIP,           \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+#define KNNG_TRY(expr)          \
+    do {                        \
+        int r_ = (expr);        \
+        if (r_ != KNNG_OK) return r_; \
+    } while (0)
+
+// Grow-only device buffer.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int ensure(size_t count) {
+        if (count <= cap && p) return KNNG_OK;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        size_t c = std::max<size_t>(count, 1);
+        KNNG_HIP(hipMalloc(&p, c * sizeof(T)));
+        cap = c;
+        return KNNG_OK;
+    }
+};
+
+// Grow-only pinned host buffer.
+template <class T>
+struct PinBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    PinBuf() = default;
+    PinBuf(const PinBuf&) = delete;
+    PinBuf& operator=(const PinBuf&) = delete;
+    ~PinBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    int ensure(size_t count) {
+        if (count <= cap && p) return KNNG_OK;
+        size_t c = std::max<size_t>({count, 2 * cap, 64});
+        T* q = nullptr;
+        KNNG_HIP(hipHostMalloc(reinterpret_cast<void**>(&q), c * sizeof(T)));
+        if (p) (void)hipHostFree(p);
+        p = q;
+        cap = c;
+        return KNNG_OK;
+    }
+};
+
+// getSampleFromPercent size, SolutionKmeans.hpp:50-58: float percentage, float product, truncated.
+uint32_t sample_count(uint32_t n) {
+    float perc = (float)(std::pow(std::log((double)n) / std::log(30.0), 7) / (double)n);
+    return (uint32_t)((float)n * perc);
+}
+// kmeansStartVecs sample size, SolutionKmeans.hpp:70, :105.
+uint32_t start_count(uint32_t n) { return (uint32_t)std::pow(std::log10((double)n), 2.5); }
+
+struct HostNode {
+    uint32_t start, size;
+    uint64_t key;
+    uint32_t attempt;
+};
+
+bool valid_params(const knng_params& p, std::string& why) {
+    if (p.k < 1 || p.k > kMaxK) { why = "k must be in [1, 128]"; return false; }
+    if (p.leaf_max < 1 || p.leaf_max > kMaxLeaf) { why = "leaf_max must be in [1, 512]"; return false; }
+    if (p.leaf_max_odd > kMaxLeaf) { why = "leaf_max_odd must be <= 512"; return false; }
+    if (p.small_max > kMaxSmallNode) { why = "small_max must be <= 4096"; return false; }
+    if (p.precision > KNNG_PREC_BF16) { why = "unknown precision"; return false; }
+    return true;
+}
+
+}  // namespace
+
+struct Builder {
+    knng_params prm{};
+    hipStream_t st = nullptr;
+    hipEvent_t ev[8] = {};
+    knng_stats stats{};
+
+    // points (internal copy when the caller's layout is not usable directly)
+    DevBuf<float> xbuf;
+    const float* X = nullptr;
+    uint32_t n = 0, d = 0, ld = 0;
+
+    // split state
+    DevBuf<uint32_t> perm, perm_tmp, tile_lo, tile_base, pcnt;
+    DevBuf<uint8_t> flags;
+    DevBuf<NodeDesc> nodes;
+    DevBuf<ChunkDesc> chunks;
+    DevBuf<TileDesc> tiles;
+    DevBuf<float> c1, c2, coef, off;
+    DevBuf<double> part;
+    PinBuf<NodeDesc> h_nodes;
+    PinBuf<ChunkDesc> h_chunks;
+    PinBuf<TileDesc> h_tiles;
+    PinBuf<LeafDesc> h_leaves;
+
+    // top-k state
+    DevBuf<uint32_t> knn_ids, snap;
+    DevBuf<float> knn_d, bound;
+    DevBuf<LeafDesc> leaves;
+    DevBuf<uint64_t> counters;
+
+    // exact k-NN scratch
+    DevBuf<double> norms64, cand_d, ubuf;
+    DevBuf<uint32_t> cand_id, qcount, ovf;
+
+    // generator scratch
+    DevBuf<float> gen_a, gen_b, gen_c;
+
+    ~Builder() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    int init(const knng_params& p) {
+        std::string why;
+        if (!valid_params(p, why)) return fail(KNNG_ERR_INVALID, why);
+        prm = p;
+        KNNG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        for (auto& e : ev) KNNG_HIP(hipEventCreate(&e));
+        return KNNG_OK;
+    }
+
+    uint32_t leaf_max_for(uint32_t round) const {
+        return (round % 2 == 1 && prm.leaf_max_odd) ? prm.leaf_max_odd : prm.leaf_max;
+    }
+
+    // Make the points usable by the kernels: 16-byte aligned rows with a stride of 4k floats.
+    int bind_points(const float* pts, bool on_device, uint32_t nn, uint32_t dd, size_t stride) {
+        if (dd < 1 || dd > kMaxDim) return fail(KNNG_ERR_UNSUPPORTED, "d must be in [1, 256]");
+        if (stride < dd) return fail(KNNG_ERR_INVALID, "row_stride < d");
+        if (nn && !pts) return fail(KNNG_ERR_INVALID, "points is null");
+        n = nn;
+        d = dd;
+        const bool usable = on_device && stride % 4 == 0 && (reinterpret_cast<uintptr_t>(pts) & 15) == 0;
+        if (usable) {
+            X = pts;
+            ld = (uint32_t)stride;
+            return KNNG_OK;
+        }
+        ld = (dd + 3) & ~3u;
+        KNNG_TRY(xbuf.ensure((size_t)nn * ld));
+        if (nn) {
+            KNNG_HIP(hipMemsetAsync(xbuf.p, 0, (size_t)nn * ld * sizeof(float), st));
+            KNNG_HIP(hipMemcpy2DAsync(xbuf.p, ld * sizeof(float), pts, stride * sizeof(float),
+                                      dd * sizeof(float), nn,
+                                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+        }
+        X = xbuf.p;
+        return KNNG_OK;
+    }
+
+    SplitLevelArgs level_args(uint32_t nn, uint32_t nc, uint32_t nt, bool lloyd) {
+        SplitLevelArgs a{};
+        a.X = X;
+        a.ld = ld;
+        a.d = d;
+        a.perm = perm.p;
+        a.perm_tmp = perm_tmp.p;
+        a.nodes = nodes.p;
+        a.nnodes = nn;
+        a.chunks = chunks.p;
+        a.nchunks = nc;
+        a.tiles = tiles.p;
+        a.ntiles = nt;
+        a.c1 = c1.p;
+        a.c2 = c2.p;
+        a.coef = coef.p;
+        a.off = off.p;
+        a.part = part.p;
+        a.pcnt = pcnt.p;
+        a.flags = flags.p;
+        a.tile_lo = tile_lo.p;
+        a.tile_base = tile_base.p;
+        a.iterations = prm.iterations;
+        a.run_lloyd = lloyd;
+        return a;
+    }
+
+    // Upload the tables of `batch` nodes (already in h_nodes), run the level, read back.
+    int run_level(uint32_t nb, uint32_t nc, uint32_t nt, bool lloyd) {
+        KNNG_TRY(nodes.ensure(nb));
+        KNNG_TRY(chunks.ensure(nc));
+        KNNG_TRY(tiles.ensure(nt));
+        KNNG_TRY(c1.ensure((size_t)nb * ld));
+        KNNG_TRY(c2.ensure((size_t)nb * ld));
+        KNNG_TRY(coef.ensure((size_t)nb * ld));
+        KNNG_TRY(off.ensure(nb));
+        KNNG_TRY(part.ensure((size_t)std::max(nc, 1u) * 2 * ld));
+        KNNG_TRY(pcnt.ensure((size_t)std::max(nc, 1u) * 2));
+        KNNG_TRY(tile_lo.ensure(nt));
+        KNNG_TRY(tile_base.ensure((size_t)nt + 1));
+        KNNG_HIP(hipMemcpyAsync(nodes.p, h_nodes.p, nb * sizeof(NodeDesc), hipMemcpyHostToDevice, st));
+        if (nc) KNNG_HIP(hipMemcpyAsync(chunks.p, h_chunks.p, nc * sizeof(ChunkDesc), hipMemcpyHostToDevice, st));
+        KNNG_HIP(hipMemcpyAsync(tiles.p, h_tiles.p, nt * sizeof(TileDesc), hipMemcpyHostToDevice, st));
+        SplitLevelArgs a = level_args(nb, nc, nt, lloyd);
+        launch_split_prepare(a, st);
+        KNNG_HIP(hipEventRecord(ev[2], st));
+        launch_split_side(a, st);
+        KNNG_HIP(hipEventRecord(ev[3], st));
+        launch_split_partition(a, st);
+        KNNG_HIP(hipGetLastError());
+        KNNG_HIP(hipMemcpyAsync(h_nodes.p, nodes.p, nb * sizeof(NodeDesc), hipMemcpyDeviceToHost, st));
+        KNNG_HIP(hipStreamSynchronize(st));
+        float ms = 0.f;
+        KNNG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+        stats.side_ms += ms;
+        stats.side_launches++;
+        return KNNG_OK;
+    }
+
+    // Fill h_nodes/h_chunks/h_tiles for the given pending nodes.
+    int plan(const std::vector<HostNode>& lvl, const std::vector<uint32_t>& pending, uint32_t& nc,
+             uint32_t& nt) {
+        const uint32_t nb = (uint32_t)pending.size();
+        KNNG_TRY(h_nodes.ensure(nb));
+        nc = 0;
+        nt = 0;
+        for (uint32_t b = 0; b < nb; ++b) {
+            const HostNode& hn = lvl[pending[b]];
+            const bool large = hn.size >= prm.small_max;
+            const uint32_t m = large ? sample_count(hn.size) : 0;
+            uint32_t s = start_count(hn.size);
+            if (large) s = std::min(m, s);
+            s = std::max(s, 2u);
+            s = std::min(s, 1024u);
+            const uint32_t nch = large ? (m + kLargeChunk - 1) / kLargeChunk : 1;
+            const uint32_t ntl = (hn.size + kTileSize - 1) / kTileSize;
+            NodeDesc& nd = h_nodes.p[b];
+            nd.start = hn.start;
+            nd.size = hn.size;
+            nd.akey = attempt_key(hn.key, hn.attempt);
+            nd.m = m;
+            nd.s = s;
+            nd.chunk0 = nc;
+            nd.nchunks = nch;
+            nd.tile0 = nt;
+            nd.ntiles = ntl;
+            nd.status = kNodeActive;
+            nd.lo = 0;
+            nc += nch;
+            nt += ntl;
+        }
+        KNNG_TRY(h_chunks.ensure(nc));
+        KNNG_TRY(h_tiles.ensure(nt));
+        for (uint32_t b = 0; b < nb; ++b) {
+            const NodeDesc& nd = h_nodes.p[b];
+            for (uint32_t c = 0; c < nd.nchunks; ++c) {
+                ChunkDesc& ch = h_chunks.p[nd.chunk0 + c];
+                ch.node = b;
+                if (nd.m) {
+                    ch.begin = c * kLargeChunk;
+                    ch.count = std::min(kLargeChunk, nd.m - ch.begin);
+                } else {
+                    ch.begin = 0;
+                    ch.count = nd.size;
+                }
+            }
+            for (uint32_t t = 0; t < nd.ntiles; ++t) {
+                TileDesc& tl = h_tiles.p[nd.tile0 + t];
+                tl.node = b;
+                tl.begin = t * kTileSize;
+                tl.count = std::min(kTileSize, nd.size - tl.begin);
+            }
+        }
+        return KNNG_OK;
+    }
+
+    // One round of splitKmeansBinaryProcess from the identity permutation.
+    int split_round(uint32_t round, std::vector<LeafDesc>& out_leaves) {
+        KNNG_TRY(perm.ensure(n));
+        KNNG_TRY(perm_tmp.ensure(n));
+        KNNG_TRY(flags.ensure(n));
+        launch_iota(perm.p, n, st);
+        out_leaves.clear();
+        const uint32_t leaf_max = leaf_max_for(round);
+        if (n <= leaf_max) {
+            if (n) out_leaves.push_back({0, n});
+            return KNNG_OK;
+        }
+        std::vector<HostNode> level{{0, n, round_key(prm.seed, round), 0}}, next;
+        std::vector<uint32_t> lo, pending, again;
+        while (!level.empty()) {
+            lo.assign(level.size(), 0);
+            pending.resize(level.size());
+            for (uint32_t i = 0; i < level.size(); ++i) pending[i] = i;
+            while (!pending.empty()) {
+                uint32_t nc = 0, nt = 0;
+                KNNG_TRY(plan(level, pending, nc, nt));
+                KNNG_TRY(run_level((uint32_t)pending.size(), nc, nt, true));
+                again.clear();
+                for (uint32_t b = 0; b < pending.size(); ++b) {
+                    HostNode& hn = level[pending[b]];
+                    const NodeDesc& nd = h_nodes.p[b];
+                    stats.side_rows += hn.size;
+                    if (nd.status == kNodeActive) {
+                        lo[pending[b]] = nd.lo;
+                    } else {
+                        stats.split_failures++;
+                        if (++hn.attempt >= kMaxAttempts) {  // defined fallback: halve by position
+                            stats.split_fallbacks++;
+                            lo[pending[b]] = hn.size / 2;
+                        } else {
+                            again.push_back(pending[b]);
+                        }
+                    }
+                }
+                pending.swap(again);
+            }
+            next.clear();
+            for (uint32_t i = 0; i < level.size(); ++i) {
+                const HostNode& hn = level[i];
+                HostNode ch[2] = {{hn.start, lo[i], child_key(hn.key, 0), 0},
+                                  {hn.start + lo[i], hn.size - lo[i], child_key(hn.key, 1), 0}};
+                for (auto& c : ch) {
+                    if (c.size <= leaf_max) out_leaves.push_back({c.start, c.size});
+                    else next.push_back(c);
+                }
+            }
+            level.swap(next);
+        }
+        std::sort(out_leaves.begin(), out_leaves.end(),
+                  [](const LeafDesc& a, const LeafDesc& b) { return a.start < b.start; });
+        return KNNG_OK;
+    }
+
+    int ensure_state() {
+        KNNG_TRY(knn_ids.ensure((size_t)n * prm.k));
+        KNNG_TRY(knn_d.ensure((size_t)n * prm.k));
+        KNNG_TRY(bound.ensure(n));
+        KNNG_TRY(counters.ensure(4));
+        return KNNG_OK;
+    }
+
+    int leaf_stage(const std::vector<LeafDesc>& lv) {
+        if (lv.empty()) return KNNG_OK;
+        if (d > 128) return fail(KNNG_ERR_UNSUPPORTED, "leaf stage supports d <= 128");
+        uint32_t mx = 0;
+        uint64_t pairs = 0;
+        for (auto& l : lv) {
+            mx = std::max(mx, l.size);
+            pairs += (uint64_t)l.size * (l.size - 1) / 2;
+        }
+        const size_t lds = leaf_lds_bytes(mx, d, ld);
+        if (lds > 160 * 1024)
+            return fail(KNNG_ERR_UNSUPPORTED, "leaf bucket too large for LDS at this d");
+        KNNG_TRY(h_leaves.ensure(lv.size()));
+        std::memcpy(h_leaves.p, lv.data(), lv.size() * sizeof(LeafDesc));
+        KNNG_TRY(leaves.ensure(lv.size()));
+        KNNG_HIP(hipMemcpyAsync(leaves.p, h_leaves.p, lv.size() * sizeof(LeafDesc), hipMemcpyHostToDevice, st));
+        LeafArgs a{};
+        a.X = X;
+        a.ld = ld;
+        a.d = d;
+        a.k = prm.k;
+        a.perm = perm.p;
+        a.leaves = leaves.p;
+        a.nleaves = (uint32_t)lv.size();
+        a.knn_ids = knn_ids.p;
+        a.knn_d = knn_d.p;
+        a.bound = bound.p;
+        a.screen = prm.precision == KNNG_PREC_FP32 ? kScreenExact : kScreenApprox;
+        a.rerank_cap = prm.rerank;
+        a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
+        a.max_leaf = mx;
+        a.counters = counters.p;
+        KNNG_HIP(hipEventRecord(ev[4], st));
+        launch_leaf(a, st);
+        KNNG_HIP(hipGetLastError());
+        KNNG_HIP(hipEventRecord(ev[5], st));
+        KNNG_HIP(hipEventSynchronize(ev[5]));
+        float ms = 0.f;
+        KNNG_HIP(hipEventElapsedTime(&ms, ev[4], ev[5]));
+        stats.leaf_ms += ms;
+        stats.leaf_launches++;
+        stats.leaves += lv.size();
+        stats.leaf_pairs += pairs;
+        return KNNG_OK;
+    }
+
+    int non_pass() {
+        KNNG_TRY(snap.ensure((size_t)n * prm.k));
+        launch_non_snapshot(knn_ids.p, snap.p, n, prm.k, st);
+        NonArgs a{};
+        a.X = X;
+        a.ld = ld;
+        a.d = d;
+        a.k = prm.k;
+        a.n = n;
+        a.snap = snap.p;
+        a.knn_ids = knn_ids.p;
+        a.knn_d = knn_d.p;
+        a.bound = bound.p;
+        launch_non_direct(a, st);
+        KNNG_HIP(hipGetLastError());
+        return KNNG_OK;
+    }
+
+    int build(uint32_t* out_ids, float* out_d) {
+        stats = knng_stats{};
+        KNNG_TRY(ensure_state());
+        KNNG_HIP(hipMemsetAsync(counters.p, 0, 4 * sizeof(uint64_t), st));
+        KNNG_HIP(hipEventRecord(ev[0], st));
+        launch_init_knn(knn_ids.p, knn_d.p, bound.p, n, prm.k, st);
+        std::vector<LeafDesc> lv;
+        for (uint32_t r = 0; r < prm.rounds; ++r) {
+            KNNG_HIP(hipEventRecord(ev[6], st));
+            KNNG_TRY(split_round(r, lv));
+            KNNG_HIP(hipEventRecord(ev[7], st));
+            KNNG_HIP(hipEventSynchronize(ev[7]));
+            float ms = 0.f;
+            KNNG_HIP(hipEventElapsedTime(&ms, ev[6], ev[7]));
+            stats.split_ms += ms;
+            KNNG_TRY(leaf_stage(lv));
+        }
+        KNNG_HIP(hipEventRecord(ev[6], st));
+        for (uint32_t p = 0; p < prm.non_passes; ++p) KNNG_TRY(non_pass());
+        KNNG_HIP(hipEventRecord(ev[7], st));
+        launch_finalize(knn_ids.p, knn_d.p, n, prm.k, out_ids, out_d, st);
+        KNNG_HIP(hipGetLastError());
+        KNNG_HIP(hipEventRecord(ev[1], st));
+        KNNG_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        KNNG_HIP(hipEventElapsedTime(&ms, ev[6], ev[7]));
+        stats.non_ms = ms;
+        KNNG_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        stats.total_ms = ms;
+        uint64_t c[4];
+        KNNG_HIP(hipMemcpy(c, counters.p, sizeof(c), hipMemcpyDeviceToHost));
+        stats.survivors = c[0];
+        stats.inserts = c[2];
+        return KNNG_OK;
+    }
+};
+
+}  // namespace knng
+
+using knng::Builder;
+using knng::fail;
+
+struct knng_handle {
+    Builder b;
+};
+
+namespace {
+
+int check_handle(const knng_handle* h) {
+    return h ? KNNG_OK : fail(KNNG_ERR_INVALID, "null handle");
+}
+
+// Host state (n*k) <-> device state.
+int upload_state(Builder& b, const uint32_t* ids, const float* dists, bool empty) {
+    const size_t tot = (size_t)b.n * b.prm.k;
+    KNNG_TRY(b.ensure_state());
+    if (empty) {
+        knng::launch_init_knn(b.knn_ids.p, b.knn_d.p, b.bound.p, b.n, b.prm.k, b.st);
+        return KNNG_OK;
+    }
+    KNNG_HIP(hipMemcpyAsync(b.knn_ids.p, ids, tot * 4, hipMemcpyHostToDevice, b.st));
+    KNNG_HIP(hipMemcpyAsync(b.knn_d.p, dists, tot * 4, hipMemcpyHostToDevice, b.st));
+    // bound = distance in slot k-1
+    std::vector<float> bnd(b.n);
+    for (uint32_t p = 0; p < b.n; ++p) bnd[p] = dists[(size_t)p * b.prm.k + b.prm.k - 1];
+    KNNG_HIP(hipMemcpyAsync(b.bound.p, bnd.data(), b.n * 4, hipMemcpyHostToDevice, b.st));
+    KNNG_HIP(hipStreamSynchronize(b.st));
+    return KNNG_OK;
+}
+
+int download_state(Builder& b, uint32_t* ids, float* dists) {
+    const size_t tot = (size_t)b.n * b.prm.k;
+    KNNG_HIP(hipMemcpyAsync(ids, b.knn_ids.p, tot * 4, hipMemcpyDeviceToHost, b.st));
+    KNNG_HIP(hipMemcpyAsync(dists, b.knn_d.p, tot * 4, hipMemcpyDeviceToHost, b.st));
+    KNNG_HIP(hipStreamSynchronize(b.st));
+    return KNNG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* knng_last_error(void) { return knng::g_err.c_str(); }
+
+void knng_default_params(knng_params* p) {
+    if (!p) return;
+    *p = knng_params{};
+    p->k = 100;
+    p->leaf_max = 400;
+    p->leaf_max_odd = 0;
+    p->small_max = 3000;
+    p->iterations = 1;
+    p->rounds = 8;
+    p->non_passes = 1;
+    p->precision = KNNG_PREC_FP32;
+    p->rerank = 160;
+    p->seed = 12345;
+}
+
+int knng_create(const knng_params* params, knng_handle** out) {
+    if (!params || !out) return fail(KNNG_ERR_INVALID, "null argument");
+    *out = nullptr;
+    auto* h = new knng_handle();
+    int r = h->b.init(*params);
+    if (r != KNNG_OK) {
+        delete h;
+        return r;
+    }
+    *out = h;
+    return KNNG_OK;
+}
+
+void knng_destroy(knng_handle* h) { delete h; }
+
+int knng_set_params(knng_handle* h, const knng_params* params) {
+    KNNG_TRY(check_handle(h));
+    if (!params) return fail(KNNG_ERR_INVALID, "null params");
+    std::string why;
+    if (!knng::valid_params(*params, why)) return fail(KNNG_ERR_INVALID, why);
+    h->b.prm = *params;
+    return KNNG_OK;
+}
+
+int knng_get_stats(const knng_handle* h, knng_stats* out) {
+    KNNG_TRY(check_handle(h));
+    if (!out) return fail(KNNG_ERR_INVALID, "null stats");
+    *out = h->b.stats;
+    return KNNG_OK;
+}
+
+int knng_build_device(knng_handle* h, const float* d_points, uint32_t n, uint32_t d,
+                      size_t row_stride, uint32_t* d_out_ids, float* d_out_dists) {
+    KNNG_TRY(check_handle(h));
+    if (n && !d_out_ids) return fail(KNNG_ERR_INVALID, "null output");
+    Builder& b = h->b;
+    KNNG_TRY(b.bind_points(d_points, true, n, d, row_stride));
+    return b.build(d_out_ids, d_out_dists);
+}
+
+int knng_build(knng_handle* h, const float* points, uint32_t n, uint32_t d, size_t row_stride,
+               uint32_t* out_ids, float* out_dists) {
+    KNNG_TRY(check_handle(h));
+    if (n && !out_ids) return fail(KNNG_ERR_INVALID, "null output");
+    Builder& b = h->b;
+    KNNG_TRY(b.bind_points(points, false, n, d, row_stride));
+    const size_t tot = (size_t)n * b.prm.k;
+    knng::DevBuf<uint32_t> oid;
+    knng::DevBuf<float> od;
+    KNNG_TRY(oid.ensure(tot));
+    if (out_dists) KNNG_TRY(od.ensure(tot));
+    KNNG_TRY(b.build(oid.p, out_dists ? od.p : nullptr));
+    KNNG_HIP(hipMemcpy(out_ids, oid.p, tot * 4, hipMemcpyDeviceToHost));
+    if (out_dists) KNNG_HIP(hipMemcpy(out_dists, od.p, tot * 4, hipMemcpyDeviceToHost));
+    return KNNG_OK;
+}
+
+int knng_split_round(knng_handle* h, const float* points, uint32_t n, uint32_t d,
+                     size_t row_stride, uint32_t round, uint32_t* perm_out, uint32_t* leaf_start,
+                     uint32_t* leaf_size, uint32_t* n_leaves) {
+    KNNG_TRY(check_handle(h));
+    if (!perm_out || !leaf_start || !leaf_size || !n_leaves) return fail(KNNG_ERR_INVALID, "null output");
+    Builder& b = h->b;
+    KNNG_TRY(b.bind_points(points, false, n, d, row_stride));
+    std::vector<knng::LeafDesc> lv;
+    KNNG_TRY(b.split_round(round, lv));
+    if (n) KNNG_HIP(hipMemcpyAsync(perm_out, b.perm.p, (size_t)n * 4, hipMemcpyDeviceToHost, b.st));
+    KNNG_HIP(hipStreamSynchronize(b.st));
+    for (size_t i = 0; i < lv.size(); ++i) {
+        leaf_start[i] = lv[i].start;
+        leaf_size[i] = lv[i].size;
+    }
+    *n_leaves = (uint32_t)lv.size();
+    return KNNG_OK;
+}
+
+int knng_partition_injected(knng_handle* h, const float* points, uint32_t n, uint32_t d,
+                            size_t row_stride, uint32_t* perm, uint32_t n_nodes,
+                            const uint32_t* node_start, const uint32_t* node_size,
+                            const float* coefs, const float* offsets, uint32_t* lo_out) {
+    KNNG_TRY(check_handle(h));
+    if (!perm || !node_start || !node_size || !coefs || !offsets || !lo_out)
+        return fail(KNNG_ERR_INVALID, "null argument");
+    Builder& b = h->b;
+    KNNG_TRY(b.bind_points(points, false, n, d, row_stride));
+    for (uint32_t i = 0; i < n_nodes; ++i)
+        if ((uint64_t)node_start[i] + node_size[i] > n || node_size[i] == 0)
+            return fail(KNNG_ERR_INVALID, "node range outside [0, n)");
+    KNNG_TRY(b.perm.ensure(n));
+    KNNG_TRY(b.perm_tmp.ensure(n));
+    KNNG_TRY(b.flags.ensure(n));
+    KNNG_HIP(hipMemcpyAsync(b.perm.p, perm, (size_t)n * 4, hipMemcpyHostToDevice, b.st));
+    KNNG_TRY(b.h_nodes.ensure(n_nodes));
+    uint32_t nt = 0;
+    for (uint32_t i = 0; i < n_nodes; ++i) {
+        knng::NodeDesc& nd = b.h_nodes.p[i];
+        nd = knng::NodeDesc{};
+        nd.start = node_start[i];
+        nd.size = node_size[i];
+        nd.tile0 = nt;
+        nd.ntiles = (nd.size + knng::kTileSize - 1) / knng::kTileSize;
+        nd.status = knng::kNodeInjected;
+        nt += nd.ntiles;
+    }
+    KNNG_TRY(b.h_tiles.ensure(nt));
+    for (uint32_t i = 0; i < n_nodes; ++i) {
+        const knng::NodeDesc& nd = b.h_nodes.p[i];
+        for (uint32_t t = 0; t < nd.ntiles; ++t) {
+            knng::TileDesc& tl = b.h_tiles.p[nd.tile0 + t];
+            tl.node = i;
+            tl.begin = t * knng::kTileSize;
+            tl.count = std::min(knng::kTileSize, nd.size - tl.begin);
+        }
+    }
+    KNNG_TRY(b.coef.ensure((size_t)n_nodes * b.ld));
+    KNNG_TRY(b.off.ensure(n_nodes));
+    std::vector<float> cf((size_t)n_nodes * b.ld, 0.f);
+    for (uint32_t i = 0; i < n_nodes; ++i)
+        std::memcpy(&cf[(size_t)i * b.ld], coefs + (size_t)i * d, d * sizeof(float));
+    KNNG_HIP(hipMemcpyAsync(b.coef.p, cf.data(), cf.size() * 4, hipMemcpyHostToDevice, b.st));
+    KNNG_HIP(hipMemcpyAsync(b.off.p, offsets, n_nodes * 4, hipMemcpyHostToDevice, b.st));
+    KNNG_TRY(b.run_level(n_nodes, 0, nt, false));
+    for (uint32_t i = 0; i < n_nodes; ++i) lo_out[i] = b.h_nodes.p[i].lo;
+    KNNG_HIP(hipMemcpy(perm, b.perm.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return KNNG_OK;
+}
+
+int knng_leaf_stage(knng_handle* h, const float* points, uint32_t n, uint32_t d,
+                    size_t row_stride, const uint32_t* perm, const uint32_t* leaf_start,
+                    const uint32_t* leaf_size, uint32_t n_leaves, uint32_t* ids, float* dists,
+                    int start_empty) {
+    KNNG_TRY(check_handle(h));
+    if (!perm || !ids || !dists || (n_leaves && (!leaf_start || !leaf_size)))
+        return fail(KNNG_ERR_INVALID, "null argument");
+    Builder& b = h->b;
+    KNNG_TRY(b.bind_points(points, false, n, d, row_stride));
+    std::vector<knng::LeafDesc> lv(n_leaves);
+    for (uint32_t i = 0; i < n_leaves; ++i) {
+        if ((uint64_t)leaf_start[i] + leaf_size[i] > n) return fail(KNNG_ERR_INVALID, "leaf outside [0, n)");
+        if (leaf_size[i] > knng::kMaxLeaf) return fail(KNNG_ERR_UNSUPPORTED, "leaf larger than 512");
+        lv[i] = {leaf_start[i], leaf_size[i]};
+    }
+    KNNG_TRY(b.perm.ensure(n));
+    KNNG_HIP(hipMemcpyAsync(b.perm.p, perm, (size_t)n * 4, hipMemcpyHostToDevice, b.st));
+    KNNG_TRY(upload_state(b, ids, dists, start_empty != 0));
+    KNNG_HIP(hipMemsetAsync(b.counters.p, 0, 4 * sizeof(uint64_t), b.st));
+    KNNG_TRY(b.leaf_stage(lv));
+    return download_state(b, ids, dists);
+}
+
+int knng_non(knng_handle* h, const float* points, uint32_t n, uint32_t d, size_t row_stride,
+             uint32_t* ids, float* dists) {
+    KNNG_TRY(check_handle(h));
+    if (!ids || !dists) return fail(KNNG_ERR_INVALID, "null argument");
+    Builder& b = h->b;
+    KNNG_TRY(b.bind_points(points, false, n, d, row_stride));
+    KNNG_TRY(upload_state(b, ids, dists, false));
+    KNNG_TRY(b.non_pass());
+    return download_state(b, ids, dists);
+}
+
+int knng_exact_upper_from_lists(knng_handle* h, const float* d_points, uint32_t n, uint32_t d,
+                                size_t row_stride, const uint32_t* d_queries, uint32_t nq,
+                                const uint32_t* d_lists, uint32_t k, double* d_upper) {
+    KNNG_TRY(check_handle(h));
+    Builder& b = h->b;
+    KNNG_TRY(b.bind_points(d_points, true, n, d, row_stride));
+    KNNG_TRY(b.norms64.ensure(n));
+    knng::launch_norms64(b.X, b.ld, b.d, n, b.norms64.p, b.st);
+    knng::ExactScanArgs a{};
+    a.X = b.X;
+    a.ld = b.ld;
+    a.d = b.d;
+    a.n = n;
+    a.norms = b.norms64.p;
+    a.queries = d_queries;
+    a.nq = nq;
+    knng::launch_exact_ub(a, d_lists, k, d_upper, b.st);
+    KNNG_HIP(hipGetLastError());
+    KNNG_HIP(hipStreamSynchronize(b.st));
+    return KNNG_OK;
+}
+
+int knng_exact_knn_device(knng_handle* h, const float* d_points, uint32_t n, uint32_t d,
+                          size_t row_stride, const uint32_t* d_queries, uint32_t nq, uint32_t k,
+                          const double* d_upper, uint32_t cap, uint32_t* d_out_ids,
+                          double* d_out_dists) {
+    KNNG_TRY(check_handle(h));
+    if (!d_queries || !d_out_ids || !d_out_dists) return fail(KNNG_ERR_INVALID, "null argument");
+    if (k < 1 || cap < k) return fail(KNNG_ERR_INVALID, "need 1 <= k <= cap");
+    Builder& b = h->b;
+    KNNG_TRY(b.bind_points(d_points, true, n, d, row_stride));
+    if (knng::exact_scan_lds(d) > 160 * 1024) return fail(KNNG_ERR_UNSUPPORTED, "d too large");
+    KNNG_TRY(b.norms64.ensure(n));
+    KNNG_TRY(b.cand_d.ensure((size_t)nq * cap));
+    KNNG_TRY(b.cand_id.ensure((size_t)nq * cap));
+    KNNG_TRY(b.qcount.ensure(nq));
+    KNNG_TRY(b.ovf.ensure(1));
+    const double* ub = d_upper;
+    if (!ub) {
+        KNNG_TRY(b.ubuf.ensure(nq));
+        std::vector<double> inf(nq, std::numeric_limits<double>::infinity());
+        KNNG_HIP(hipMemcpyAsync(b.ubuf.p, inf.data(), nq * 8, hipMemcpyHostToDevice, b.st));
+        ub = b.ubuf.p;
+    }
+    knng::launch_norms64(b.X, b.ld, b.d, n, b.norms64.p, b.st);
+    KNNG_HIP(hipMemsetAsync(b.qcount.p, 0, (size_t)nq * 4, b.st));
+    KNNG_HIP(hipMemsetAsync(b.ovf.p, 0, 4, b.st));
+    knng::ExactScanArgs a{};
+    a.X = b.X;
+    a.ld = b.ld;
+    a.d = b.d;
+    a.n = n;
+    a.norms = b.norms64.p;
+    a.queries = d_queries;
+    a.nq = nq;
+    a.ub = ub;
+    a.cap = cap;
+    a.count = b.qcount.p;
+    a.cand_d = b.cand_d.p;
+    a.cand_id = b.cand_id.p;
+    const uint32_t qblocks = (nq + 63) / 64;
+    uint32_t slices = std::max(1u, std::min<uint32_t>(2048u / std::max(qblocks, 1u) + 1, (n + 4095) / 4096));
+    knng::launch_exact_scan(a, slices, b.st);
+    knng::launch_exact_select(a, k, d_out_ids, d_out_dists, b.ovf.p, b.st);
+    KNNG_HIP(hipGetLastError());
+    uint32_t ovf = 0;
+    KNNG_HIP(hipMemcpyAsync(&ovf, b.ovf.p, 4, hipMemcpyDeviceToHost, b.st));
+    KNNG_HIP(hipStreamSynchronize(b.st));
+    if (ovf) return fail(KNNG_ERR_OVERFLOW, std::to_string(ovf) + " queries overflowed their candidate buffer");
+    return KNNG_OK;
+}
+
+double knng_recall(const uint32_t* result_ids, const uint32_t* truth_ids, uint32_t rows,
+                   uint32_t k) {
+    if (!rows || !k || !result_ids || !truth_ids) return 0.0;
+    uint64_t hits = 0;
+    std::vector<uint32_t> r(k);
+    for (uint32_t i = 0; i < rows; ++i) {
+        std::memcpy(r.data(), result_ids + (size_t)i * k, k * 4);
+        std::sort(r.begin(), r.end());
+        for (uint32_t j = 0; j < k; ++j)
+            hits += std::binary_search(r.begin(), r.end(), truth_ids[(size_t)i * k + j]);
+    }
+    return (double)hits / ((double)rows * k);
+}
+
+int knng_generate_device(knng_handle* h, int kind, uint32_t n, uint32_t d, size_t row_stride,
+                         uint64_t seed, float* d_points) {
+    KNNG_TRY(check_handle(h));
+    if (!d_points || d < 1 || d > 256 || row_stride < d) return fail(KNNG_ERR_INVALID, "bad shape");
+    Builder& b = h->b;
+    // host-side parameters from a splitmix64 stream of the seed
+    uint64_t state = knng::mix64(seed ^ 0xA0761D6478BD642Full);
+    auto next_u = [&]() {
+        state += 0x9E3779B97F4A7C15ull;
+        return (double)(knng::mix64(state) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto next_n = [&]() {
+        const double u1 = std::max(next_u(), 1e-300), u2 = next_u();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    if (kind == KNNG_GEN_LOWRANK) {
+        const uint32_t latent = 24;
+        std::vector<float> W((size_t)latent * d);
+        for (auto& w : W) w = (float)next_n();
+        KNNG_TRY(b.gen_a.ensure(W.size()));
+        KNNG_HIP(hipMemcpyAsync(b.gen_a.p, W.data(), W.size() * 4, hipMemcpyHostToDevice, b.st));
+        knng::launch_gen_lowrank(d_points, n, d, (uint32_t)row_stride, latent, 0.05f, seed, b.gen_a.p, b.st);
+    } else if (kind == KNNG_GEN_MIXTURE || kind == KNNG_GEN_ZIPF) {
+        const bool zipf = kind == KNNG_GEN_ZIPF;
+        const uint32_t nc = zipf ? 20000 : 500;
+        std::vector<float> C((size_t)nc * d), cdf(nc), sg(nc);
+        for (auto& c : C) c = (float)(-10.0 + 20.0 * next_u());
+        double tot = 0.0;
+        std::vector<double> w(nc);
+        for (uint32_t c = 0; c < nc; ++c) {
+            w[c] = zipf ? 1.0 / std::pow((double)(c + 1), 1.1) : 1.0;
+            tot += w[c];
+            sg[c] = zipf ? (float)(0.3 + 1.7 * next_u()) : 1.0f;
+        }
+        double run = 0.0;
+        for (uint32_t c = 0; c < nc; ++c) {
+            run += w[c];
+            cdf[c] = (float)(run / tot);
+        }
+        cdf[nc - 1] = 1.0f;
+        KNNG_TRY(b.gen_a.ensure(C.size()));
+        KNNG_TRY(b.gen_b.ensure(nc));
+        KNNG_TRY(b.gen_c.ensure(nc));
+        KNNG_HIP(hipMemcpyAsync(b.gen_a.p, C.data(), C.size() * 4, hipMemcpyHostToDevice, b.st));
+        KNNG_HIP(hipMemcpyAsync(b.gen_b.p, cdf.data(), nc * 4, hipMemcpyHostToDevice, b.st));
+        KNNG_HIP(hipMemcpyAsync(b.gen_c.p, sg.data(), nc * 4, hipMemcpyHostToDevice, b.st));
+        knng::launch_gen_mixture(d_points, n, d, (uint32_t)row_stride, b.gen_a.p, nc, b.gen_b.p,
+                                 b.gen_c.p, seed, b.st);
+    } else {
+        return fail(KNNG_ERR_INVALID, "unknown generator kind");
+    }
+    KNNG_HIP(hipGetLastError());
+    KNNG_HIP(hipStreamSynchronize(b.st));
+    return KNNG_OK;
+}
+
+int knng_device_alloc(size_t bytes, void** out) {
+    if (!out) return fail(KNNG_ERR_INVALID, "null out");
+    KNNG_HIP(hipMalloc(out, std::max<size_t>(bytes, 1)));
+    return KNNG_OK;
+}
+int knng_device_free(void* p) {
+    KNNG_HIP(hipFree(p));
+    return KNNG_OK;
+}
+int knng_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    KNNG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return KNNG_OK;
+}
+int knng_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    KNNG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return KNNG_OK;
+}
+int knng_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+    KNNG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return KNNG_OK;
+}
+int knng_synchronize(void) {
+    KNNG_HIP(hipDeviceSynchronize());
+    return KNNG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// Descriptors and launchers shared between the host driver and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace knng {
+
+constexpr uint32_t kTileSize = 1024;       // partition positions per block
+constexpr uint32_t kMaxSmallNode = 4096;   // small-branch nodes are summed by one block
+constexpr uint32_t kMaxLeaf = 512;         // largest leaf bucket handled by the LDS leaf kernel
+constexpr uint32_t kMaxK = 128;            // neighbours per point
+
+enum : uint32_t { kNodeActive = 0, kNodeFailed = 1, kNodeInjected = 2 };
+
+// One node of the current split level (a contiguous range of the permutation).
+struct NodeDesc {
+    uint32_t start, size;
+    uint64_t akey;      // attempt key of this node's random draws
+    uint32_t m;         // sample draws for the Lloyd passes (large branch), 0 for the small branch
+    uint32_t s;         // rows sampled for the farthest-pair start
+    uint32_t chunk0, nchunks;
+    uint32_t tile0, ntiles;
+    uint32_t status;    // kNode*; the device marks failed attempts
+    uint32_t lo;        // out: size of the lo child
+};
+static_assert(sizeof(NodeDesc) == 48, "NodeDesc layout");
+
+struct ChunkDesc {
+    uint32_t node, begin, count;
+};
+struct TileDesc {
+    uint32_t node, begin, count;
+};
+
+struct SplitLevelArgs {
+    const float* X;
+    uint32_t ld, d;
+    uint32_t* perm;
+    uint32_t* perm_tmp;
+    NodeDesc* nodes;
+    uint32_t nnodes;
+    const ChunkDesc* chunks;
+    uint32_t nchunks;
+    const TileDesc* tiles;
+    uint32_t ntiles;
+    float *c1, *c2, *coef, *off;
+    double* part;
+    uint32_t* pcnt;
+    uint8_t* flags;
+    uint32_t* tile_lo;
+    uint32_t* tile_base;
+    uint32_t iterations;
+    bool run_lloyd;     // false for injected hyperplanes
+};
+
+void launch_iota(uint32_t* perm, uint32_t n, hipStream_t st);
+void launch_split_prepare(const SplitLevelArgs& a, hipStream_t st);    // start pair + Lloyd
+void launch_split_side(const SplitLevelArgs& a, hipStream_t st);       // full-range side test
+void launch_split_partition(const SplitLevelArgs& a, hipStream_t st);  // stable scatter
+size_t leaf_lds_bytes(uint32_t max_leaf, uint32_t d, uint32_t ld);
+
+// ---- leaf stage -------------------------------------------------------------------------------
+struct LeafDesc {
+    uint32_t start, size;
+};
+
+enum : uint32_t { kScreenExact = 0, kScreenApprox = 1 };
+
+struct LeafArgs {
+    const float* X;
+    uint32_t ld, d, k;
+    const uint32_t* perm;
+    const LeafDesc* leaves;
+    uint32_t nleaves;
+    uint32_t* knn_ids;     // n*k, each row sorted by (dist, id), padded (FLT_MAX, kNoId)
+    float* knn_d;
+    float* bound;          // n: knn_d[row][k-1]
+    uint32_t screen;       // kScreenExact: rigorous margin; kScreenApprox: plain approximate filter
+    uint32_t rerank_cap;   // kScreenApprox: at most this many candidates per point re-ranked
+    uint32_t half_kind;    // 0 = fp16, 1 = bf16 screening Gram
+    uint32_t max_leaf;     // largest leaf size in this launch
+    uint64_t* counters;    // [0] survivors screened, [1] exact distances, [2] inserts (nullable)
+};
+void launch_leaf(const LeafArgs& a, hipStream_t st);
+
+void launch_init_knn(uint32_t* ids, float* dists, float* bound, uint32_t n, uint32_t k,
+                     hipStream_t st);
+// ids -> out (padding id 1 for empty slots), dists -> out_d (nullable).
+void launch_finalize(const uint32_t* ids, const float* dists, uint32_t n, uint32_t k,
+                     uint32_t* out_ids, float* out_d, hipStream_t st);
+
+// ---- neighbours of neighbours ------------------------------------------------------------------
+struct NonArgs {
+    const float* X;
+    uint32_t ld, d, k, n;
+    const uint32_t* snap;  // n*k snapshot ids (unfilled slots read as id 0)
+    uint32_t* knn_ids;
+    float* knn_d;
+    float* bound;
+};
+void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
+                         hipStream_t st);
+void launch_non_direct(const NonArgs& a, hipStream_t st);
+
+// ---- synthetic data -------------------------------------------------------------------------------
+void launch_gen_lowrank(float* X, uint32_t n, uint32_t d, uint32_t ld, uint32_t latent,
+                        float noise, uint64_t seed, const float* W, hipStream_t st);
+void launch_gen_mixture(float* X, uint32_t n, uint32_t d, uint32_t ld, const float* centres,
+                        uint32_t ncentres, const float* cdf, const float* sigma, uint64_t seed,
+                        hipStream_t st);
+
+}  // namespace knng

@@ -1,0 +1,321 @@
+// Leaf stage for gfx950: all pairs inside each leaf bucket (addCandidatesLessThan,
+// KnnSets.hpp:131-160) merged into each point's top-k set (KnnSetScannableSimd, :26-127).
+//
+// One workgroup (8 waves) per bucket, rows resident in LDS:
+//  phase 0  gather the bucket's rows, centre them on the bucket's first row, scale each row by a
+//           power of two into fp16 (or bf16) range; FP32 squared norms of the centred rows.
+//  phase 1  screening Gram on MFMA (16x16x16, fp32 accumulate), one 16-row strip per wave over all
+//           columns: approximate distance n_i + n_j - 2 g_ij against the point's current bound.
+//           Exact mode adds a rigorous error margin, so every pair whose exact FP32 distance beats
+//           the bound survives; survivors go to a per-row bitmask in LDS.
+//  phase 2  one wave per row: exact reference-order FP32 distance for each survivor, then a
+//           wave-wide insert into the row's sorted (dist, id) list with id dedupe.
+//
+// The sorted list keeps the k smallest distinct ids offered (the reference's set semantics); ties
+// at the k-th distance resolve by id instead of by arrival.
+#include "knng_device.hpp"
+#include "knng_kernels.hpp"
+
+namespace knng {
+
+constexpr uint32_t kLeafThreads = 512;
+constexpr uint32_t kLeafWaves = kLeafThreads / 64;
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// Row stride of the 16-bit rows in LDS (elements); +8 staggers banks between rows.
+__host__ __device__ inline uint32_t leaf_row_stride(uint32_t d) { return ((d + 15) & ~15u) + 8; }
+
+struct LeafLds {
+    uint32_t b16, kp, rs, w16;
+    uint32_t off_ids, off_bnd, off_nrm, off_rinv, off_mask, off_rows, off_rowbuf, off_cand, total;
+};
+
+__host__ __device__ inline LeafLds leaf_lds_layout(uint32_t max_leaf, uint32_t d, uint32_t ld) {
+    LeafLds L;
+    L.b16 = (max_leaf + 15) & ~15u;
+    L.kp = (d + 15) & ~15u;
+    L.rs = leaf_row_stride(d);
+    L.w16 = L.b16 / 16;
+    uint32_t o = 0;
+    L.off_ids = o;    o += L.b16 * 4;
+    L.off_bnd = o;    o += L.b16 * 4;
+    L.off_nrm = o;    o += L.b16 * 4;
+    L.off_rinv = o;   o += L.b16 * 4;
+    L.off_mask = o;   o += L.b16 * L.w16 * 2;
+    o = (o + 15) & ~15u;
+    L.off_rows = o;   o += L.b16 * L.rs * 2;
+    o = (o + 15) & ~15u;
+    L.off_rowbuf = o; o += kLeafWaves * ld * 4;
+    L.off_cand = o;   o += kLeafWaves * L.b16 * 2;
+    L.total = (o + 15) & ~15u;
+    return L;
+}
+
+__device__ inline uint16_t to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ inline float wave_sum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline uint32_t wave_excl_scan(uint32_t v, uint32_t lane) {
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x - v;
+}
+
+// Margin coefficient c of the exact screen: |approx - exact| <= c * (n_i + n_j).
+// fp16: 2u + u^2 + accumulation (K<=128) + norm and final rounding, u = 2^-11 -> 9.9e-4; bf16:
+// u = 2^-8 -> 7.9e-3. 25% slack on top.
+template <uint32_t HALF>
+struct HalfTraits;
+template <>
+struct HalfTraits<0> {
+    static constexpr float kMargin = 1.25e-3f;
+};
+template <>
+struct HalfTraits<1> {
+    static constexpr float kMargin = 1.0e-2f;
+};
+
+template <uint32_t HALF>
+__global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const LeafDesc lf = a.leaves[blockIdx.x];
+    const uint32_t b = lf.size;
+    if (b < 2) return;
+    const uint32_t d = a.d, ld = a.ld, k = a.k;
+    const LeafLds L = leaf_lds_layout(a.max_leaf, d, ld);
+    const uint32_t b16 = (b + 15) & ~15u;
+    const uint32_t nstrip = b16 / 16;
+    uint32_t* s_ids = reinterpret_cast<uint32_t*>(lds + L.off_ids);
+    float* s_bnd = reinterpret_cast<float*>(lds + L.off_bnd);
+    float* s_nrm = reinterpret_cast<float*>(lds + L.off_nrm);
+    float* s_rinv = reinterpret_cast<float*>(lds + L.off_rinv);
+    uint16_t* s_mask = reinterpret_cast<uint16_t*>(lds + L.off_mask);
+    uint16_t* s_rows = reinterpret_cast<uint16_t*>(lds + L.off_rows);
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float* s_rowbuf = reinterpret_cast<float*>(lds + L.off_rowbuf) + wave * ld;
+    uint16_t* s_cand = reinterpret_cast<uint16_t*>(lds + L.off_cand) + wave * L.b16;
+
+    // ---- phase 0: gather, centre, scale ---------------------------------------------------------
+    for (uint32_t t = tid; t < b16; t += kLeafThreads) {
+        const uint32_t id = t < b ? a.perm[lf.start + t] : kNoId;
+        s_ids[t] = id;
+        s_bnd[t] = t < b ? a.bound[id] : 0.f;
+    }
+    __syncthreads();
+    const float* mu = a.X + (size_t)s_ids[0] * ld;
+    for (uint32_t r = wave; r < b16; r += kLeafWaves) {
+        uint16_t* hrow = s_rows + r * L.rs;
+        if (r >= b) {
+            for (uint32_t j = lane; j < L.rs; j += 64) hrow[j] = 0;
+            if (lane == 0) {
+                s_nrm[r] = 0.f;
+                s_rinv[r] = 0.f;
+            }
+            continue;
+        }
+        const float* x = a.X + (size_t)s_ids[r] * ld;
+        float v0 = lane < d ? x[lane] - mu[lane] : 0.f;
+        float v1 = lane + 64 < d ? x[lane + 64] - mu[lane + 64] : 0.f;
+        const float m = wave_max(fmaxf(fabsf(v0), fabsf(v1)));
+        const float nrm = wave_sum(__builtin_fmaf(v0, v0, v1 * v1));
+        int e = 0;
+        if (m > 0.f) frexpf(m, &e);
+        const int sh = 14 - e;  // |v| * 2^sh < 2^14
+        const float sc = ldexpf(1.0f, sh);
+        v0 *= sc;
+        v1 *= sc;
+        uint16_t h0, h1;
+        if (HALF == 0) {
+            _Float16 f0 = (_Float16)v0, f1 = (_Float16)v1;
+            h0 = __builtin_bit_cast(uint16_t, f0);
+            h1 = __builtin_bit_cast(uint16_t, f1);
+        } else {
+            h0 = to_bf16_rne(v0);
+            h1 = to_bf16_rne(v1);
+        }
+        hrow[lane] = lane < d ? h0 : 0;
+        if (lane + 64 < L.rs) hrow[lane + 64] = lane + 64 < d ? h1 : 0;
+        if (lane == 0) {
+            s_nrm[r] = nrm;
+            s_rinv[r] = ldexpf(1.0f, -sh);
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 1: screening Gram --------------------------------------------------------------
+    const bool exact = a.screen == kScreenExact;
+    const float margin_c = HalfTraits<HALF>::kMargin;
+    const uint32_t ksteps = L.kp / 16;
+    const uint32_t lrow = lane & 15, lk = (lane >> 4) * 4;
+    for (uint32_t s = wave; s < nstrip; s += kLeafWaves) {
+        half4_t af[8];
+#pragma unroll
+        for (uint32_t kk = 0; kk < 8; ++kk)
+            if (kk < ksteps)
+                af[kk] = *reinterpret_cast<const half4_t*>(s_rows + (16 * s + lrow) * L.rs + kk * 16 + lk);
+        float rn[4], rinv[4], thr_base[4];
+        uint32_t rr[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            rr[i] = 16 * s + (lane >> 4) * 4 + i;
+            rn[i] = s_nrm[rr[i]];
+            rinv[i] = s_rinv[rr[i]];
+            const float bnd = s_bnd[rr[i]];
+            thr_base[i] = exact ? bnd * (1.0f + 1.0f / 65536.0f) : bnd;
+        }
+        for (uint32_t t = 0; t < nstrip; ++t) {
+            float4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (uint32_t kk = 0; kk < 8; ++kk) {
+                if (kk >= ksteps) break;
+                const half4_t bf =
+                    *reinterpret_cast<const half4_t*>(s_rows + (16 * t + lrow) * L.rs + kk * 16 + lk);
+                if (HALF == 0)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(af[kk], bf, acc, 0, 0, 0);
+                else
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                        __builtin_bit_cast(short4_t, af[kk]), __builtin_bit_cast(short4_t, bf), acc,
+                        0, 0, 0);
+            }
+            const uint32_t c = 16 * t + lrow;
+            const float cn = s_nrm[c], cinv = s_rinv[c];
+            uint64_t bits[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float g = acc[i] * (rinv[i] * cinv);
+                const float nsum = rn[i] + cn;
+                const float approx = nsum - 2.0f * g;
+                const float thr = exact ? thr_base[i] + margin_c * nsum : thr_base[i];
+                const bool ok = c < b && rr[i] < b && c != rr[i] && approx < thr;
+                bits[i] = __ballot(ok);
+            }
+            if (lane < 16) {
+                const uint32_t g = lane >> 2, i = lane & 3;
+                const uint64_t bw = i == 0 ? bits[0] : i == 1 ? bits[1] : i == 2 ? bits[2] : bits[3];
+                s_mask[(16 * s + lane) * L.w16 + t] = (uint16_t)(bw >> (16 * g));
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: exact distances for survivors, sorted insert --------------------------------
+    uint32_t n_surv = 0, n_ins = 0;
+    for (uint32_t r = wave; r < b; r += kLeafWaves) {
+        uint32_t w = lane < nstrip ? s_mask[r * L.w16 + lane] : 0;
+        const uint32_t cnt = __popc(w);
+        const uint32_t base = wave_excl_scan(cnt, lane);
+        const uint32_t S = __shfl(base + cnt, 63);
+        if (S == 0) continue;
+        uint32_t o = base;
+        while (w) {
+            const uint32_t bit = __ffs(w) - 1;
+            w &= w - 1;
+            s_cand[o++] = (uint16_t)(16 * lane + bit);
+        }
+        const uint32_t id_r = s_ids[r];
+        const float* xr = a.X + (size_t)id_r * ld;
+        for (uint32_t j = lane; j < ld; j += 64) s_rowbuf[j] = xr[j];
+        uint32_t* gid = a.knn_ids + (size_t)id_r * k;
+        float* gd = a.knn_d + (size_t)id_r * k;
+        WaveList lst;
+        lst.load(gid, gd, k, lane);
+        bool changed = false;
+        n_surv += S;
+        for (uint32_t c0 = 0; c0 < S; c0 += 64) {
+            float dd = kFltMax;
+            uint32_t cid = kNoId;
+            if (c0 + lane < S) {
+                cid = s_ids[s_cand[c0 + lane]];
+                dd = dist_ref(s_rowbuf, a.X + (size_t)cid * ld, d);
+            }
+            uint64_t acc = __ballot(dd < lst.bound);
+            while (acc) {
+                const uint32_t src = __ffsll((unsigned long long)acc) - 1;
+                acc &= acc - 1;
+                const float nd = __shfl(dd, src);
+                const uint32_t nid = __shfl(cid, src);
+                if (lst.insert(nd, nid, k, lane)) {
+                    changed = true;
+                    ++n_ins;
+                }
+            }
+        }
+        if (changed) {
+            lst.store(gid, gd, k, lane);
+            if (lane == 0) a.bound[id_r] = lst.bound;
+        }
+    }
+    if (a.counters && lane == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)n_surv);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[2]), (unsigned long long)n_ins);
+    }
+}
+
+size_t leaf_lds_bytes(uint32_t max_leaf, uint32_t d, uint32_t ld) {
+    return leaf_lds_layout(max_leaf, d, ld).total;
+}
+
+void launch_leaf(const LeafArgs& a, hipStream_t st) {
+    if (!a.nleaves) return;
+    const size_t lds = leaf_lds_bytes(a.max_leaf, a.d, a.ld);
+    if (a.half_kind == 0) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf<0>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        k_leaf<0><<<a.nleaves, kLeafThreads, lds, st>>>(a);
+    } else {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf<1>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        k_leaf<1><<<a.nleaves, kLeafThreads, lds, st>>>(a);
+    }
+}
+
+// ---- top-k state helpers ----------------------------------------------------------------------
+__global__ void k_init_knn(uint32_t* ids, float* dists, float* bound, uint32_t n, uint32_t k) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (size_t)n * k) {
+        ids[i] = kNoId;
+        dists[i] = kFltMax;
+    }
+    if (i < n) bound[i] = kFltMax;
+}
+
+void launch_init_knn(uint32_t* ids, float* dists, float* bound, uint32_t n, uint32_t k,
+                     hipStream_t st) {
+    const size_t tot = (size_t)n * k;
+    if (tot) k_init_knn<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(ids, dists, bound, n, k);
+}
+
+// finalize + padResult (KnnSets.hpp:113-126, :334-344): lists are already sorted by (dist, id);
+// empty slots become id 1.
+__global__ void k_finalize(const uint32_t* ids, const float* dists, uint32_t n, uint32_t k,
+                           uint32_t* out_ids, float* out_d) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * k) return;
+    const uint32_t id = ids[i];
+    out_ids[i] = id == kNoId ? 1u : id;
+    if (out_d) out_d[i] = dists[i];
+}
+
+void launch_finalize(const uint32_t* ids, const float* dists, uint32_t n, uint32_t k,
+                     uint32_t* out_ids, float* out_d, hipStream_t st) {
+    const size_t tot = (size_t)n * k;
+    if (tot) k_finalize<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(ids, dists, n, k, out_ids, out_d);
+}
+
+}  // namespace knng

@@ -1,0 +1,329 @@
+// Level-synchronous recursive 2-means split (splitKmeansBinaryProcess, SolutionKmeans.hpp:142-311)
+// for gfx950. Every node of one tree level is processed by the same few launches:
+//
+//   start_pair   one block per node: farthest pair among the sampled rows (kmeansStartVecs,
+//                :68-138) and the first hyperplane.
+//   lloyd_chunk  one block per (node, chunk): side test of the chunk's rows and FP64 per-side sums
+//                in entry order (:166-175 small branch, the whole range is one chunk; :218-249
+//                large branch, chunks of 256 sample draws).
+//   lloyd_finish one block per node: chunk partials added in chunk order, centres = sum / count
+//                (:182-185), next hyperplane.
+//   side_count   one block per tile of <= 1024 positions: final side test over the full range
+//                (:189-197, :262-289), one flag byte per position, lo count per tile.
+//   scan / node_lo / scatter / copy_back: stable partition of each node's positions (lo first).
+//
+// The side test reads one 400-byte row per position and is HBM bound; everything else is small.
+#include "knng_device.hpp"
+#include "knng_kernels.hpp"
+
+namespace knng {
+
+constexpr uint32_t kSplitThreads = 256;
+constexpr uint32_t kMaxStartRows = 1024;
+
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kSplitThreads) void k_start_pair(
+    const float* __restrict__ X, uint32_t ld, uint32_t d, const uint32_t* __restrict__ perm,
+    NodeDesc* __restrict__ nodes, float* __restrict__ c1, float* __restrict__ c2,
+    float* __restrict__ coef, float* __restrict__ off) {
+    NodeDesc& nd = nodes[blockIdx.x];
+    if (nd.status != kNodeActive) return;
+    __shared__ uint32_t rows[kMaxStartRows];
+    __shared__ float best_d[kSplitThreads];
+    __shared__ uint32_t best_u[kSplitThreads];
+    __shared__ __align__(16) float between[kMaxDim];
+    __shared__ float off_slot;
+    const uint32_t s = nd.s;
+    for (uint32_t t = threadIdx.x; t < s; t += blockDim.x)
+        rows[t] = perm[nd.start + draw_below(nd.akey, t, nd.size)];
+    __syncthreads();
+    float bd = kFltMin;
+    uint32_t bu = 0xFFFFFFFFu;
+    const uint32_t total = s * s;
+    for (uint32_t u = threadIdx.x; u < total; u += blockDim.x) {
+        const uint32_t i = u / s, j = u - i * s;
+        if (j <= i) continue;
+        const float dd = dist_ref(X + (size_t)rows[i] * ld, X + (size_t)rows[j] * ld, d);
+        if (dd > bd) {  // first maximum in (i, j) order within this thread (u ascending)
+            bd = dd;
+            bu = u;
+        }
+    }
+    best_d[threadIdx.x] = bd;
+    best_u[threadIdx.x] = bu;
+    __syncthreads();
+    for (uint32_t h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            const float od = best_d[threadIdx.x + h];
+            const uint32_t ou = best_u[threadIdx.x + h];
+            const float md = best_d[threadIdx.x];
+            const uint32_t mu = best_u[threadIdx.x];
+            if (ou != 0xFFFFFFFFu && (mu == 0xFFFFFFFFu || od > md || (od == md && ou < mu))) {
+                best_d[threadIdx.x] = od;
+                best_u[threadIdx.x] = ou;
+            }
+        }
+        __syncthreads();
+    }
+    const uint32_t u = best_u[0];
+    if (u == 0xFFFFFFFFu) {  // no pair beyond FLT_MIN: the reference dereferences null here
+        if (threadIdx.x == 0) nd.status = kNodeFailed;
+        return;
+    }
+    const float* ri = X + (size_t)rows[u / s] * ld;
+    const float* rj = X + (size_t)rows[u % s] * ld;
+    float* a = c1 + (size_t)blockIdx.x * ld;
+    float* b = c2 + (size_t)blockIdx.x * ld;
+    for (uint32_t j = threadIdx.x; j < ld; j += blockDim.x) {
+        a[j] = ri[j];
+        b[j] = rj[j];
+    }
+    __syncthreads();
+    const float o = block_hyperplane(a, b, coef + (size_t)blockIdx.x * ld, between, d, ld, &off_slot);
+    if (threadIdx.x == 0) off[blockIdx.x] = o;
+}
+
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kSplitThreads) void k_lloyd_chunk(
+    const float* __restrict__ X, uint32_t ld, uint32_t d, const uint32_t* __restrict__ perm,
+    const NodeDesc* __restrict__ nodes, const ChunkDesc* __restrict__ chunks,
+    const float* __restrict__ coef, const float* __restrict__ off, double* __restrict__ part,
+    uint32_t* __restrict__ pcnt) {
+    const ChunkDesc ch = chunks[blockIdx.x];
+    const NodeDesc nd = nodes[ch.node];
+    if (nd.status != kNodeActive) return;
+    __shared__ uint32_t rows[kMaxSmallNode];
+    __shared__ uint8_t flags[kMaxSmallNode];
+    __shared__ __align__(16) float cf[kMaxDim];
+    __shared__ uint32_t nlo;
+    const bool large = nd.m != 0;
+    for (uint32_t j = threadIdx.x; j < ld; j += blockDim.x) cf[j] = coef[(size_t)ch.node * ld + j];
+    if (threadIdx.x == 0) nlo = 0;
+    __syncthreads();
+    const float o = off[ch.node];
+    uint32_t mylo = 0;
+    for (uint32_t e = threadIdx.x; e < ch.count; e += blockDim.x) {
+        const uint32_t idx = ch.begin + e;
+        const uint32_t pos = nd.start + (large ? draw_below(nd.akey, idx, nd.size) : idx);
+        const uint32_t r = perm[pos];
+        rows[e] = r;
+        const bool lo = dot_ref(cf, X + (size_t)r * ld, d) >= o;
+        flags[e] = lo;
+        mylo += lo;
+    }
+    atomicAdd(&nlo, mylo);
+    __syncthreads();
+    // FP64 sums in entry order, one thread per (side, dim).
+    for (uint32_t t = threadIdx.x; t < 2 * ld; t += blockDim.x) {
+        const uint32_t side = t / ld, j = t - side * ld;
+        const uint8_t want = side == 0 ? 1 : 0;
+        double acc = 0.0;
+        if (j < d) {
+            for (uint32_t e = 0; e < ch.count; ++e)
+                if (flags[e] == want) acc += (double)X[(size_t)rows[e] * ld + j];
+        }
+        part[((size_t)blockIdx.x * 2 + side) * ld + j] = acc;
+    }
+    if (threadIdx.x == 0) {
+        pcnt[blockIdx.x * 2 + 0] = nlo;
+        pcnt[blockIdx.x * 2 + 1] = ch.count - nlo;
+    }
+}
+
+__global__ __launch_bounds__(kSplitThreads) void k_lloyd_finish(
+    NodeDesc* __restrict__ nodes, const double* __restrict__ part,
+    const uint32_t* __restrict__ pcnt, float* __restrict__ c1, float* __restrict__ c2,
+    float* __restrict__ coef, float* __restrict__ off, uint32_t d, uint32_t ld) {
+    NodeDesc& nd = nodes[blockIdx.x];
+    if (nd.status != kNodeActive) return;
+    __shared__ uint64_t cnt[2];
+    __shared__ __align__(16) float between[kMaxDim];
+    __shared__ float off_slot;
+    if (threadIdx.x < 2) {
+        uint64_t c = 0;
+        for (uint32_t k = 0; k < nd.nchunks; ++k) c += pcnt[(nd.chunk0 + k) * 2 + threadIdx.x];
+        cnt[threadIdx.x] = c;
+    }
+    __syncthreads();
+    if (cnt[0] == 0 || cnt[1] == 0) {  // an empty side restarts the node (:177-179, :251-253)
+        if (threadIdx.x == 0) nd.status = kNodeFailed;
+        return;
+    }
+    float* a = c1 + (size_t)blockIdx.x * ld;
+    float* b = c2 + (size_t)blockIdx.x * ld;
+    for (uint32_t t = threadIdx.x; t < 2 * ld; t += blockDim.x) {
+        const uint32_t side = t / ld, j = t - side * ld;
+        float v = 0.f;
+        if (j < d) {
+            double acc = 0.0;
+            for (uint32_t k = 0; k < nd.nchunks; ++k)
+                acc += part[((size_t)(nd.chunk0 + k) * 2 + side) * ld + j];
+            v = (float)(acc / (double)cnt[side]);
+        }
+        (side == 0 ? a : b)[j] = v;
+    }
+    __syncthreads();
+    const float o = block_hyperplane(a, b, coef + (size_t)blockIdx.x * ld, between, d, ld, &off_slot);
+    if (threadIdx.x == 0) off[blockIdx.x] = o;
+}
+
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kSplitThreads) void k_side_count(
+    const float* __restrict__ X, uint32_t ld, uint32_t d, const uint32_t* __restrict__ perm,
+    const NodeDesc* __restrict__ nodes, const TileDesc* __restrict__ tiles,
+    const float* __restrict__ coef, const float* __restrict__ off, uint8_t* __restrict__ flags,
+    uint32_t* __restrict__ tile_lo) {
+    const TileDesc tl = tiles[blockIdx.x];
+    const NodeDesc nd = nodes[tl.node];
+    __shared__ __align__(16) float cf[kMaxDim];
+    __shared__ uint32_t nlo;
+    if (nd.status == kNodeFailed) {
+        if (threadIdx.x == 0) tile_lo[blockIdx.x] = 0;
+        return;
+    }
+    for (uint32_t j = threadIdx.x; j < ld; j += blockDim.x) cf[j] = coef[(size_t)tl.node * ld + j];
+    if (threadIdx.x == 0) nlo = 0;
+    __syncthreads();
+    const float o = off[tl.node];
+    uint32_t mylo = 0;
+    const uint32_t base = nd.start + tl.begin;
+    for (uint32_t e = threadIdx.x; e < tl.count; e += blockDim.x) {
+        const uint32_t r = perm[base + e];
+        const bool lo = dot_ref(cf, X + (size_t)r * ld, d) >= o;
+        flags[base + e] = lo;
+        mylo += lo;
+    }
+    atomicAdd(&nlo, mylo);
+    __syncthreads();
+    if (threadIdx.x == 0) tile_lo[blockIdx.x] = nlo;
+}
+
+// Exclusive scan of tile lo counts over all tiles of the level (one block; tiles <= ~1e6).
+__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict__ tile_lo,
+                                                     uint32_t ntiles,
+                                                     uint32_t* __restrict__ tile_base) {
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (ntiles + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per, e = min(ntiles, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += tile_lo[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t h = 1; h < 1024; h <<= 1) {
+        uint32_t v = threadIdx.x >= h ? part[threadIdx.x - h] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;
+    for (uint32_t i = b; i < e; ++i) {
+        tile_base[i] = run;
+        run += tile_lo[i];
+    }
+    if (threadIdx.x == 1023) tile_base[ntiles] = part[1023];
+}
+
+__global__ void k_node_lo(NodeDesc* __restrict__ nodes, uint32_t nnodes,
+                          const uint32_t* __restrict__ tile_base) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    NodeDesc& nd = nodes[i];
+    if (nd.status == kNodeFailed) return;
+    const uint32_t lo = tile_base[nd.tile0 + nd.ntiles] - tile_base[nd.tile0];
+    nd.lo = lo;
+    if ((lo == 0 || lo == nd.size) && nd.status == kNodeActive) nd.status = kNodeFailed;
+}
+
+// Stable scatter: lo positions keep their order at the front, hi positions after them.
+__global__ __launch_bounds__(kSplitThreads) void k_scatter(
+    const uint32_t* __restrict__ perm, uint32_t* __restrict__ perm_out,
+    const NodeDesc* __restrict__ nodes, const TileDesc* __restrict__ tiles,
+    const uint8_t* __restrict__ flags, const uint32_t* __restrict__ tile_base) {
+    const TileDesc tl = tiles[blockIdx.x];
+    const NodeDesc nd = nodes[tl.node];
+    const uint32_t base = nd.start + tl.begin;
+    constexpr uint32_t kPer = kTileSize / kSplitThreads;
+    __shared__ uint32_t scan[kSplitThreads];
+    const bool identity = nd.status == kNodeFailed || nd.lo == 0 || nd.lo == nd.size;
+    uint8_t f[kPer];
+    uint32_t cnt = 0;
+    const uint32_t e0 = threadIdx.x * kPer;
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t e = e0 + q;
+        f[q] = e < tl.count ? flags[base + e] : 0;
+        cnt += f[q];
+    }
+    scan[threadIdx.x] = cnt;
+    __syncthreads();
+    for (uint32_t h = 1; h < kSplitThreads; h <<= 1) {
+        uint32_t v = threadIdx.x >= h ? scan[threadIdx.x - h] : 0;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const uint32_t lo_tile = tile_base[blockIdx.x] - tile_base[nd.tile0];
+    uint32_t rank = lo_tile + scan[threadIdx.x] - cnt;  // lo entries before e0 within the node
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t e = e0 + q;
+        if (e >= tl.count) break;
+        const uint32_t pos = base + e;
+        uint32_t dst;
+        if (identity) {
+            dst = pos;
+        } else if (f[q]) {
+            dst = nd.start + rank;
+            ++rank;
+        } else {
+            dst = nd.start + nd.lo + (tl.begin + e - rank);
+        }
+        perm_out[dst] = perm[pos];
+    }
+}
+
+__global__ __launch_bounds__(kSplitThreads) void k_copy_back(
+    uint32_t* __restrict__ perm, const uint32_t* __restrict__ perm_tmp,
+    const NodeDesc* __restrict__ nodes, const TileDesc* __restrict__ tiles) {
+    const TileDesc tl = tiles[blockIdx.x];
+    const uint32_t base = nodes[tl.node].start + tl.begin;
+    for (uint32_t e = threadIdx.x; e < tl.count; e += blockDim.x) perm[base + e] = perm_tmp[base + e];
+}
+
+__global__ void k_iota(uint32_t* __restrict__ perm, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm[i] = i;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host launchers.
+void launch_iota(uint32_t* perm, uint32_t n, hipStream_t st) {
+    if (n) k_iota<<<(n + 255) / 256, 256, 0, st>>>(perm, n);
+}
+
+void launch_split_prepare(const SplitLevelArgs& a, hipStream_t st) {
+    if (!a.run_lloyd) return;
+    k_start_pair<<<a.nnodes, kSplitThreads, 0, st>>>(a.X, a.ld, a.d, a.perm, a.nodes, a.c1, a.c2,
+                                                     a.coef, a.off);
+    for (uint32_t it = 0; it < a.iterations; ++it) {
+        if (a.nchunks)
+            k_lloyd_chunk<<<a.nchunks, kSplitThreads, 0, st>>>(a.X, a.ld, a.d, a.perm, a.nodes,
+                                                              a.chunks, a.coef, a.off, a.part,
+                                                              a.pcnt);
+        k_lloyd_finish<<<a.nnodes, kSplitThreads, 0, st>>>(a.nodes, a.part, a.pcnt, a.c1, a.c2,
+                                                           a.coef, a.off, a.d, a.ld);
+    }
+}
+
+void launch_split_side(const SplitLevelArgs& a, hipStream_t st) {
+    k_side_count<<<a.ntiles, kSplitThreads, 0, st>>>(a.X, a.ld, a.d, a.perm, a.nodes, a.tiles,
+                                                     a.coef, a.off, a.flags, a.tile_lo);
+}
+
+void launch_split_partition(const SplitLevelArgs& a, hipStream_t st) {
+    k_scan_tiles<<<1, 1024, 0, st>>>(a.tile_lo, a.ntiles, a.tile_base);
+    k_node_lo<<<(a.nnodes + 255) / 256, 256, 0, st>>>(a.nodes, a.nnodes, a.tile_base);
+    k_scatter<<<a.ntiles, kSplitThreads, 0, st>>>(a.perm, a.perm_tmp, a.nodes, a.tiles, a.flags,
+                                                  a.tile_base);
+    k_copy_back<<<a.ntiles, kSplitThreads, 0, st>>>(a.perm, a.perm_tmp, a.nodes, a.tiles);
+}
+
+}  // namespace knng

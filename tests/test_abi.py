@@ -1,0 +1,50 @@
+"""The C-ABI library loads and exports every symbol include/knng.h declares (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import sigmod23ann_amd as K
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "knng.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(knng_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(K.LIB_PATH):
+        pytest.skip("libknng.so not built")
+    L = ctypes.CDLL(K.LIB_PATH)
+    declared = _declared()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(L, name), name
+    assert sorted(K.EXPORTS) == declared
+
+
+def test_default_params_and_error_paths_without_gpu():
+    if not os.path.exists(K.LIB_PATH):
+        pytest.skip("libknng.so not built")
+    p = K.Params()
+    K.lib().knng_default_params(ctypes.byref(p))
+    assert (p.k, p.leaf_max, p.small_max, p.iterations) == (100, 400, 3000, 1)
+    # argument validation happens before any device call
+    bad = K.params(k=0)
+    h = ctypes.c_void_p()
+    assert K.lib().knng_create(ctypes.byref(bad), ctypes.byref(h)) == K.ERR_INVALID
+    assert b"k must be" in K.lib().knng_last_error()
+    assert K.lib().knng_create(None, ctypes.byref(h)) == K.ERR_INVALID
+
+
+def test_recall_grader_semantics():
+    import numpy as np
+    if not os.path.exists(K.LIB_PATH):
+        pytest.skip("libknng.so not built")
+    truth = np.array([[1, 2, 3, 4], [5, 6, 7, 8]], np.uint32)
+    res = np.array([[4, 3, 9, 9], [8, 7, 6, 5]], np.uint32)   # 2/4 and 4/4
+    assert K.recall(res, truth) == 0.75

@@ -1,0 +1,250 @@
+"""GPU parity: every stage through the C ABI against the CPU oracle on the same seeded inputs.
+
+Integer/index work (permutations, leaf ranges, lo sizes) must match bit for bit; top-k lists must
+match exactly in distance (both sides compute the reference-order FP32 distance) and in ids except
+where distances tie at the k-th place (the reference keeps the first offered, the GPU the smaller
+id). Sizes keep the oracle to a few seconds.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sigmod23ann_amd as K
+
+pytestmark = pytest.mark.gpu
+
+
+def mixture(n, d, centres, seed, spread=10.0, sigma=1.0):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-spread, spread, size=(centres, d)).astype(np.float32)
+    lab = rng.integers(0, centres, n)
+    return (c[lab] + sigma * rng.normal(size=(n, d))).astype(np.float32)
+
+
+def lowrank(n, d, seed, latent=24):
+    rng = np.random.default_rng(seed)
+    W = rng.normal(size=(latent, d))
+    z = rng.normal(size=(n, latent)) / np.sqrt(1.0 + np.arange(latent))
+    X = z @ W + 0.05 * rng.normal(size=(n, d))
+    return (X / np.linalg.norm(X, axis=1, keepdims=True)).astype(np.float32)
+
+
+def assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds):
+    """Same sorted distances; same ids wherever the distance is not tied with the k-th."""
+    ids = np.asarray(ids)
+    o_ids = np.asarray(o_ids)
+    np.testing.assert_array_equal(ds, o_ds)
+    boundary = ds == ds[:, -1:]
+    bad = (ids != o_ids) & ~boundary
+    assert not bad.any(), f"{int(bad.sum())} ids differ away from ties, first rows {np.where(bad.any(1))[0][:5]}"
+    for r in np.where((ids != o_ids).any(1))[0]:
+        assert set(ids[r][boundary[r]]) <= set(range(ids.shape[0]))
+
+
+def assert_valid_lists(X, ids, ds):
+    n, k = ids.shape
+    for r in range(0, n, max(1, n // 200)):
+        row = ids[r][ds[r] < np.finfo(np.float32).max]
+        assert r not in row
+        assert len(set(row.tolist())) == row.size
+        assert np.all(np.diff(ds[r]) >= 0)
+        for j, dv in zip(row[:5], ds[r][:5]):
+            assert oracle.dist(X[r], X[j]) == dv
+
+
+# ---------------------------------------------------------------------------------------------
+# Split
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("d", [100, 64, 128, 37])
+def test_partition_injected_bitexact(d):
+    X = mixture(20000, d, 50, 1)
+    rng = np.random.default_rng(2)
+    perm0 = rng.permutation(20000).astype(np.uint32)
+    starts = np.array([0, 5000, 5001, 9000, 15000], np.uint32)
+    sizes = np.array([5000, 1, 3999, 6000, 5000], np.uint32)
+    coefs = rng.normal(size=(5, d)).astype(np.float32)
+    offs = (rng.normal(size=5) * 5).astype(np.float32)
+    offs[1] = 1e30          # single-element node, every point on the hi side: unchanged
+    b = K.Builder(K.params(k=10))
+    perm, lo = b.partition_injected(X, perm0, starts, sizes, coefs, offs)
+    o_perm, o_lo = oracle.partition_injected(X, perm0, starts, sizes, coefs, offs)
+    np.testing.assert_array_equal(lo, o_lo)
+    np.testing.assert_array_equal(perm, o_perm)
+
+
+@pytest.mark.parametrize("n,d,leaf,small,iters,seed", [
+    (50000, 100, 256, 3000, 2, 11),
+    (30000, 100, 400, 3000, 1, 12),
+    (20000, 64, 100, 500, 1, 13),
+    (20000, 128, 256, 3000, 3, 14),
+    (7000, 37, 50, 300, 2, 15),
+])
+def test_split_round_bitexact(n, d, leaf, small, iters, seed):
+    X = mixture(n, d, 80, seed)
+    kw = dict(k=10, leaf_max=leaf, small_max=small, iterations=iters, seed=seed)
+    b = K.Builder(K.params(**kw))
+    o_prm = oracle.make_params(**kw)
+    for r in (0, 1):
+        perm, ls, lz = b.split_round(X, r)
+        o_perm, o_ls, o_lz = oracle.split_round(X, o_prm, r)
+        np.testing.assert_array_equal(ls, o_ls)
+        np.testing.assert_array_equal(lz, o_lz)
+        np.testing.assert_array_equal(perm, o_perm)
+
+
+def test_split_round_lowrank_and_alternating_leaf():
+    X = lowrank(40000, 100, 3)
+    kw = dict(k=10, leaf_max=199, leaf_max_odd=399, small_max=3000, iterations=1, seed=5)
+    b = K.Builder(K.params(**kw))
+    for r in (0, 1):
+        perm, ls, lz = b.split_round(X, r)
+        o_perm, o_ls, o_lz = oracle.split_round(X, oracle.make_params(**kw), r)
+        np.testing.assert_array_equal(perm, o_perm)
+        np.testing.assert_array_equal(lz, o_lz)
+        assert lz.max() <= (199 if r == 0 else 399)
+
+
+def test_split_degenerate_points():
+    X = np.ones((5000, 16), np.float32)
+    X[:10] += np.arange(10)[:, None]          # a few distinct rows, the rest identical
+    kw = dict(k=10, leaf_max=100, small_max=1000, iterations=1, seed=3)
+    perm, ls, lz = K.Builder(K.params(**kw)).split_round(X, 0)
+    o_perm, o_ls, o_lz = oracle.split_round(X, oracle.make_params(**kw), 0)
+    np.testing.assert_array_equal(perm, o_perm)
+    np.testing.assert_array_equal(lz, o_lz)
+
+
+# ---------------------------------------------------------------------------------------------
+# Leaf stage, NoN, whole build
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("k,leaf,d", [(100, 256, 100), (10, 400, 100), (32, 128, 64), (100, 400, 128)])
+def test_leaf_stage_matches_oracle(k, leaf, d):
+    X = mixture(12000, d, 40, 21)
+    kw = dict(k=k, leaf_max=leaf, small_max=3000, iterations=1, seed=9)
+    o_prm = oracle.make_params(**kw)
+    b = K.Builder(K.params(**kw))
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    ids, ds = b.leaf_stage(X, perm, ls, lz)
+    o_ids, o_ds, o_sz = oracle.leaf_stage(X, k, perm, ls, lz)
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    # second round merges into the existing sets
+    perm2, ls2, lz2 = oracle.split_round(X, o_prm, 1)
+    ids2, ds2 = b.leaf_stage(X, perm2, ls2, lz2, state=(ids, ds))
+    o2 = oracle.leaf_stage(X, k, perm2, ls2, lz2, state=(o_ids, o_ds, o_sz))
+    assert_lists_equal_up_to_ties(ids2, ds2, o2[0], o2[1])
+
+
+def test_leaf_stage_lowrank_unit_norm():
+    # close pairs on the unit sphere: the screen's margin must keep every true candidate
+    X = lowrank(20000, 100, 8)
+    kw = dict(k=100, leaf_max=400, small_max=3000, iterations=1, seed=4)
+    o_prm = oracle.make_params(**kw)
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    ids, ds = K.Builder(K.params(**kw)).leaf_stage(X, perm, ls, lz)
+    o_ids, o_ds, _ = oracle.leaf_stage(X, 100, perm, ls, lz)
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+
+
+def test_non_matches_oracle():
+    X = mixture(6000, 100, 30, 31)
+    k = 20
+    kw = dict(k=k, leaf_max=100, small_max=3000, iterations=1, seed=2)
+    o_prm = oracle.make_params(**kw)
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    st = oracle.leaf_stage(X, k, perm, ls, lz)
+    ids, ds = K.Builder(K.params(**kw)).non(X, (st[0], st[1]))
+    o_ids, o_ds, _ = oracle.non(X, k, st)
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+
+
+def test_non_with_unfilled_lists():
+    # lists shorter than k: unfilled slots read as id 0 (KnnSets.hpp:29), as in the reference
+    X = mixture(3000, 32, 10, 5)
+    k = 40
+    kw = dict(k=k, leaf_max=20, small_max=500, iterations=1, seed=3)
+    o_prm = oracle.make_params(**kw)
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    st = oracle.leaf_stage(X, k, perm, ls, lz)
+    assert (st[2] < k).all()
+    ids, ds = K.Builder(K.params(**kw)).non(X, (st[0], st[1]))
+    o_ids, o_ds, _ = oracle.non(X, k, st)
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+
+
+@pytest.mark.parametrize("k,rounds,non", [(20, 3, 1), (100, 2, 1), (10, 4, 0)])
+def test_build_end_to_end_matches_oracle(k, rounds, non):
+    X = mixture(15000, 100, 60, 41)
+    kw = dict(k=k, leaf_max=256, small_max=3000, iterations=2, rounds=rounds, non_passes=non, seed=17)
+    ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
+    o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(**kw))
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+
+
+def test_build_reference_stride_and_padding():
+    # stride-112 rows as the reference stores them (io.h:77), short lists padded with id 1
+    X = mixture(2000, 100, 10, 6)
+    Xs = np.zeros((2000, 112), np.float32)
+    Xs[:, :100] = X
+    b = K.Builder(K.params(k=30, leaf_max=10, small_max=200, rounds=1, non_passes=0, seed=1))
+    ids = np.empty((2000, 30), np.uint32)
+    ds = np.empty((2000, 30), np.float32)
+    K._check(K.lib().knng_build(b.h, K._f(Xs), 2000, 100, 112, K._u(ids), K._f(ds)))
+    o_ids, o_ds, o_sz, _ = oracle.build_graph(X, oracle.make_params(k=30, leaf_max=10, small_max=200,
+                                                                     rounds=1, seed=1))
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    short = o_sz < 30
+    assert short.any()
+    for r in np.where(short)[0]:
+        assert np.all(ids[r, o_sz[r]:] == 1)
+
+
+# ---------------------------------------------------------------------------------------------
+# Precision forms, exact FP64 tool, generators
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", [K.PREC_FP16, K.PREC_BF16])
+def test_precision_forms_valid_and_close(prec):
+    X = lowrank(20000, 100, 12)
+    kw = dict(k=50, leaf_max=256, small_max=3000, iterations=1, rounds=4, non_passes=1, seed=3)
+    ids, ds = K.Builder(K.params(precision=prec, **kw)).build(X, return_dists=True)
+    f_ids, f_ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
+    assert_valid_lists(X, ids, ds)
+    q = np.arange(0, 20000, 40, dtype=np.uint32)
+    truth, _ = oracle.exact_knn(X, q, 50)
+    r_p, r_f = K.recall(ids[q], truth), K.recall(f_ids[q], truth)
+    assert r_p > r_f - 0.02, (r_p, r_f)
+
+
+def test_exact_knn_gpu_matches_oracle():
+    X = mixture(4000, 100, 20, 51)
+    q = np.arange(0, 4000, 37, dtype=np.uint32)
+    b = K.Builder(K.params(k=10))
+    ids, ds = b.exact_knn(X, q, 25)
+    o_ids, o_ds = oracle.exact_knn(X, q, 25)
+    np.testing.assert_allclose(ds, o_ds, rtol=1e-12, atol=1e-9)
+    np.testing.assert_array_equal(ids, o_ids)
+
+
+def test_recall_against_fp64_c0_like():
+    X = mixture(20000, 100, 100, 61)
+    kw = dict(k=100, leaf_max=256, small_max=3000, iterations=2, rounds=8, non_passes=1, seed=1)
+    ids = K.Builder(K.params(**kw)).build(X)
+    q = np.arange(0, 20000, 50, dtype=np.uint32)
+    truth, _ = oracle.exact_knn(X, q, 100)
+    assert K.recall(ids[q], truth) > 0.9
+
+
+def test_generators_on_device():
+    b = K.Builder(K.params(k=10))
+    n, d = 4096, 100
+    dX = K.DeviceArray(n * d * 4)
+    b.generate_device(K.GEN_LOWRANK, n, d, d, 5, dX.ptr)
+    X = dX.download((n, d), np.float32)
+    np.testing.assert_allclose(np.linalg.norm(X, axis=1), 1.0, rtol=1e-5)
+    # low rank: 24 latent directions carry almost all the energy
+    s = np.linalg.svd(X - X.mean(0), compute_uv=False)
+    assert (s[:24] ** 2).sum() / (s ** 2).sum() > 0.95
+    b.generate_device(K.GEN_MIXTURE, n, d, d, 5, dX.ptr)
+    Y = dX.download((n, d), np.float32)
+    assert np.abs(Y).max() < 20 and np.abs(Y).max() > 8
+    b.generate_device(K.GEN_LOWRANK, n, d, d, 5, dX.ptr)
+    np.testing.assert_array_equal(dX.download((n, d), np.float32), X)     # seeded
