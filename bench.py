@@ -1,0 +1,178 @@
+"""Benchmark: KNN-graph build throughput on one MI355X.
+
+Workload (default): 10,000,000 x 100 float32 points, low-rank embedding-like data (24 latent dims,
+sigma_i = 1/sqrt(1+i), noise 0.05, unit norm), k = 100, leaf <= 400, generated on the device so the
+input is resident in HBM before timing. One step = one whole KNN-graph build (all split rounds,
+leaf stages, the neighbours-of-neighbours pass and the finalize) through the C ABI
+(knng_build_device). value = points / second of the build.
+
+Also reported: recall@k against exact FP64 k-NN of sampled points, the roofline of the dominant
+kernel (HIP-event timings from the library), and a CPU baseline: the CPU restatement of the
+reference path (oracle/) timed on this host's cores on a bounded prefix of the same data.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "KNN-graph points/sec at recall@100≥0.94 (10M×100, k=100); p99 job latency"
+HBM_PEAK_GBS = 8000.0
+MFMA_FP16_PEAK_TFLOPS = 2500.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--d", type=int, default=100)
+    ap.add_argument("--k", type=int, default=100)
+    ap.add_argument("--rounds", type=int, default=8)
+    ap.add_argument("--leaf", type=int, default=400)
+    ap.add_argument("--iterations", type=int, default=1)
+    ap.add_argument("--non", type=int, default=2)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"])
+    ap.add_argument("--seed", type=int, default=2023)
+    ap.add_argument("--recall-queries", type=int, default=100_000)
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the build does not "
+              "shard across GPUs in this version", file=sys.stderr)
+        sys.exit(2)
+    import sigmod23ann_amd as K
+
+    prec = {"fp32": K.PREC_FP32, "fp16": K.PREC_FP16, "bf16": K.PREC_BF16}[a.precision]
+    prm = K.params(k=a.k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
+                   rounds=a.rounds, non_passes=a.non, precision=prec, seed=a.seed)
+    b = K.Builder(prm)
+    n, d, k = a.n, a.d, a.k
+    ld = (d + 3) & ~3
+    log(f"[bench] generating {n}x{d} low-rank points on the device")
+    dX = K.DeviceArray(n * ld * 4)
+    b.generate_device(K.GEN_LOWRANK, n, d, ld, a.seed, dX.ptr)
+    dIds = K.DeviceArray(n * k * 4)
+    dD = K.DeviceArray(n * k * 4)
+
+    for w in range(a.warmup):
+        t0 = time.perf_counter()
+        b.build_device(dX.ptr, n, d, ld, dIds.ptr, dD.ptr)
+        log(f"[bench] warmup {w}: {time.perf_counter() - t0:.3f} s")
+    times, stats = [], []
+    for s in range(a.steps):
+        K.lib().knng_synchronize()
+        t0 = time.perf_counter()
+        b.build_device(dX.ptr, n, d, ld, dIds.ptr, dD.ptr)
+        K.lib().knng_synchronize()
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        st = b.stats()
+        stats.append(st)
+        log(f"[bench] step {s}: {dt:.3f} s  split {st['split_ms']:.1f} ms (side {st['side_ms']:.1f}) "
+            f"leaf {st['leaf_ms']:.1f} ms non {st['non_ms']:.1f} ms")
+    mean_t = float(np.mean(times))
+    value = n / mean_t
+    st = stats[-1]
+
+    # ---- recall against exact FP64 k-NN of sampled points ---------------------------------------
+    rec = None
+    nq = min(a.recall_queries, n)
+    if nq > 0:
+        t0 = time.perf_counter()
+        rng = np.random.default_rng(a.seed + 1)
+        q = np.sort(rng.choice(n, size=nq, replace=False)).astype(np.uint32)
+        ids = dIds.download((n, k), np.uint32)
+        got = np.ascontiguousarray(ids[q])
+        dq = K.DeviceArray(nq * 4)
+        dq.upload(q)
+        dl = K.DeviceArray(nq * k * 4)
+        dl.upload(got)
+        dub = K.DeviceArray(nq * 8)
+        b.exact_upper_from_lists(dX.ptr, n, d, ld, dq.ptr, nq, dl.ptr, k, dub.ptr)
+        dti = K.DeviceArray(nq * k * 4)
+        dtd = K.DeviceArray(nq * k * 8)
+        b.exact_knn_device(dX.ptr, n, d, ld, dq.ptr, nq, k, dub.ptr, 4 * k + 256, dti.ptr, dtd.ptr)
+        truth = dti.download((nq, k), np.uint32)
+        rec = K.recall(got, truth)
+        log(f"[bench] recall@{k} on {nq} sampled points vs FP64: {rec:.4f} "
+            f"({time.perf_counter() - t0:.1f} s)")
+
+    # ---- roofline of the dominant kernel -----------------------------------------------------
+    side_bytes = st["side_rows"] * (4 * d + 4 + 1)          # row, perm entry, flag byte
+    side_gbs = side_bytes / (st["side_ms"] * 1e-3) / 1e9 if st["side_ms"] > 0 else 0.0
+    leaf_flops = 2.0 * d * st["leaf_pairs"]                  # d * sum b(b-1): unique pairs x 2d
+    leaf_tfs = leaf_flops / (st["leaf_ms"] * 1e-3) / 1e12 if st["leaf_ms"] > 0 else 0.0
+    if st["side_ms"] >= st["leaf_ms"]:
+        roof = {"kernel": "k_side_count", "bound": "hbm", "achieved": round(side_gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(side_gbs / HBM_PEAK_GBS, 4),
+                "traffic": None, "launches": st["side_launches"],
+                "avg_launch_ms": st["side_ms"] / max(1, st["side_launches"]),
+                "bytes_per_launch": side_bytes / max(1, st["side_launches"])}
+    else:
+        roof = {"kernel": "k_leaf", "bound": "mfma", "achieved": round(leaf_tfs, 2),
+                "peak": MFMA_FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(leaf_tfs / MFMA_FP16_PEAK_TFLOPS, 5), "traffic": None,
+                "launches": st["leaf_launches"],
+                "avg_launch_ms": st["leaf_ms"] / max(1, st["leaf_launches"]),
+                "flops_per_launch": leaf_flops / max(1, st["leaf_launches"])}
+    roof["side_test_gbs"] = round(side_gbs, 1)
+    roof["leaf_tflops"] = round(leaf_tfs, 3)
+
+    # ---- CPU baseline: the oracle's restatement on a prefix sample ---------------------------
+    cpu = None
+    if not a.no_cpu and a.cpu_sample > 0:
+        import oracle
+        ns = min(a.cpu_sample, n)
+        Xh = dX.download((n, ld), np.float32)[:ns, :d].copy()
+        threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
+        op = oracle.make_params(k=k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
+                                rounds=a.rounds, non_passes=a.non, seed=a.seed)
+        t0 = time.perf_counter()
+        oracle.build_graph(Xh, op, threads=threads)
+        ct = time.perf_counter() - t0
+        cpu = {"value": round(ns / ct, 1), "unit": "points/s", "cores": threads, "kind": "port",
+               "sample": f"first {ns} points of the same data, same rounds/leaf/k/NoN, "
+                         f"{ct:.1f} s on {threads} threads"}
+        log(f"[bench] cpu baseline: {ns / ct:.0f} points/s ({ct:.1f} s, {threads} threads)")
+
+    lat = sorted(t * 1e3 for t in times)
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "points/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(mean_t * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"low-rank {n}x{d} (24 latent, unit norm), k={k}, leaf<={a.leaf}",
+                   "n": n, "d": d, "k": k, "rounds": a.rounds, "iterations": a.iterations,
+                   "non_passes": a.non, "precision": a.precision,
+                   "recall_at_k": None if rec is None else round(rec, 4),
+                   "recall_queries": nq, "parallelism": "single GPU"},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                       "p99": round(float(np.percentile(lat, 99)), 2)},
+        "stages_ms": {"split": round(st["split_ms"], 2), "side": round(st["side_ms"], 2),
+                      "leaf": round(st["leaf_ms"], 2), "non": round(st["non_ms"], 2),
+                      "total": round(st["total_ms"], 2)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -33,6 +33,8 @@ extern "C" {
 #define KNNG_PREC_FP16 1 /* fp16 MFMA screen, no margin, FP32 re-rank of the survivors */
 #define KNNG_PREC_BF16 2 /* bf16 MFMA screen, no margin, FP32 re-rank of the survivors */
 
+#define KNNG_FLAG_DIRECT_NON 1u /* neighbours-of-neighbours by the per-point direct kernel */
+
 /* Build parameters. The reference hard-codes k = dims = 100 (src/Constants.hpp:23-24),
  * maxGroupSize 200/400 alternating (src/SolutionKmeans.hpp:582-587), knnIterations = 1 (:591),
  * the 3000-point small-branch threshold (:156) and a wall-clock round budget (:562). */
@@ -46,7 +48,7 @@ typedef struct knng_params {
     uint32_t non_passes;   /* neighbours-of-neighbours passes after the rounds */
     uint32_t precision;    /* KNNG_PREC_* */
     uint32_t rerank;       /* reserved: per-point re-rank cap of the approximate screens */
-    uint32_t flags;        /* reserved, 0 */
+    uint32_t flags;        /* KNNG_FLAG_* */
     uint64_t seed;
 } knng_params;
 
@@ -66,6 +68,7 @@ typedef struct knng_stats {
     uint64_t inserts;      /* list insertions */
     uint64_t split_failures;
     uint64_t split_fallbacks;
+    uint64_t non_overflow;  /* points whose NoN candidate buffer overflowed (redone directly) */
 } knng_stats;
 
 typedef struct knng_handle knng_handle;

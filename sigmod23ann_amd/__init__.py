@@ -32,6 +32,8 @@ PREC_FP32 = 0
 PREC_FP16 = 1
 PREC_BF16 = 2
 
+FLAG_DIRECT_NON = 1
+
 GEN_LOWRANK = 0
 GEN_MIXTURE = 1
 GEN_ZIPF = 2
@@ -88,6 +90,7 @@ class Stats(C.Structure):
         ("inserts", C.c_uint64),
         ("split_failures", C.c_uint64),
         ("split_fallbacks", C.c_uint64),
+        ("non_overflow", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -154,9 +157,9 @@ def _check(code):
 
 
 def params(k=100, leaf_max=400, leaf_max_odd=0, small_max=3000, iterations=1, rounds=8,
-           non_passes=1, precision=PREC_FP32, rerank=160, seed=12345) -> Params:
+           non_passes=1, precision=PREC_FP32, rerank=160, seed=12345, flags=0) -> Params:
     return Params(k, leaf_max, leaf_max_odd, small_max, iterations, rounds, non_passes, precision,
-                  rerank, 0, seed)
+                  rerank, flags, seed)
 
 
 def _f(a):

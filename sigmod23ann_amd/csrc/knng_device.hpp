@@ -171,4 +171,85 @@ struct WaveList {
     }
 };
 
+// ---- wave helpers and the 16-bit screening rows ----------------------------------------------
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// Row stride of the 16-bit rows in LDS (elements); +8 staggers banks between rows.
+__host__ __device__ inline uint32_t leaf_row_stride(uint32_t d) { return ((d + 15) & ~15u) + 8; }
+
+__device__ inline uint16_t to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ inline float wave_sum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline uint32_t wave_excl_scan(uint32_t v, uint32_t lane) {
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x - v;
+}
+
+// Margin coefficient c of the exact screen: |approx - exact| <= c * (n_i + n_j).
+// fp16: 2u + u^2 + accumulation (K<=128) + norm and final rounding, u = 2^-11 -> 9.9e-4; bf16:
+// u = 2^-8 -> 7.9e-3. 25% slack on top.
+template <uint32_t HALF>
+struct HalfTraits;
+template <>
+struct HalfTraits<0> {
+    static constexpr float kMargin = 1.25e-3f;
+};
+template <>
+struct HalfTraits<1> {
+    static constexpr float kMargin = 1.0e-2f;
+};
+
+// Converts one row (d <= 128) centred on mu into the 16-bit screening layout; lane j holds dims j
+// and j + 64. Returns the FP32 squared norm of the centred row and the inverse power-of-two scale.
+template <uint32_t HALF>
+__device__ inline void screen_row(const float* __restrict__ x, float mu0, float mu1, uint32_t d,
+                                  uint32_t rs, uint16_t* __restrict__ hrow, uint32_t lane,
+                                  float& nrm, float& rinv) {
+    float v0 = lane < d ? x[lane] - mu0 : 0.f;
+    float v1 = lane + 64 < d ? x[lane + 64] - mu1 : 0.f;
+    const float m = wave_max(fmaxf(fabsf(v0), fabsf(v1)));
+    nrm = wave_sum(__builtin_fmaf(v0, v0, v1 * v1));
+    int e = 0;
+    if (m > 0.f) frexpf(m, &e);
+    const int sh = 14 - e;  // |v| * 2^sh < 2^14
+    const float sc = ldexpf(1.0f, sh);
+    v0 *= sc;
+    v1 *= sc;
+    uint16_t h0, h1;
+    if (HALF == 0) {
+        _Float16 f0 = (_Float16)v0, f1 = (_Float16)v1;
+        h0 = __builtin_bit_cast(uint16_t, f0);
+        h1 = __builtin_bit_cast(uint16_t, f1);
+    } else {
+        h0 = to_bf16_rne(v0);
+        h1 = to_bf16_rne(v1);
+    }
+    hrow[lane] = lane < d ? h0 : 0;
+    if (lane + 64 < rs) hrow[lane + 64] = lane + 64 < d ? h1 : 0;
+    rinv = ldexpf(1.0f, -sh);
+}
+
+__device__ inline float4_t mfma16(half4_t a, half4_t b, float4_t c, uint32_t half_kind) {
+    if (half_kind == 0) return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4_t, a),
+                                                     __builtin_bit_cast(short4_t, b), c, 0, 0, 0);
+}
+
 }  // namespace knng

@@ -171,6 +171,43 @@ __global__ void k_exact_ub(ExactScanArgs a, const uint32_t* __restrict__ lists, 
     ub[q] = valid >= k ? worst * (1.0 + 1e-12) + 1e-300 : __builtin_inf();
 }
 
+// Rows q of (ids, dists) produced for a sub-list of queries go back to rows where[q].
+__global__ void k_exact_scatter(const uint32_t* __restrict__ where, uint32_t nsub, uint32_t k,
+                                const uint32_t* __restrict__ sub_ids, const double* __restrict__ sub_d,
+                                uint32_t* __restrict__ out_ids, double* __restrict__ out_d) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nsub * k) return;
+    const size_t q = i / k, j = i - q * k;
+    out_ids[(size_t)where[q] * k + j] = sub_ids[i];
+    out_d[(size_t)where[q] * k + j] = sub_d[i];
+}
+
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx,
+                             uint32_t n, uint32_t* __restrict__ dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+__global__ void k_gather_f64(const double* __restrict__ src, const uint32_t* __restrict__ idx,
+                             uint32_t n, double* __restrict__ dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+void launch_exact_scatter(const uint32_t* where, uint32_t nsub, uint32_t k, const uint32_t* sub_ids,
+                          const double* sub_d, uint32_t* out_ids, double* out_d, hipStream_t st) {
+    const size_t tot = (size_t)nsub * k;
+    if (tot) k_exact_scatter<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(where, nsub, k, sub_ids,
+                                                                          sub_d, out_ids, out_d);
+}
+void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* dst,
+                       hipStream_t st) {
+    if (n) k_gather_u32<<<(n + 255) / 256, 256, 0, st>>>(src, idx, n, dst);
+}
+void launch_gather_f64(const double* src, const uint32_t* idx, uint32_t n, double* dst,
+                       hipStream_t st) {
+    if (n) k_gather_f64<<<(n + 255) / 256, 256, 0, st>>>(src, idx, n, dst);
+}
+
 void launch_norms64(const float* X, uint32_t ld, uint32_t d, uint32_t n, double* out,
                     hipStream_t st) {
     if (n) k_norms64<<<(n + 255) / 256, 256, 0, st>>>(X, ld, d, n, out);

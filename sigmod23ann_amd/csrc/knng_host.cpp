@@ -14,6 +14,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -155,6 +156,14 @@ struct Builder {
     DevBuf<double> norms64, cand_d, ubuf;
     DevBuf<uint32_t> cand_id, qcount, ovf;
 
+    // neighbours-of-neighbours scratch
+    static constexpr uint32_t kNonCap = 192;  // candidate slots per point before the direct path
+    static constexpr uint32_t kNonSubpasses = 16;
+    DevBuf<uint32_t> ovf_flag;
+    DevBuf<uint32_t> rcnt, rfill, rsl, ncnt, nbuf_id;
+    DevBuf<uint64_t> roff, scan_tmp;
+    DevBuf<float> nbuf_d;
+
     // generator scratch
     DevBuf<float> gen_a, gen_b, gen_c;
 
@@ -164,7 +173,10 @@ struct Builder {
         if (st) (void)hipStreamDestroy(st);
     }
 
+    bool verbose = false;
+
     int init(const knng_params& p) {
+        verbose = std::getenv("KNNG_VERBOSE") != nullptr;
         std::string why;
         if (!valid_params(p, why)) return fail(KNNG_ERR_INVALID, why);
         prm = p;
@@ -324,6 +336,7 @@ struct Builder {
         KNNG_TRY(perm_tmp.ensure(n));
         KNNG_TRY(flags.ensure(n));
         launch_iota(perm.p, n, st);
+        perm_is_tree = true;
         out_leaves.clear();
         const uint32_t leaf_max = leaf_max_for(round);
         if (n <= leaf_max) {
@@ -430,26 +443,141 @@ struct Builder {
         return KNNG_OK;
     }
 
+    // Permutation of the last split round; processing order of the reverse NoN kernel.
+    bool perm_is_tree = false;
+
     int non_pass() {
-        KNNG_TRY(snap.ensure((size_t)n * prm.k));
-        launch_non_snapshot(knn_ids.p, snap.p, n, prm.k, st);
-        NonArgs a{};
+        const uint32_t k = prm.k;
+        const size_t nk = (size_t)n * k;
+        KNNG_TRY(snap.ensure(nk));
+        launch_non_snapshot(knn_ids.p, snap.p, n, k, st);
+        NonArgs da{};
+        da.X = X;
+        da.ld = ld;
+        da.d = d;
+        da.k = k;
+        da.n = n;
+        da.snap = snap.p;
+        da.knn_ids = knn_ids.p;
+        da.knn_d = knn_d.p;
+        da.bound = bound.p;
+        const bool reverse = non_reverse_lds(k, ld) <= 160 * 1024 &&
+                             !(prm.flags & KNNG_FLAG_DIRECT_NON);
+        if (!reverse) {
+            launch_non_direct(da, nullptr, n, st);
+            KNNG_HIP(hipGetLastError());
+            return KNNG_OK;
+        }
+        KNNG_TRY(rcnt.ensure(n));
+        KNNG_TRY(rfill.ensure(n));
+        KNNG_TRY(roff.ensure((size_t)n + 1));
+        KNNG_TRY(scan_tmp.ensure(scan_scratch_elems(n)));
+        KNNG_TRY(rsl.ensure(nk));
+        KNNG_TRY(ncnt.ensure(n));
+        KNNG_TRY(nbuf_d.ensure((size_t)n * kNonCap));
+        KNNG_TRY(nbuf_id.ensure((size_t)n * kNonCap));
+        KNNG_TRY(ovf.ensure((size_t)n + 1));
+        KNNG_TRY(ovf_flag.ensure(n));
+        KNNG_TRY(perm.ensure(n));
+        if (!perm_is_tree) launch_iota(perm.p, n, st);
+        launch_reverse_lists(snap.p, n, k, rcnt.p, roff.p, scan_tmp.p, rfill.p, rsl.p, st);
+        KNNG_HIP(hipMemsetAsync(ovf.p, 0, ((size_t)n + 1) * 4, st));
+        KNNG_HIP(hipMemsetAsync(ovf_flag.p, 0, (size_t)n * 4, st));
+        NonRevArgs a{};
         a.X = X;
         a.ld = ld;
         a.d = d;
-        a.k = prm.k;
+        a.k = k;
         a.n = n;
         a.snap = snap.p;
+        a.roff = roff.p;
+        a.rs = rsl.p;
         a.knn_ids = knn_ids.p;
         a.knn_d = knn_d.p;
         a.bound = bound.p;
-        launch_non_direct(a, st);
+        a.ncnt = ncnt.p;
+        a.nbuf_d = nbuf_d.p;
+        a.nbuf_id = nbuf_id.p;
+        a.cap = kNonCap;
+        a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
+        a.counters = counters.p;
+        // Sub-passes over interleaved parts of the tree order (every subs-th q, so a point's q are
+        // spread over all of them), each merged before the next: the bounds tighten as the pass
+        // goes, which keeps the candidate buffers small.
+        const uint32_t subs = std::max(1u, std::min(kNonSubpasses, n / 1024 + 1));
+        float t_rev = 0.f, t_merge = 0.f, ms = 0.f;
+        for (uint32_t sp = 0; sp < subs; ++sp) {
+            a.order = perm.p + sp;
+            a.order_stride = subs;
+            a.nq = (n - sp + subs - 1) / subs;
+            KNNG_HIP(hipMemsetAsync(ncnt.p, 0, (size_t)n * 4, st));
+            KNNG_HIP(hipEventRecord(ev[2], st));
+            launch_non_reverse(a, st);
+            KNNG_HIP(hipEventRecord(ev[3], st));
+            launch_non_merge(a, ovf.p, ovf.p + 1, ovf_flag.p, st);
+            KNNG_HIP(hipEventRecord(ev[4], st));
+            KNNG_HIP(hipGetLastError());
+            if (verbose) {
+                KNNG_HIP(hipEventSynchronize(ev[4]));
+                KNNG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+                t_rev += ms;
+                KNNG_HIP(hipEventElapsedTime(&ms, ev[3], ev[4]));
+                t_merge += ms;
+            }
+        }
+        uint32_t novf = 0;
+        KNNG_HIP(hipMemcpyAsync(&novf, ovf.p, 4, hipMemcpyDeviceToHost, st));
+        KNNG_HIP(hipStreamSynchronize(st));
+        stats.non_overflow += novf;
+        KNNG_HIP(hipEventRecord(ev[2], st));
+        if (novf) launch_non_direct(da, ovf.p + 1, novf, st);
+        KNNG_HIP(hipEventRecord(ev[3], st));
         KNNG_HIP(hipGetLastError());
+        if (verbose) {
+            KNNG_HIP(hipEventSynchronize(ev[3]));
+            KNNG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+            std::fprintf(stderr, "[knng] non pass: %u sub-passes, reverse %.1f ms merge %.1f ms, "
+                                 "%u overflowed points direct %.1f ms\n",
+                         subs, t_rev, t_merge, novf, ms);
+        }
+        return KNNG_OK;
+    }
+
+    // One scan + select over `nq` queries with `cap` candidate slots each; qcount keeps the counts.
+    int exact_pass(const uint32_t* d_queries, uint32_t nq, uint32_t k, const double* ub,
+                   uint32_t cap, uint32_t* out_ids, double* out_d) {
+        KNNG_TRY(cand_d.ensure((size_t)nq * cap));
+        KNNG_TRY(cand_id.ensure((size_t)nq * cap));
+        KNNG_TRY(qcount.ensure(nq));
+        KNNG_TRY(ovf.ensure(1));
+        KNNG_HIP(hipMemsetAsync(qcount.p, 0, (size_t)nq * 4, st));
+        KNNG_HIP(hipMemsetAsync(ovf.p, 0, 4, st));
+        ExactScanArgs a{};
+        a.X = X;
+        a.ld = ld;
+        a.d = d;
+        a.n = n;
+        a.norms = norms64.p;
+        a.queries = d_queries;
+        a.nq = nq;
+        a.ub = ub;
+        a.cap = cap;
+        a.count = qcount.p;
+        a.cand_d = cand_d.p;
+        a.cand_id = cand_id.p;
+        const uint32_t qblocks = (nq + 63) / 64;
+        const uint32_t slices =
+            std::max(1u, std::min<uint32_t>(2048u / std::max(qblocks, 1u) + 1, (n + 4095) / 4096));
+        launch_exact_scan(a, slices, st);
+        launch_exact_select(a, k, out_ids, out_d, ovf.p, st);
+        KNNG_HIP(hipGetLastError());
+        KNNG_HIP(hipStreamSynchronize(st));
         return KNNG_OK;
     }
 
     int build(uint32_t* out_ids, float* out_d) {
         stats = knng_stats{};
+        perm_is_tree = false;
         KNNG_TRY(ensure_state());
         KNNG_HIP(hipMemsetAsync(counters.p, 0, 4 * sizeof(uint64_t), st));
         KNNG_HIP(hipEventRecord(ev[0], st));
@@ -464,10 +592,29 @@ struct Builder {
             KNNG_HIP(hipEventElapsedTime(&ms, ev[6], ev[7]));
             stats.split_ms += ms;
             KNNG_TRY(leaf_stage(lv));
+            if (verbose) {
+                uint64_t c[4];
+                KNNG_HIP(hipMemcpy(c, counters.p, sizeof(c), hipMemcpyDeviceToHost));
+                std::fprintf(stderr, "[knng] round %u: split %.2f ms leaves %zu survivors %llu inserts %llu (cumulative leaf %.1f ms)\n",
+                             r, ms, lv.size(), (unsigned long long)c[0], (unsigned long long)c[2],
+                             stats.leaf_ms);
+            }
         }
+        uint64_t leafc[4];
+        KNNG_HIP(hipMemcpy(leafc, counters.p, sizeof(leafc), hipMemcpyDeviceToHost));
+        stats.survivors = leafc[0];
+        stats.inserts = leafc[2];
+        KNNG_HIP(hipMemsetAsync(counters.p, 0, 4 * sizeof(uint64_t), st));
         KNNG_HIP(hipEventRecord(ev[6], st));
         for (uint32_t p = 0; p < prm.non_passes; ++p) KNNG_TRY(non_pass());
         KNNG_HIP(hipEventRecord(ev[7], st));
+        if (verbose && prm.non_passes) {
+            uint64_t c[4];
+            KNNG_HIP(hipMemcpy(c, counters.p, sizeof(c), hipMemcpyDeviceToHost));
+            std::fprintf(stderr, "[knng] non: screened %llu appended %llu overflow %llu\n",
+                         (unsigned long long)c[0], (unsigned long long)c[1],
+                         (unsigned long long)stats.non_overflow);
+        }
         launch_finalize(knn_ids.p, knn_d.p, n, prm.k, out_ids, out_d, st);
         KNNG_HIP(hipGetLastError());
         KNNG_HIP(hipEventRecord(ev[1], st));
@@ -477,10 +624,6 @@ struct Builder {
         stats.non_ms = ms;
         KNNG_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
         stats.total_ms = ms;
-        uint64_t c[4];
-        KNNG_HIP(hipMemcpy(c, counters.p, sizeof(c), hipMemcpyDeviceToHost));
-        stats.survivors = c[0];
-        stats.inserts = c[2];
         return KNNG_OK;
     }
 };
@@ -704,6 +847,8 @@ int knng_non(knng_handle* h, const float* points, uint32_t n, uint32_t d, size_t
     Builder& b = h->b;
     KNNG_TRY(b.bind_points(points, false, n, d, row_stride));
     KNNG_TRY(upload_state(b, ids, dists, false));
+    KNNG_HIP(hipMemsetAsync(b.counters.p, 0, 4 * sizeof(uint64_t), b.st));
+    b.perm_is_tree = false;
     KNNG_TRY(b.non_pass());
     return download_state(b, ids, dists);
 }
@@ -753,30 +898,37 @@ int knng_exact_knn_device(knng_handle* h, const float* d_points, uint32_t n, uin
         ub = b.ubuf.p;
     }
     knng::launch_norms64(b.X, b.ld, b.d, n, b.norms64.p, b.st);
-    KNNG_HIP(hipMemsetAsync(b.qcount.p, 0, (size_t)nq * 4, b.st));
-    KNNG_HIP(hipMemsetAsync(b.ovf.p, 0, 4, b.st));
-    knng::ExactScanArgs a{};
-    a.X = b.X;
-    a.ld = b.ld;
-    a.d = b.d;
-    a.n = n;
-    a.norms = b.norms64.p;
-    a.queries = d_queries;
-    a.nq = nq;
-    a.ub = ub;
-    a.cap = cap;
-    a.count = b.qcount.p;
-    a.cand_d = b.cand_d.p;
-    a.cand_id = b.cand_id.p;
-    const uint32_t qblocks = (nq + 63) / 64;
-    uint32_t slices = std::max(1u, std::min<uint32_t>(2048u / std::max(qblocks, 1u) + 1, (n + 4095) / 4096));
-    knng::launch_exact_scan(a, slices, b.st);
-    knng::launch_exact_select(a, k, d_out_ids, d_out_dists, b.ovf.p, b.st);
+    KNNG_TRY(b.exact_pass(d_queries, nq, k, ub, cap, d_out_ids, d_out_dists));
+    // queries whose bound admitted more than cap points: one more scan over just those, with
+    // a buffer sized from their counts
+    std::vector<uint32_t> cnt(nq);
+    KNNG_HIP(hipMemcpy(cnt.data(), b.qcount.p, (size_t)nq * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> over;
+    uint32_t cap2 = 0;
+    for (uint32_t i = 0; i < nq; ++i)
+        if (cnt[i] > cap) {
+            over.push_back(i);
+            cap2 = std::max(cap2, cnt[i]);
+        }
+    if (over.empty()) return KNNG_OK;
+    if ((double)over.size() * cap2 * 12.0 > 8e9)
+        return fail(KNNG_ERR_OVERFLOW, std::to_string(over.size()) +
+                                           " queries need more candidate memory than allowed");
+    const uint32_t no = (uint32_t)over.size();
+    knng::DevBuf<uint32_t> d_where, d_q2, d_i2;
+    knng::DevBuf<double> d_u2, d_d2;
+    KNNG_TRY(d_where.ensure(no));
+    KNNG_TRY(d_q2.ensure(no));
+    KNNG_TRY(d_u2.ensure(no));
+    KNNG_TRY(d_i2.ensure((size_t)no * k));
+    KNNG_TRY(d_d2.ensure((size_t)no * k));
+    KNNG_HIP(hipMemcpyAsync(d_where.p, over.data(), (size_t)no * 4, hipMemcpyHostToDevice, b.st));
+    knng::launch_gather_u32(d_queries, d_where.p, no, d_q2.p, b.st);
+    knng::launch_gather_f64(ub, d_where.p, no, d_u2.p, b.st);
+    KNNG_TRY(b.exact_pass(d_q2.p, no, k, d_u2.p, cap2, d_i2.p, d_d2.p));
+    knng::launch_exact_scatter(d_where.p, no, k, d_i2.p, d_d2.p, d_out_ids, d_out_dists, b.st);
     KNNG_HIP(hipGetLastError());
-    uint32_t ovf = 0;
-    KNNG_HIP(hipMemcpyAsync(&ovf, b.ovf.p, 4, hipMemcpyDeviceToHost, b.st));
     KNNG_HIP(hipStreamSynchronize(b.st));
-    if (ovf) return fail(KNNG_ERR_OVERFLOW, std::to_string(ovf) + " queries overflowed their candidate buffer");
     return KNNG_OK;
 }
 

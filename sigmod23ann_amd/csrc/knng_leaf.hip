@@ -21,16 +21,12 @@ namespace knng {
 constexpr uint32_t kLeafThreads = 512;
 constexpr uint32_t kLeafWaves = kLeafThreads / 64;
 
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-typedef short short4_t __attribute__((ext_vector_type(4)));
-typedef float float4_t __attribute__((ext_vector_type(4)));
 
-// Row stride of the 16-bit rows in LDS (elements); +8 staggers banks between rows.
-__host__ __device__ inline uint32_t leaf_row_stride(uint32_t d) { return ((d + 15) & ~15u) + 8; }
 
 struct LeafLds {
     uint32_t b16, kp, rs, w16;
     uint32_t off_ids, off_bnd, off_nrm, off_rinv, off_mask, off_rows, off_rowbuf, off_cand, total;
+    uint32_t part_rows;  // FP32 rows that fit the row region at once (phase 2 re-rank)
 };
 
 __host__ __device__ inline LeafLds leaf_lds_layout(uint32_t max_leaf, uint32_t d, uint32_t ld) {
@@ -48,48 +44,12 @@ __host__ __device__ inline LeafLds leaf_lds_layout(uint32_t max_leaf, uint32_t d
     o = (o + 15) & ~15u;
     L.off_rows = o;   o += L.b16 * L.rs * 2;
     o = (o + 15) & ~15u;
+    L.part_rows = (L.b16 * L.rs * 2) / (ld * 4);
     L.off_rowbuf = o; o += kLeafWaves * ld * 4;
     L.off_cand = o;   o += kLeafWaves * L.b16 * 2;
     L.total = (o + 15) & ~15u;
     return L;
 }
-
-__device__ inline uint16_t to_bf16_rne(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
-__device__ inline float wave_sum(float v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-__device__ inline float wave_max(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ inline uint32_t wave_excl_scan(uint32_t v, uint32_t lane) {
-    uint32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    return x - v;
-}
-
-// Margin coefficient c of the exact screen: |approx - exact| <= c * (n_i + n_j).
-// fp16: 2u + u^2 + accumulation (K<=128) + norm and final rounding, u = 2^-11 -> 9.9e-4; bf16:
-// u = 2^-8 -> 7.9e-3. 25% slack on top.
-template <uint32_t HALF>
-struct HalfTraits;
-template <>
-struct HalfTraits<0> {
-    static constexpr float kMargin = 1.25e-3f;
-};
-template <>
-struct HalfTraits<1> {
-    static constexpr float kMargin = 1.0e-2f;
-};
 
 template <uint32_t HALF>
 __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
@@ -215,50 +175,74 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
     __syncthreads();
 
     // ---- phase 2: exact distances for survivors, sorted insert --------------------------------
+    // The screen is done with the 16-bit rows, so their LDS region now holds FP32 rows, in column
+    // parts of part_rows rows when the whole bucket does not fit; each part re-ranks the survivors
+    // whose column lies in it.
+    float* s_f32 = reinterpret_cast<float*>(lds + L.off_rows);
+    const uint32_t ld4 = ld / 4;
     uint32_t n_surv = 0, n_ins = 0;
-    for (uint32_t r = wave; r < b; r += kLeafWaves) {
-        uint32_t w = lane < nstrip ? s_mask[r * L.w16 + lane] : 0;
-        const uint32_t cnt = __popc(w);
-        const uint32_t base = wave_excl_scan(cnt, lane);
-        const uint32_t S = __shfl(base + cnt, 63);
-        if (S == 0) continue;
-        uint32_t o = base;
-        while (w) {
-            const uint32_t bit = __ffs(w) - 1;
-            w &= w - 1;
-            s_cand[o++] = (uint16_t)(16 * lane + bit);
+    for (uint32_t c_lo = 0; c_lo < b; c_lo += L.part_rows) {
+        const uint32_t c_hi = min(b, c_lo + L.part_rows);
+        __syncthreads();
+        for (uint32_t t = tid; t < (c_hi - c_lo) * ld4; t += kLeafThreads) {
+            const uint32_t rr = t / ld4, q = t - rr * ld4;
+            reinterpret_cast<float4*>(s_f32)[t] =
+                reinterpret_cast<const float4*>(a.X + (size_t)s_ids[c_lo + rr] * ld)[q];
         }
-        const uint32_t id_r = s_ids[r];
-        const float* xr = a.X + (size_t)id_r * ld;
-        for (uint32_t j = lane; j < ld; j += 64) s_rowbuf[j] = xr[j];
-        uint32_t* gid = a.knn_ids + (size_t)id_r * k;
-        float* gd = a.knn_d + (size_t)id_r * k;
-        WaveList lst;
-        lst.load(gid, gd, k, lane);
-        bool changed = false;
-        n_surv += S;
-        for (uint32_t c0 = 0; c0 < S; c0 += 64) {
-            float dd = kFltMax;
-            uint32_t cid = kNoId;
-            if (c0 + lane < S) {
-                cid = s_ids[s_cand[c0 + lane]];
-                dd = dist_ref(s_rowbuf, a.X + (size_t)cid * ld, d);
+        __syncthreads();
+        // tiles overlapping [c_lo, c_hi): bits outside the part are masked off
+        for (uint32_t r = wave; r < b; r += kLeafWaves) {
+            uint32_t w = 0;
+            if (lane < nstrip) {
+                w = s_mask[r * L.w16 + lane];
+                const uint32_t t0 = 16 * lane;
+                const uint32_t lo_b = c_lo > t0 ? min(16u, c_lo - t0) : 0u;
+                const uint32_t hi_b = c_hi > t0 ? min(16u, c_hi - t0) : 0u;
+                w &= ((1u << hi_b) - 1u) & ~((1u << lo_b) - 1u);
             }
-            uint64_t acc = __ballot(dd < lst.bound);
-            while (acc) {
-                const uint32_t src = __ffsll((unsigned long long)acc) - 1;
-                acc &= acc - 1;
-                const float nd = __shfl(dd, src);
-                const uint32_t nid = __shfl(cid, src);
-                if (lst.insert(nd, nid, k, lane)) {
-                    changed = true;
-                    ++n_ins;
+            const uint32_t cnt = __popc(w);
+            const uint32_t base = wave_excl_scan(cnt, lane);
+            const uint32_t S = __shfl(base + cnt, 63);
+            if (S == 0) continue;
+            uint32_t o = base;
+            while (w) {
+                const uint32_t bit = __ffs(w) - 1;
+                w &= w - 1;
+                s_cand[o++] = (uint16_t)(16 * lane + bit);
+            }
+            const uint32_t id_r = s_ids[r];
+            const float* xr = a.X + (size_t)id_r * ld;
+            for (uint32_t j = lane; j < ld; j += 64) s_rowbuf[j] = xr[j];
+            uint32_t* gid = a.knn_ids + (size_t)id_r * k;
+            float* gd = a.knn_d + (size_t)id_r * k;
+            WaveList lst;
+            lst.load(gid, gd, k, lane);
+            bool changed = false;
+            n_surv += S;
+            for (uint32_t c0 = 0; c0 < S; c0 += 64) {
+                float dd = kFltMax;
+                uint32_t cid = kNoId;
+                if (c0 + lane < S) {
+                    const uint32_t c = s_cand[c0 + lane];
+                    cid = s_ids[c];
+                    dd = dist_ref(s_rowbuf, s_f32 + (size_t)(c - c_lo) * ld, d);
+                }
+                uint64_t acc = __ballot(dd < lst.bound);
+                while (acc) {
+                    const uint32_t src = __ffsll((unsigned long long)acc) - 1;
+                    acc &= acc - 1;
+                    const float nd = __shfl(dd, src);
+                    const uint32_t nid = __shfl(cid, src);
+                    if (lst.insert(nd, nid, k, lane)) {
+                        changed = true;
+                        ++n_ins;
+                    }
                 }
             }
-        }
-        if (changed) {
-            lst.store(gid, gd, k, lane);
-            if (lane == 0) a.bound[id_r] = lst.bound;
+            if (changed) {
+                lst.store(gid, gd, k, lane);
+                if (lane == 0) a.bound[id_r] = lst.bound;
+            }
         }
     }
     if (a.counters && lane == 0) {

@@ -1,39 +1,373 @@
 // Neighbours-of-neighbours pass (topUpSingle, SolutionKmeans.hpp:480-555) for gfx950.
 //
-// The snapshot copies every point's k id slots (unfilled slots read as id 0, as the reference's
-// zero-initialised current_ids do, KnnSets.hpp:29). Each point p is then offered every id in the
-// snapshot lists of its snapshot neighbours except itself; the reference dedupes candidates in a
-// hash set first, here duplicates reach the list insert, whose id check rejects them (same pair,
-// same exact distance, so the result is the same set).
+// Snapshot: every point's k id slots, sorted by id (the reference sorts too, :495); unfilled slots
+// read as id 0, as the reference's zero-initialised current_ids do (KnnSets.hpp:29). Point p is
+// offered every id in the snapshot lists of its snapshot neighbours except itself.
+//
+// Reverse form (the fast path). The offers (p <- r) with q in S[p] and r in S[q] are grouped by q:
+// one workgroup per q screens all pairs (p in RS[q], r in S[q]) with an FP32 MFMA Gram, drops
+// pairs with r already in S[p] (binary search in p's id-sorted snapshot) and pairs that
+// cannot beat p's bound, re-ranks the rest in exact reference-order FP32 and appends the ones below
+// the bound to p's candidate buffer. A second kernel merges each buffer into its list (the list
+// insert rejects the duplicates that arrive through several q). The pre-pass bound admits a
+// superset of what the reference's live bound admits, so the merged sets are the same. A point whose
+// buffer overflows is redone by the direct kernel.
+//
+// Direct form: one wave per point walks its 2-hop candidates itself (used for overflow and as the
+// small-input path).
 #include "knng_device.hpp"
 #include "knng_kernels.hpp"
 
 namespace knng {
 
-__global__ void k_non_snapshot(const uint32_t* __restrict__ ids, uint32_t* __restrict__ snap,
-                               size_t total) {
+// ---- snapshot sorted by id -------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __restrict__ ids,
+                                                             uint32_t* __restrict__ snap,
+                                                             uint32_t n, uint32_t k) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= n) return;
+    const uint32_t* src = ids + (size_t)p * k;
+    uint32_t v0 = lane < k ? src[lane] : 0xFFFFFFFFu;
+    uint32_t v1 = lane + 64 < k ? src[lane + 64] : 0xFFFFFFFFu;
+    if (lane < k && v0 == kNoId) v0 = 0;
+    if (lane + 64 < k && v1 == kNoId) v1 = 0;
+    // rank = number of (value, slot) pairs before mine
+    uint32_t r0 = 0, r1 = 0;
+    const uint32_t e0 = lane, e1 = lane + 64;
+    for (uint32_t j = 0; j < 64; ++j) {
+        const uint32_t a = __shfl(v0, j), b = __shfl(v1, j);
+        if (j < k) {
+            r0 += (a < v0 || (a == v0 && j < e0));
+            r1 += (a < v1 || (a == v1 && j < e1));
+        }
+        if (j + 64 < k) {
+            r0 += (b < v0 || (b == v0 && j + 64 < e0));
+            r1 += (b < v1 || (b == v1 && j + 64 < e1));
+        }
+    }
+    uint32_t* dst = snap + (size_t)p * k;
+    if (lane < k) dst[r0] = v0;
+    if (lane + 64 < k) dst[r1] = v1;
+}
+
+// ---- reverse lists: RS[q] = { p : q in S[p] } ----------------------------------------------------
+__global__ void k_rs_count(const uint32_t* __restrict__ snap, uint32_t n, uint32_t k,
+                           uint32_t* __restrict__ rcnt) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < total) {
-        const uint32_t v = ids[i];
-        snap[i] = v == kNoId ? 0u : v;
+    if (i >= (size_t)n * k) return;
+    const uint32_t slot = (uint32_t)(i % k);
+    const uint32_t q = snap[i];
+    if (slot > 0 && snap[i - 1] == q) return;  // repeated id 0 of an unfilled list
+    atomicAdd(&rcnt[q], 1u);
+}
+
+__global__ void k_rs_fill(const uint32_t* __restrict__ snap, uint32_t n, uint32_t k,
+                          const uint64_t* __restrict__ roff, uint32_t* __restrict__ rfill,
+                          uint32_t* __restrict__ rs) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * k) return;
+    const uint32_t slot = (uint32_t)(i % k);
+    const uint32_t q = snap[i];
+    if (slot > 0 && snap[i - 1] == q) return;
+    const uint32_t pos = atomicAdd(&rfill[q], 1u);
+    rs[roff[q] + pos] = (uint32_t)(i / k);
+}
+
+// ---- exclusive scan of n u32 counts into n+1 u64 offsets ----------------------------------------
+constexpr uint32_t kScanPer = 16, kScanThreads = 256, kScanBlock = kScanPer * kScanThreads;
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(const uint32_t* __restrict__ in,
+                                                            uint32_t n, uint64_t* __restrict__ sums) {
+    __shared__ uint64_t red[kScanThreads];
+    const size_t b0 = (size_t)blockIdx.x * kScanBlock;
+    uint64_t s = 0;
+    for (uint32_t j = 0; j < kScanPer; ++j) {
+        const size_t i = b0 + (size_t)j * kScanThreads + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t h = kScanThreads / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_bases(uint64_t* __restrict__ sums, uint32_t nb) {
+    __shared__ uint64_t part[1024];
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per, e = min(nb, b + per);
+    uint64_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += sums[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t h = 1; h < 1024; h <<= 1) {
+        const uint64_t v = threadIdx.x >= h ? part[threadIdx.x - h] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint64_t run = part[threadIdx.x] - s;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint64_t v = sums[i];
+        sums[i] = run;
+        run += v;
     }
 }
 
-void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
-                         hipStream_t st) {
-    const size_t tot = (size_t)n * k;
-    if (tot) k_non_snapshot<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(ids, snap, tot);
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __restrict__ in,
+                                                             uint32_t n,
+                                                             const uint64_t* __restrict__ bases,
+                                                             uint64_t* __restrict__ out) {
+    __shared__ uint64_t part[kScanThreads];
+    const size_t b0 = (size_t)blockIdx.x * kScanBlock + (size_t)threadIdx.x * kScanPer;
+    uint32_t v[kScanPer];
+    uint64_t s = 0;
+    for (uint32_t j = 0; j < kScanPer; ++j) {
+        v[j] = b0 + j < n ? in[b0 + j] : 0;
+        s += v[j];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t h = 1; h < kScanThreads; h <<= 1) {
+        const uint64_t x = threadIdx.x >= h ? part[threadIdx.x - h] : 0;
+        __syncthreads();
+        part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint64_t run = bases[blockIdx.x] + part[threadIdx.x] - s;
+    for (uint32_t j = 0; j < kScanPer; ++j) {
+        if (b0 + j < n) out[b0 + j] = run;
+        run += v[j];
+    }
 }
 
-// One wave per point: for each snapshot neighbour q, lanes take q's snapshot ids, compute the exact
-// reference-order distance and offer the ones under the live bound.
+__global__ void k_scan_total(const uint32_t* __restrict__ in, uint32_t n, uint64_t* __restrict__ out) {
+    out[n] = out[n - 1] + in[n - 1];
+}
+
+// ---- reverse screening kernel --------------------------------------------------------------------
+// One workgroup per q: FP32 rows of S[q] and of a 32-row chunk of RS[q] are bulk-loaded into LDS by
+// all threads; an FP32 MFMA Gram (16x16x4) screens every pair with a rigorous margin; survivors not
+// already in p's snapshot are re-ranked exactly from the LDS rows.
+constexpr uint32_t kNrThreads = 256;  // 4 waves: 2 strips of the chunk x 2 halves of S[q]'s tiles
+constexpr uint32_t kNrChunk = 32;     // reverse-neighbour rows per pass
+
+__host__ __device__ inline uint32_t nr_row_stride(uint32_t ld) { return ld % 8 == 0 ? ld + 4 : ld; }
+
+struct NrLds {
+    uint32_t kb16, ldp;
+    uint32_t off_bid, off_bnrm, off_brow;
+    uint32_t off_aid, off_abnd, off_anrm, off_arow, off_asnap, total;
+};
+
+__host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
+    NrLds L;
+    L.kb16 = (k + 15) & ~15u;
+    L.ldp = nr_row_stride(ld);
+    uint32_t o = 0;
+    L.off_bid = o;  o += L.kb16 * 4;
+    L.off_bnrm = o; o += L.kb16 * 4;
+    o = (o + 15) & ~15u;
+    L.off_brow = o; o += L.kb16 * L.ldp * 4;
+    L.off_aid = o;  o += kNrChunk * 4;
+    L.off_abnd = o; o += kNrChunk * 4;
+    L.off_anrm = o; o += kNrChunk * 4;
+    o = (o + 15) & ~15u;
+    L.off_arow = o; o += kNrChunk * L.ldp * 4;
+    L.off_asnap = o; o += kNrChunk * k * 4;
+    L.total = (o + 15) & ~15u;
+    return L;
+}
+
+size_t non_reverse_lds(uint32_t k, uint32_t ld) { return nr_layout(k, ld).total; }
+
+// rows[r] <- X[ids[r]] for r < nr, zero rows up to nrpad; all threads, float4 copies.
+__device__ inline void bulk_rows(const float* __restrict__ X, uint32_t ld, uint32_t ldp,
+                                 const uint32_t* __restrict__ ids, uint32_t nr, uint32_t nrpad,
+                                 float* __restrict__ rows) {
+    const uint32_t ld4 = ld / 4, ldp4 = ldp / 4;
+    for (uint32_t t = threadIdx.x; t < nrpad * ldp4; t += blockDim.x) {
+        const uint32_t r = t / ldp4, q4 = t - r * ldp4;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (r < nr && q4 < ld4) v = reinterpret_cast<const float4*>(X + (size_t)ids[r] * ld)[q4];
+        reinterpret_cast<float4*>(rows)[t] = v;
+    }
+}
+
+// squared norms of LDS rows, one thread per row
+__device__ inline void row_norms(const float* __restrict__ rows, uint32_t ldp, uint32_t d,
+                                 uint32_t nr, float* __restrict__ out) {
+    for (uint32_t r = threadIdx.x; r < nr; r += blockDim.x) {
+        const float* x = rows + r * ldp;
+        float s = 0.f;
+        for (uint32_t j = 0; j < d; ++j) s = __builtin_fmaf(x[j], x[j], s);
+        out[r] = s;
+    }
+}
+
+__global__ __launch_bounds__(kNrThreads) void k_non_reverse(NonRevArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t k = a.k, d = a.d, ld = a.ld;
+    // XCD-aware order: the blocks an XCD runs take one contiguous stretch of this launch's q list
+    const uint32_t per = (a.nq + 7) / 8;
+    const uint32_t idx = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (idx >= a.nq) return;
+    const uint32_t q = a.order[(size_t)idx * a.order_stride];
+    const uint64_t a0 = a.roff[q], a1 = a.roff[q + 1];
+    if (a0 == a1) return;
+    const NrLds L = nr_layout(k, ld);
+    const uint32_t ldp = L.ldp;
+    uint32_t* s_bid = reinterpret_cast<uint32_t*>(lds + L.off_bid);
+    float* s_bnrm = reinterpret_cast<float*>(lds + L.off_bnrm);
+    float* s_brow = reinterpret_cast<float*>(lds + L.off_brow);
+    uint32_t* s_aid = reinterpret_cast<uint32_t*>(lds + L.off_aid);
+    float* s_abnd = reinterpret_cast<float*>(lds + L.off_abnd);
+    float* s_anrm = reinterpret_cast<float*>(lds + L.off_anrm);
+    float* s_arow = reinterpret_cast<float*>(lds + L.off_arow);
+    uint32_t* s_asnap = reinterpret_cast<uint32_t*>(lds + L.off_asnap);
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    for (uint32_t t = tid; t < L.kb16; t += kNrThreads) s_bid[t] = t < k ? a.snap[(size_t)q * k + t] : kNoId;
+    __syncthreads();
+    bulk_rows(a.X, ld, ldp, s_bid, k, L.kb16, s_brow);
+    __syncthreads();
+    row_norms(s_brow, ldp, d, L.kb16, s_bnrm);
+
+    // |approx - exact| <= (K + 8) 2^-23 (n_p + n_r) for the FP32 Gram over K = ld terms
+    const float margin_c = (float)(ld + 8) * (1.0f / 8388608.0f);
+    const uint32_t ksteps = ld / 4;
+    const uint32_t ntile = L.kb16 / 16;
+    const uint32_t strip = wave & 1, th = wave >> 1;
+    const uint32_t t_per = (ntile + 1) / 2;
+    const uint32_t t_lo = th * t_per, t_hi = min(ntile, t_lo + t_per);
+    const uint32_t lrow = lane & 15, lk = lane >> 4;
+    uint32_t n_scr = 0, n_app = 0;
+    for (uint64_t c0 = a0; c0 < a1; c0 += kNrChunk) {
+        const uint32_t na = (uint32_t)min<uint64_t>(kNrChunk, a1 - c0);
+        __syncthreads();
+        for (uint32_t t = tid; t < kNrChunk; t += kNrThreads) {
+            const uint32_t p = t < na ? a.rs[c0 + t] : kNoId;
+            s_aid[t] = p;
+            s_abnd[t] = t < na ? a.bound[p] : 0.f;
+        }
+        __syncthreads();
+        bulk_rows(a.X, ld, ldp, s_aid, na, kNrChunk, s_arow);
+        for (uint32_t t = tid; t < na * k; t += kNrThreads) {
+            const uint32_t r = t / k, j = t - r * k;
+            s_asnap[t] = a.snap[(size_t)s_aid[r] * k + j];
+        }
+        __syncthreads();
+        row_norms(s_arow, ldp, d, kNrChunk, s_anrm);
+        __syncthreads();
+        if (16 * strip < na) {
+            float rn[4], thr0[4], bnd[4];
+            uint32_t rr[4], pid[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                rr[i] = 16 * strip + (lane >> 4) * 4 + i;
+                rn[i] = s_anrm[rr[i]];
+                pid[i] = s_aid[rr[i]];
+                bnd[i] = s_abnd[rr[i]];
+                thr0[i] = bnd[i] * (1.0f + 1.0f / 65536.0f);
+            }
+            const float* arow = s_arow + (16 * strip + lrow) * ldp + lk;
+            for (uint32_t t = t_lo; t < t_hi; ++t) {
+                float4_t acc = {0.f, 0.f, 0.f, 0.f};
+                const float* brow = s_brow + (16 * t + lrow) * ldp + lk;
+                for (uint32_t kk = 0; kk < ksteps; ++kk)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * kk], brow[4 * kk], acc, 0, 0, 0);
+                const uint32_t c = 16 * t + lrow;
+                const float cn = s_bnrm[c];
+                const uint32_t rid = s_bid[c];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float nsum = rn[i] + cn;
+                    const float approx = nsum - 2.0f * acc[i];
+                    if (rr[i] < na && c < k && rid != pid[i] && approx < thr0[i] + margin_c * nsum) {
+                        ++n_scr;
+                        // r already in p's snapshot: the list insert would reject it (id 0 may be
+                        // the padding of an unfilled list, not a member: it is always offered)
+                        const uint32_t* sp = s_asnap + rr[i] * k;
+                        uint32_t lo = 0, hi = k;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (sp[mid] < rid) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        if (rid != 0 && lo < k && sp[lo] == rid) continue;
+                        const float ex = dist_ref(s_brow + c * ldp, s_arow + rr[i] * ldp, d);
+                        if (ex < bnd[i]) {
+                            const uint32_t slot = atomicAdd(&a.ncnt[pid[i]], 1u);
+                            if (slot < a.cap) {
+                                a.nbuf_d[(size_t)pid[i] * a.cap + slot] = ex;
+                                a.nbuf_id[(size_t)pid[i] * a.cap + slot] = rid;
+                            }
+                            ++n_app;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (a.counters) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)n_scr);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[1]), (unsigned long long)n_app);
+    }
+}
+
+// One wave per point: merge its candidate buffer into the list; overflowed points are queued for
+// the direct kernel instead.
+__global__ __launch_bounds__(256) void k_non_merge(NonRevArgs a, uint32_t* __restrict__ ovf_cnt,
+                                                   uint32_t* __restrict__ ovf_list,
+                                                   uint32_t* __restrict__ ovf_flag) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= a.n) return;
+    const uint32_t m = a.ncnt[p];
+    if (m == 0) return;
+    if (m > a.cap) {
+        if (lane == 0 && atomicExch(&ovf_flag[p], 1u) == 0u) ovf_list[atomicAdd(ovf_cnt, 1u)] = p;
+        return;
+    }
+    const uint32_t k = a.k;
+    uint32_t* gid = a.knn_ids + (size_t)p * k;
+    float* gd = a.knn_d + (size_t)p * k;
+    WaveList lst;
+    lst.load(gid, gd, k, lane);
+    bool changed = false;
+    for (uint32_t c0 = 0; c0 < m; c0 += 64) {
+        float dd = kFltMax;
+        uint32_t cid = kNoId;
+        if (c0 + lane < m) {
+            dd = a.nbuf_d[(size_t)p * a.cap + c0 + lane];
+            cid = a.nbuf_id[(size_t)p * a.cap + c0 + lane];
+        }
+        uint64_t acc = __ballot(dd < lst.bound);
+        while (acc) {
+            const uint32_t src = __ffsll((unsigned long long)acc) - 1;
+            acc &= acc - 1;
+            changed |= lst.insert(__shfl(dd, src), __shfl(cid, src), k, lane);
+        }
+    }
+    if (changed) {
+        lst.store(gid, gd, k, lane);
+        if (lane == 0) a.bound[p] = lst.bound;
+    }
+}
+
+// ---- direct kernel ---------------------------------------------------------------------------------
 constexpr uint32_t kNonWaves = 4;
 
-__global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a) {
+__global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a, const uint32_t* plist,
+                                                               uint32_t np) {
     __shared__ __align__(16) float rowbuf[kNonWaves][kMaxDim];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t p = blockIdx.x * kNonWaves + wave;
-    if (p >= a.n) return;
+    const uint32_t w = blockIdx.x * kNonWaves + wave;
+    if (w >= np) return;
+    const uint32_t p = plist ? plist[w] : w;
     const uint32_t k = a.k, ld = a.ld, d = a.d;
     float* xr = rowbuf[wave];
     for (uint32_t j = lane; j < ld; j += 64) xr[j] = a.X[(size_t)p * ld + j];
@@ -45,6 +379,7 @@ __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a) {
     const uint32_t* sp = a.snap + (size_t)p * k;
     for (uint32_t a2 = 0; a2 < k; ++a2) {
         const uint32_t q = sp[a2];
+        if (a2 > 0 && sp[a2 - 1] == q) continue;  // repeated id 0 of an unfilled list
         const uint32_t* sq = a.snap + (size_t)q * k;
         for (uint32_t c0 = 0; c0 < k; c0 += 64) {
             float dd = kFltMax;
@@ -57,9 +392,7 @@ __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a) {
             while (acc) {
                 const uint32_t src = __ffsll((unsigned long long)acc) - 1;
                 acc &= acc - 1;
-                const float nd = __shfl(dd, src);
-                const uint32_t nid = __shfl(cid, src);
-                changed |= lst.insert(nd, nid, k, lane);
+                changed |= lst.insert(__shfl(dd, src), __shfl(cid, src), k, lane);
             }
         }
     }
@@ -69,8 +402,56 @@ __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a) {
     }
 }
 
-void launch_non_direct(const NonArgs& a, hipStream_t st) {
-    if (a.n) k_non_direct<<<(a.n + kNonWaves - 1) / kNonWaves, kNonWaves * 64, 0, st>>>(a);
+// ---- launchers ----------------------------------------------------------------------------------------
+void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
+                         hipStream_t st) {
+    if (n) k_non_snapshot_sorted<<<(n + 3) / 4, 256, 0, st>>>(ids, snap, n, k);
+}
+
+void launch_non_direct(const NonArgs& a, const uint32_t* plist, uint32_t np, hipStream_t st) {
+    if (np) k_non_direct<<<(np + kNonWaves - 1) / kNonWaves, kNonWaves * 64, 0, st>>>(a, plist, np);
+}
+
+size_t scan_scratch_elems(uint32_t n) { return (n + kScanBlock - 1) / kScanBlock + 1; }
+
+void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint64_t* out, uint64_t* scratch,
+                           hipStream_t st) {
+    const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+    if (nb == 0) {
+        hipMemsetAsync(out, 0, sizeof(uint64_t), st);
+        return;
+    }
+    k_scan_sums<<<nb, kScanThreads, 0, st>>>(in, n, scratch);
+    k_scan_bases<<<1, 1024, 0, st>>>(scratch, nb);
+    k_scan_apply<<<nb, kScanThreads, 0, st>>>(in, n, scratch, out);
+    k_scan_total<<<1, 1, 0, st>>>(in, n, out);
+}
+
+void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t* rcnt,
+                          uint64_t* roff, uint64_t* scratch, uint32_t* rfill, uint32_t* rs,
+                          hipStream_t st) {
+    const size_t tot = (size_t)n * k;
+    if (!tot) return;
+    const unsigned g = (unsigned)((tot + 255) / 256);
+    hipMemsetAsync(rcnt, 0, (size_t)n * 4, st);
+    hipMemsetAsync(rfill, 0, (size_t)n * 4, st);
+    k_rs_count<<<g, 256, 0, st>>>(snap, n, k, rcnt);
+    launch_exclusive_scan(rcnt, n, roff, scratch, st);
+    k_rs_fill<<<g, 256, 0, st>>>(snap, n, k, roff, rfill, rs);
+}
+
+void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {
+    if (!a.nq) return;
+    const size_t lds = non_reverse_lds(a.k, a.ld);
+    const uint32_t grid = 8 * ((a.nq + 7) / 8);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_non_reverse),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_non_reverse<<<grid, kNrThreads, lds, st>>>(a);
+}
+
+void launch_non_merge(const NonRevArgs& a, uint32_t* ovf_cnt, uint32_t* ovf_list, uint32_t* ovf_flag,
+                      hipStream_t st) {
+    if (a.n) k_non_merge<<<(a.n + 3) / 4, 256, 0, st>>>(a, ovf_cnt, ovf_list, ovf_flag);
 }
 
 }  // namespace knng

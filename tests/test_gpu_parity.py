@@ -145,14 +145,29 @@ def test_leaf_stage_lowrank_unit_norm():
     assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
 
 
-def test_non_matches_oracle():
+@pytest.mark.parametrize("flags", [0, K.FLAG_DIRECT_NON])
+def test_non_matches_oracle(flags):
     X = mixture(6000, 100, 30, 31)
     k = 20
     kw = dict(k=k, leaf_max=100, small_max=3000, iterations=1, seed=2)
     o_prm = oracle.make_params(**kw)
     perm, ls, lz = oracle.split_round(X, o_prm, 0)
     st = oracle.leaf_stage(X, k, perm, ls, lz)
-    ids, ds = K.Builder(K.params(**kw)).non(X, (st[0], st[1]))
+    ids, ds = K.Builder(K.params(flags=flags, **kw)).non(X, (st[0], st[1]))
+    o_ids, o_ds, _ = oracle.non(X, k, st)
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+
+
+def test_non_k100_lowrank():
+    # k = 100 lists on close unit-norm points: many 2-hop candidates per point, buffer overflow path
+    X = lowrank(8000, 100, 33)
+    k = 100
+    kw = dict(k=k, leaf_max=256, small_max=3000, iterations=1, seed=5)
+    o_prm = oracle.make_params(**kw)
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    st = oracle.leaf_stage(X, k, perm, ls, lz)
+    b = K.Builder(K.params(**kw))
+    ids, ds = b.non(X, (st[0], st[1]))
     o_ids, o_ds, _ = oracle.non(X, k, st)
     assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
 
