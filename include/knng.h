@@ -69,6 +69,9 @@ typedef struct knng_stats {
     uint64_t split_failures;
     uint64_t split_fallbacks;
     uint64_t non_overflow;  /* points whose NoN candidate buffer overflowed (redone directly) */
+    double non_screen_ms;   /* of non_ms: the reverse screening kernel launches */
+    uint64_t non_screen_launches;
+    uint64_t non_screen_bytes; /* rows and snapshots the screening kernel gathers (algorithmic) */
 } knng_stats;
 
 typedef struct knng_handle knng_handle;

@@ -91,6 +91,9 @@ class Stats(C.Structure):
         ("split_failures", C.c_uint64),
         ("split_fallbacks", C.c_uint64),
         ("non_overflow", C.c_uint64),
+        ("non_screen_ms", C.c_double),
+        ("non_screen_launches", C.c_uint64),
+        ("non_screen_bytes", C.c_uint64),
     ]
 
     def as_dict(self):

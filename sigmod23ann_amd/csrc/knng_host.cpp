@@ -162,7 +162,7 @@ struct Builder {
     DevBuf<uint32_t> ovf_flag;
     DevBuf<uint32_t> rcnt, rfill, rsl, ncnt, nbuf_id;
     DevBuf<uint64_t> roff, scan_tmp;
-    DevBuf<float> nbuf_d;
+    DevBuf<float> nbuf_d, norms32;
 
     // generator scratch
     DevBuf<float> gen_a, gen_b, gen_c;
@@ -461,7 +461,7 @@ struct Builder {
         da.knn_ids = knn_ids.p;
         da.knn_d = knn_d.p;
         da.bound = bound.p;
-        const bool reverse = non_reverse_lds(k, ld) <= 160 * 1024 &&
+        const bool reverse = ld <= 128 && non_reverse_lds(k, ld) <= 160 * 1024 &&
                              !(prm.flags & KNNG_FLAG_DIRECT_NON);
         if (!reverse) {
             launch_non_direct(da, nullptr, n, st);
@@ -481,6 +481,18 @@ struct Builder {
         KNNG_TRY(perm.ensure(n));
         if (!perm_is_tree) launch_iota(perm.p, n, st);
         launch_reverse_lists(snap.p, n, k, rcnt.p, roff.p, scan_tmp.p, rfill.p, rsl.p, st);
+        if (verbose) {
+            std::vector<uint32_t> rc(n);
+            KNNG_HIP(hipMemcpyAsync(rc.data(), rcnt.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+            KNNG_HIP(hipStreamSynchronize(st));
+            std::sort(rc.begin(), rc.end());
+            uint64_t tot = 0, big = 0;
+            for (uint32_t v : rc) { tot += v; if (v > 1024) big += v; }
+            std::fprintf(stderr, "[knng] reverse lists: total %llu, p50 %u p99 %u p99.9 %u max %u, "
+                                 "entries in lists > 1024: %llu\n",
+                         (unsigned long long)tot, rc[n / 2], rc[(size_t)n * 99 / 100],
+                         rc[(size_t)n * 999 / 1000], rc[n - 1], (unsigned long long)big);
+        }
         KNNG_HIP(hipMemsetAsync(ovf.p, 0, ((size_t)n + 1) * 4, st));
         KNNG_HIP(hipMemsetAsync(ovf_flag.p, 0, (size_t)n * 4, st));
         NonRevArgs a{};
@@ -501,6 +513,9 @@ struct Builder {
         a.cap = kNonCap;
         a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
         a.counters = counters.p;
+        KNNG_TRY(norms32.ensure(n));
+        launch_row_norms(X, ld, d, n, norms32.p, st);
+        a.norms = norms32.p;
         // Sub-passes over interleaved parts of the tree order (every subs-th q, so a point's q are
         // spread over all of them), each merged before the next: the bounds tighten as the pass
         // goes, which keeps the candidate buffers small.
@@ -517,18 +532,22 @@ struct Builder {
             launch_non_merge(a, ovf.p, ovf.p + 1, ovf_flag.p, st);
             KNNG_HIP(hipEventRecord(ev[4], st));
             KNNG_HIP(hipGetLastError());
-            if (verbose) {
-                KNNG_HIP(hipEventSynchronize(ev[4]));
-                KNNG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
-                t_rev += ms;
-                KNNG_HIP(hipEventElapsedTime(&ms, ev[3], ev[4]));
-                t_merge += ms;
-            }
+            KNNG_HIP(hipEventSynchronize(ev[4]));
+            KNNG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+            t_rev += ms;
+            KNNG_HIP(hipEventElapsedTime(&ms, ev[3], ev[4]));
+            t_merge += ms;
         }
         uint32_t novf = 0;
+        uint64_t entries = 0;
         KNNG_HIP(hipMemcpyAsync(&novf, ovf.p, 4, hipMemcpyDeviceToHost, st));
+        KNNG_HIP(hipMemcpyAsync(&entries, roff.p + n, 8, hipMemcpyDeviceToHost, st));
         KNNG_HIP(hipStreamSynchronize(st));
         stats.non_overflow += novf;
+        stats.non_screen_ms += t_rev;
+        stats.non_screen_launches += subs;
+        // per q: its k snapshot rows; per reverse entry: the row and the id snapshot of p
+        stats.non_screen_bytes += (uint64_t)n * k * 4 * d + entries * (4ull * d + 4ull * k);
         KNNG_HIP(hipEventRecord(ev[2], st));
         if (novf) launch_non_direct(da, ovf.p + 1, novf, st);
         KNNG_HIP(hipEventRecord(ev[3], st));

@@ -116,8 +116,11 @@ struct NonRevArgs {
     uint32_t* nbuf_id;
     uint32_t cap;
     uint32_t half_kind;
+    const float* norms;     // n FP32 squared norms
     uint64_t* counters;     // [0] screened pairs, [1] appended (nullable)
 };
+void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float* out,
+                      hipStream_t st);
 
 // Sorted snapshot (unfilled slots -> id 0).
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,

@@ -152,16 +152,19 @@ __global__ void k_scan_total(const uint32_t* __restrict__ in, uint32_t n, uint64
 // One workgroup per q: FP32 rows of S[q] and of a 32-row chunk of RS[q] are bulk-loaded into LDS by
 // all threads; an FP32 MFMA Gram (16x16x4) screens every pair with a rigorous margin; survivors not
 // already in p's snapshot are re-ranked exactly from the LDS rows.
-constexpr uint32_t kNrThreads = 256;  // 4 waves: 2 strips of the chunk x 2 halves of S[q]'s tiles
+constexpr uint32_t kNrThreads = 512;  // 8 waves: 2 strips of the chunk x 4 groups of S[q]'s tiles
 constexpr uint32_t kNrChunk = 32;     // reverse-neighbour rows per pass
+constexpr uint32_t kNrMaxK4 = 32;     // ld / 4 for ld <= 128 (larger ld takes the direct kernel)
 
 __host__ __device__ inline uint32_t nr_row_stride(uint32_t ld) { return ld % 8 == 0 ? ld + 4 : ld; }
 
 struct NrLds {
-    uint32_t kb16, ldp;
+    uint32_t kb16, ldp, mw;
     uint32_t off_bid, off_bnrm, off_brow;
-    uint32_t off_aid, off_abnd, off_anrm, off_arow, off_asnap, total;
+    uint32_t off_aid, off_abnd, off_anrm, off_arow, off_asnap, off_mask, off_queue, total;
 };
+
+constexpr uint32_t kNrQueue = 256;  // per-wave survivor queue (one tile yields at most 256)
 
 __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
     NrLds L;
@@ -178,6 +181,9 @@ __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
     o = (o + 15) & ~15u;
     L.off_arow = o; o += kNrChunk * L.ldp * 4;
     L.off_asnap = o; o += kNrChunk * k * 4;
+    L.mw = (k + 31) / 32;
+    L.off_mask = o;  o += kNrChunk * L.mw * 4;
+    L.off_queue = o; o += (kNrThreads / 64) * kNrQueue * 2;
     L.total = (o + 15) & ~15u;
     return L;
 }
@@ -194,17 +200,6 @@ __device__ inline void bulk_rows(const float* __restrict__ X, uint32_t ld, uint3
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (r < nr && q4 < ld4) v = reinterpret_cast<const float4*>(X + (size_t)ids[r] * ld)[q4];
         reinterpret_cast<float4*>(rows)[t] = v;
-    }
-}
-
-// squared norms of LDS rows, one thread per row
-__device__ inline void row_norms(const float* __restrict__ rows, uint32_t ldp, uint32_t d,
-                                 uint32_t nr, float* __restrict__ out) {
-    for (uint32_t r = threadIdx.x; r < nr; r += blockDim.x) {
-        const float* x = rows + r * ldp;
-        float s = 0.f;
-        for (uint32_t j = 0; j < d; ++j) s = __builtin_fmaf(x[j], x[j], s);
-        out[r] = s;
     }
 }
 
@@ -228,20 +223,24 @@ __global__ __launch_bounds__(kNrThreads) void k_non_reverse(NonRevArgs a) {
     float* s_anrm = reinterpret_cast<float*>(lds + L.off_anrm);
     float* s_arow = reinterpret_cast<float*>(lds + L.off_arow);
     uint32_t* s_asnap = reinterpret_cast<uint32_t*>(lds + L.off_asnap);
+    uint32_t* s_mask = reinterpret_cast<uint32_t*>(lds + L.off_mask);
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint16_t* s_queue = reinterpret_cast<uint16_t*>(lds + L.off_queue) + wave * kNrQueue;
 
-    for (uint32_t t = tid; t < L.kb16; t += kNrThreads) s_bid[t] = t < k ? a.snap[(size_t)q * k + t] : kNoId;
+    for (uint32_t t = tid; t < L.kb16; t += kNrThreads) {
+        const uint32_t r = t < k ? a.snap[(size_t)q * k + t] : kNoId;
+        s_bid[t] = r;
+        s_bnrm[t] = t < k ? a.norms[r] : 0.f;
+    }
     __syncthreads();
     bulk_rows(a.X, ld, ldp, s_bid, k, L.kb16, s_brow);
-    __syncthreads();
-    row_norms(s_brow, ldp, d, L.kb16, s_bnrm);
 
     // |approx - exact| <= (K + 8) 2^-23 (n_p + n_r) for the FP32 Gram over K = ld terms
     const float margin_c = (float)(ld + 8) * (1.0f / 8388608.0f);
     const uint32_t ksteps = ld / 4;
     const uint32_t ntile = L.kb16 / 16;
     const uint32_t strip = wave & 1, th = wave >> 1;
-    const uint32_t t_per = (ntile + 1) / 2;
+    const uint32_t t_per = (ntile + 3) / 4;
     const uint32_t t_lo = th * t_per, t_hi = min(ntile, t_lo + t_per);
     const uint32_t lrow = lane & 15, lk = lane >> 4;
     uint32_t n_scr = 0, n_app = 0;
@@ -252,6 +251,7 @@ __global__ __launch_bounds__(kNrThreads) void k_non_reverse(NonRevArgs a) {
             const uint32_t p = t < na ? a.rs[c0 + t] : kNoId;
             s_aid[t] = p;
             s_abnd[t] = t < na ? a.bound[p] : 0.f;
+            s_anrm[t] = t < na ? a.norms[p] : 0.f;
         }
         __syncthreads();
         bulk_rows(a.X, ld, ldp, s_aid, na, kNrChunk, s_arow);
@@ -259,61 +259,91 @@ __global__ __launch_bounds__(kNrThreads) void k_non_reverse(NonRevArgs a) {
             const uint32_t r = t / k, j = t - r * k;
             s_asnap[t] = a.snap[(size_t)s_aid[r] * k + j];
         }
+        for (uint32_t t = tid; t < kNrChunk * L.mw; t += kNrThreads) s_mask[t] = 0;
         __syncthreads();
-        row_norms(s_arow, ldp, d, kNrChunk, s_anrm);
+        // member[i][j]: S[q][j] is already in S[p_i] (or is p_i itself): those offers would be
+        // rejected by the list's id check. Id 0 may be the padding of an unfilled list rather than
+        // a member, so it is never marked.
+        for (uint32_t t = tid; t < na * k; t += kNrThreads) {
+            const uint32_t i = t / k, j = t - i * k;
+            const uint32_t rid = s_bid[j];
+            const uint32_t* sp = s_asnap + i * k;
+            uint32_t lo = 0, hi = k;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sp[mid] < rid) lo = mid + 1;
+                else hi = mid;
+            }
+            const bool mem = (rid != 0 && lo < k && sp[lo] == rid) || rid == s_aid[i];
+            if (mem) atomicOr(&s_mask[i * L.mw + (j >> 5)], 1u << (j & 31));
+        }
         __syncthreads();
         if (16 * strip < na) {
-            float rn[4], thr0[4], bnd[4];
-            uint32_t rr[4], pid[4];
+            float rn[4], thr0[4];
+            uint32_t rr[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 rr[i] = 16 * strip + (lane >> 4) * 4 + i;
                 rn[i] = s_anrm[rr[i]];
-                pid[i] = s_aid[rr[i]];
-                bnd[i] = s_abnd[rr[i]];
-                thr0[i] = bnd[i] * (1.0f + 1.0f / 65536.0f);
+                thr0[i] = s_abnd[rr[i]] * (1.0f + 1.0f / 65536.0f);
             }
             const float* arow = s_arow + (16 * strip + lrow) * ldp + lk;
+            float af[kNrMaxK4];
+#pragma unroll
+            for (uint32_t kk = 0; kk < kNrMaxK4; ++kk) af[kk] = kk < ksteps ? arow[4 * kk] : 0.f;
             for (uint32_t t = t_lo; t < t_hi; ++t) {
                 float4_t acc = {0.f, 0.f, 0.f, 0.f};
                 const float* brow = s_brow + (16 * t + lrow) * ldp + lk;
-                for (uint32_t kk = 0; kk < ksteps; ++kk)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * kk], brow[4 * kk], acc, 0, 0, 0);
+                float bf[kNrMaxK4];
+#pragma unroll
+                for (uint32_t kk = 0; kk < kNrMaxK4; ++kk) bf[kk] = kk < ksteps ? brow[4 * kk] : 0.f;
+#pragma unroll
+                for (uint32_t kk = 0; kk < kNrMaxK4; ++kk)
+                    if (kk < ksteps) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[kk], bf[kk], acc, 0, 0, 0);
                 const uint32_t c = 16 * t + lrow;
-                const float cn = s_bnrm[c];
-                const uint32_t rid = s_bid[c];
+                const float cn = c < k ? s_bnrm[c] : 0.f;
+                // compact the survivors of this tile into the wave's queue as (row << 7 | col)
+                uint32_t qn = 0;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float nsum = rn[i] + cn;
                     const float approx = nsum - 2.0f * acc[i];
-                    if (rr[i] < na && c < k && rid != pid[i] && approx < thr0[i] + margin_c * nsum) {
-                        ++n_scr;
-                        // r already in p's snapshot: the list insert would reject it (id 0 may be
-                        // the padding of an unfilled list, not a member: it is always offered)
-                        const uint32_t* sp = s_asnap + rr[i] * k;
-                        uint32_t lo = 0, hi = k;
-                        while (lo < hi) {
-                            const uint32_t mid = (lo + hi) >> 1;
-                            if (sp[mid] < rid) lo = mid + 1;
-                            else hi = mid;
-                        }
-                        if (rid != 0 && lo < k && sp[lo] == rid) continue;
-                        const float ex = dist_ref(s_brow + c * ldp, s_arow + rr[i] * ldp, d);
-                        if (ex < bnd[i]) {
-                            const uint32_t slot = atomicAdd(&a.ncnt[pid[i]], 1u);
+                    bool ok = rr[i] < na && c < k && approx < thr0[i] + margin_c * nsum;
+                    if (ok) ok = !((s_mask[rr[i] * L.mw + (c >> 5)] >> (c & 31)) & 1u);
+                    const uint64_t bal = __ballot(ok);
+                    if (ok) {
+                        const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
+                        s_queue[pos] = (uint16_t)((rr[i] << 7) | c);
+                    }
+                    qn += __popcll(bal);
+                }
+                n_scr += qn;
+                __builtin_amdgcn_wave_barrier();
+                // exact re-rank of the queue, one survivor per lane
+                for (uint32_t e0 = 0; e0 < qn; e0 += 64) {
+                    if (e0 + lane < qn) {
+                        const uint32_t ent = s_queue[e0 + lane];
+                        const uint32_t i = ent >> 7, cc = ent & 127;
+                        const float ex = dist_ref(s_brow + cc * ldp, s_arow + i * ldp, d);
+                        if (ex < s_abnd[i]) {
+                            const uint32_t pid = s_aid[i];
+                            const uint32_t slot = atomicAdd(&a.ncnt[pid], 1u);
                             if (slot < a.cap) {
-                                a.nbuf_d[(size_t)pid[i] * a.cap + slot] = ex;
-                                a.nbuf_id[(size_t)pid[i] * a.cap + slot] = rid;
+                                a.nbuf_d[(size_t)pid * a.cap + slot] = ex;
+                                a.nbuf_id[(size_t)pid * a.cap + slot] = s_bid[cc];
                             }
                             ++n_app;
                         }
                     }
                 }
+                __builtin_amdgcn_wave_barrier();
             }
         }
     }
     if (a.counters) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)n_scr);
+        // n_scr is wave-uniform, n_app per lane
+        if (lane == 0)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)n_scr);
         atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[1]), (unsigned long long)n_app);
     }
 }
@@ -400,6 +430,21 @@ __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a, const 
         lst.store(gid, gd, k, lane);
         if (lane == 0) a.bound[p] = lst.bound;
     }
+}
+
+__global__ void k_row_norms(const float* __restrict__ X, uint32_t ld, uint32_t d, uint32_t n,
+                            float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* x = X + (size_t)i * ld;
+    float s = 0.f;
+    for (uint32_t j = 0; j < d; ++j) s = __builtin_fmaf(x[j], x[j], s);
+    out[i] = s;
+}
+
+void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float* out,
+                      hipStream_t st) {
+    if (n) k_row_norms<<<(n + 255) / 256, 256, 0, st>>>(X, ld, d, n, out);
 }
 
 // ---- launchers ----------------------------------------------------------------------------------------
