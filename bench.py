@@ -115,7 +115,10 @@ def main():
             f"({time.perf_counter() - t0:.1f} s)")
 
     # ---- roofline of the dominant kernel -----------------------------------------------------
-    # Algorithmic bytes/flops per launch (stated in DESIGN.md) over the HIP-event launch time.
+    # Algorithmic bytes/flops per launch over the HIP-event launch time:
+    #   side test: 4d + 4 + 1 bytes per position (row, permutation entry, flag byte)
+    #   leaf: d * b(b-1) flops per bucket of b points (unique pairs x 2d)
+    #   NoN screen: 4dk bytes per q (rows of its list) + (4d + 4k) per reverse entry (row, snapshot)
     cands = []
     side_bytes = st["side_rows"] * (4 * d + 4 + 1)          # row, perm entry, flag byte
     if st["side_ms"] > 0:

@@ -216,14 +216,13 @@ struct HalfTraits<1> {
     static constexpr float kMargin = 1.0e-2f;
 };
 
-// Converts one row (d <= 128) centred on mu into the 16-bit screening layout; lane j holds dims j
-// and j + 64. Returns the FP32 squared norm of the centred row and the inverse power-of-two scale.
+// Converts one centred row (d <= 128; lane j holds dims j and j + 64) into the 16-bit screening
+// layout: scaled by a power of two so that the largest |value| is below 2^14. Returns the FP32
+// squared norm of the centred row and the inverse scale.
 template <uint32_t HALF>
-__device__ inline void screen_row(const float* __restrict__ x, float mu0, float mu1, uint32_t d,
-                                  uint32_t rs, uint16_t* __restrict__ hrow, uint32_t lane,
-                                  float& nrm, float& rinv) {
-    float v0 = lane < d ? x[lane] - mu0 : 0.f;
-    float v1 = lane + 64 < d ? x[lane + 64] - mu1 : 0.f;
+__device__ inline void screen_vals(float v0, float v1, uint32_t d, uint32_t rs,
+                                   uint16_t* __restrict__ hrow, uint32_t lane, float& nrm,
+                                   float& rinv) {
     const float m = wave_max(fmaxf(fabsf(v0), fabsf(v1)));
     nrm = wave_sum(__builtin_fmaf(v0, v0, v1 * v1));
     int e = 0;

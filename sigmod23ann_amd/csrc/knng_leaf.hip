@@ -51,7 +51,9 @@ __host__ __device__ inline LeafLds leaf_lds_layout(uint32_t max_leaf, uint32_t d
     return L;
 }
 
-template <uint32_t HALF>
+// KS = (d rounded up to 16) / 16 MFMA k-steps, a compile-time constant so the k-loop unrolls
+// without guards and its LDS loads pipeline.
+template <uint32_t HALF, uint32_t KS>
 __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const LeafDesc lf = a.leaves[blockIdx.x];
@@ -79,41 +81,38 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
     }
     __syncthreads();
     const float* mu = a.X + (size_t)s_ids[0] * ld;
-    for (uint32_t r = wave; r < b16; r += kLeafWaves) {
-        uint16_t* hrow = s_rows + r * L.rs;
-        if (r >= b) {
-            for (uint32_t j = lane; j < L.rs; j += 64) hrow[j] = 0;
-            if (lane == 0) {
-                s_nrm[r] = 0.f;
-                s_rinv[r] = 0.f;
+    const float mu0 = lane < d ? mu[lane] : 0.f;
+    const float mu1 = lane + 64 < d ? mu[lane + 64] : 0.f;
+    // four rows per wave at a time: their loads are in flight together
+    for (uint32_t r0 = wave * 4; r0 < b16; r0 += kLeafWaves * 4) {
+        float x0[4], x1[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t r = r0 + u;
+            const float* x = a.X + (size_t)s_ids[r < b ? r : 0] * ld;
+            x0[u] = lane < d ? x[lane] : 0.f;
+            x1[u] = lane + 64 < d ? x[lane + 64] : 0.f;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t r = r0 + u;
+            if (r >= b16) break;
+            uint16_t* hrow = s_rows + r * L.rs;
+            if (r >= b) {
+                for (uint32_t j = lane; j < L.rs; j += 64) hrow[j] = 0;
+                if (lane == 0) {
+                    s_nrm[r] = 0.f;
+                    s_rinv[r] = 0.f;
+                }
+                continue;
             }
-            continue;
-        }
-        const float* x = a.X + (size_t)s_ids[r] * ld;
-        float v0 = lane < d ? x[lane] - mu[lane] : 0.f;
-        float v1 = lane + 64 < d ? x[lane + 64] - mu[lane + 64] : 0.f;
-        const float m = wave_max(fmaxf(fabsf(v0), fabsf(v1)));
-        const float nrm = wave_sum(__builtin_fmaf(v0, v0, v1 * v1));
-        int e = 0;
-        if (m > 0.f) frexpf(m, &e);
-        const int sh = 14 - e;  // |v| * 2^sh < 2^14
-        const float sc = ldexpf(1.0f, sh);
-        v0 *= sc;
-        v1 *= sc;
-        uint16_t h0, h1;
-        if (HALF == 0) {
-            _Float16 f0 = (_Float16)v0, f1 = (_Float16)v1;
-            h0 = __builtin_bit_cast(uint16_t, f0);
-            h1 = __builtin_bit_cast(uint16_t, f1);
-        } else {
-            h0 = to_bf16_rne(v0);
-            h1 = to_bf16_rne(v1);
-        }
-        hrow[lane] = lane < d ? h0 : 0;
-        if (lane + 64 < L.rs) hrow[lane + 64] = lane + 64 < d ? h1 : 0;
-        if (lane == 0) {
-            s_nrm[r] = nrm;
-            s_rinv[r] = ldexpf(1.0f, -sh);
+            float nrm, rinv;
+            screen_vals<HALF>(lane < d ? x0[u] - mu0 : 0.f, lane + 64 < d ? x1[u] - mu1 : 0.f, d,
+                              L.rs, hrow, lane, nrm, rinv);
+            if (lane == 0) {
+                s_nrm[r] = nrm;
+                s_rinv[r] = rinv;
+            }
         }
     }
     __syncthreads();
@@ -121,14 +120,12 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
     // ---- phase 1: screening Gram --------------------------------------------------------------
     const bool exact = a.screen == kScreenExact;
     const float margin_c = HalfTraits<HALF>::kMargin;
-    const uint32_t ksteps = L.kp / 16;
     const uint32_t lrow = lane & 15, lk = (lane >> 4) * 4;
     for (uint32_t s = wave; s < nstrip; s += kLeafWaves) {
-        half4_t af[8];
+        half4_t af[KS];
 #pragma unroll
-        for (uint32_t kk = 0; kk < 8; ++kk)
-            if (kk < ksteps)
-                af[kk] = *reinterpret_cast<const half4_t*>(s_rows + (16 * s + lrow) * L.rs + kk * 16 + lk);
+        for (uint32_t kk = 0; kk < KS; ++kk)
+            af[kk] = *reinterpret_cast<const half4_t*>(s_rows + (16 * s + lrow) * L.rs + kk * 16 + lk);
         float rn[4], rinv[4], thr_base[4];
         uint32_t rr[4];
 #pragma unroll
@@ -140,19 +137,13 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
             thr_base[i] = exact ? bnd * (1.0f + 1.0f / 65536.0f) : bnd;
         }
         for (uint32_t t = 0; t < nstrip; ++t) {
+            half4_t bf[KS];
+#pragma unroll
+            for (uint32_t kk = 0; kk < KS; ++kk)
+                bf[kk] = *reinterpret_cast<const half4_t*>(s_rows + (16 * t + lrow) * L.rs + kk * 16 + lk);
             float4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (uint32_t kk = 0; kk < 8; ++kk) {
-                if (kk >= ksteps) break;
-                const half4_t bf =
-                    *reinterpret_cast<const half4_t*>(s_rows + (16 * t + lrow) * L.rs + kk * 16 + lk);
-                if (HALF == 0)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(af[kk], bf, acc, 0, 0, 0);
-                else
-                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-                        __builtin_bit_cast(short4_t, af[kk]), __builtin_bit_cast(short4_t, bf), acc,
-                        0, 0, 0);
-            }
+            for (uint32_t kk = 0; kk < KS; ++kk) acc = mfma16(af[kk], bf[kk], acc, HALF);
             const uint32_t c = 16 * t + lrow;
             const float cn = s_nrm[c], cinv = s_rinv[c];
             uint64_t bits[4];
@@ -190,8 +181,22 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
                 reinterpret_cast<const float4*>(a.X + (size_t)s_ids[c_lo + rr] * ld)[q];
         }
         __syncthreads();
-        // tiles overlapping [c_lo, c_hi): bits outside the part are masked off
+        // tiles overlapping [c_lo, c_hi): bits outside the part are masked off. The next row's
+        // list and FP32 row are fetched while the current row is processed.
+        WaveList nxt;
+        float nx0 = 0.f, nx1 = 0.f;
+        auto prefetch = [&](uint32_t rr) {
+            const uint32_t id = s_ids[rr];
+            nxt.load(a.knn_ids + (size_t)id * k, a.knn_d + (size_t)id * k, k, lane);
+            const float* x = a.X + (size_t)id * ld;
+            nx0 = lane < ld ? x[lane] : 0.f;
+            nx1 = lane + 64 < ld ? x[lane + 64] : 0.f;
+        };
+        if (wave < b) prefetch(wave);
         for (uint32_t r = wave; r < b; r += kLeafWaves) {
+            WaveList lst = nxt;
+            const float x0 = nx0, x1 = nx1;
+            if (r + kLeafWaves < b) prefetch(r + kLeafWaves);
             uint32_t w = 0;
             if (lane < nstrip) {
                 w = s_mask[r * L.w16 + lane];
@@ -210,13 +215,10 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
                 w &= w - 1;
                 s_cand[o++] = (uint16_t)(16 * lane + bit);
             }
+            if (lane < ld) s_rowbuf[lane] = x0;
+            if (lane + 64 < ld) s_rowbuf[lane + 64] = x1;
+            __builtin_amdgcn_wave_barrier();
             const uint32_t id_r = s_ids[r];
-            const float* xr = a.X + (size_t)id_r * ld;
-            for (uint32_t j = lane; j < ld; j += 64) s_rowbuf[j] = xr[j];
-            uint32_t* gid = a.knn_ids + (size_t)id_r * k;
-            float* gd = a.knn_d + (size_t)id_r * k;
-            WaveList lst;
-            lst.load(gid, gd, k, lane);
             bool changed = false;
             n_surv += S;
             for (uint32_t c0 = 0; c0 < S; c0 += 64) {
@@ -240,9 +242,10 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
                 }
             }
             if (changed) {
-                lst.store(gid, gd, k, lane);
+                lst.store(a.knn_ids + (size_t)id_r * k, a.knn_d + (size_t)id_r * k, k, lane);
                 if (lane == 0) a.bound[id_r] = lst.bound;
             }
+            __builtin_amdgcn_wave_barrier();
         }
     }
     if (a.counters && lane == 0) {
@@ -255,18 +258,32 @@ size_t leaf_lds_bytes(uint32_t max_leaf, uint32_t d, uint32_t ld) {
     return leaf_lds_layout(max_leaf, d, ld).total;
 }
 
+template <uint32_t HALF, uint32_t KS>
+static void launch_leaf_t(const LeafArgs& a, size_t lds, hipStream_t st) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf<HALF, KS>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_leaf<HALF, KS><<<a.nleaves, kLeafThreads, lds, st>>>(a);
+}
+
+template <uint32_t HALF>
+static void launch_leaf_h(const LeafArgs& a, size_t lds, hipStream_t st) {
+    switch ((a.d + 15) / 16) {
+        case 1: launch_leaf_t<HALF, 1>(a, lds, st); break;
+        case 2: launch_leaf_t<HALF, 2>(a, lds, st); break;
+        case 3: launch_leaf_t<HALF, 3>(a, lds, st); break;
+        case 4: launch_leaf_t<HALF, 4>(a, lds, st); break;
+        case 5: launch_leaf_t<HALF, 5>(a, lds, st); break;
+        case 6: launch_leaf_t<HALF, 6>(a, lds, st); break;
+        case 7: launch_leaf_t<HALF, 7>(a, lds, st); break;
+        default: launch_leaf_t<HALF, 8>(a, lds, st); break;
+    }
+}
+
 void launch_leaf(const LeafArgs& a, hipStream_t st) {
     if (!a.nleaves) return;
     const size_t lds = leaf_lds_bytes(a.max_leaf, a.d, a.ld);
-    if (a.half_kind == 0) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf<0>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        k_leaf<0><<<a.nleaves, kLeafThreads, lds, st>>>(a);
-    } else {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf<1>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        k_leaf<1><<<a.nleaves, kLeafThreads, lds, st>>>(a);
-    }
+    if (a.half_kind == 0) launch_leaf_h<0>(a, lds, st);
+    else launch_leaf_h<1>(a, lds, st);
 }
 
 // ---- top-k state helpers ----------------------------------------------------------------------

@@ -149,22 +149,29 @@ __global__ void k_scan_total(const uint32_t* __restrict__ in, uint32_t n, uint64
 }
 
 // ---- reverse screening kernel --------------------------------------------------------------------
-// One workgroup per q: FP32 rows of S[q] and of a 32-row chunk of RS[q] are bulk-loaded into LDS by
-// all threads; an FP32 MFMA Gram (16x16x4) screens every pair with a rigorous margin; survivors not
-// already in p's snapshot are re-ranked exactly from the LDS rows.
+// One workgroup per q. FP32 rows of S[q] and 32-row chunks of RS[q] sit in LDS; an FP32 MFMA Gram
+// (16x16x4) screens every pair with a rigorous margin; pairs with r already in S[p] are masked by
+// a per-chunk membership bitmap; the survivors queue up per wave and are re-ranked exactly, eight
+// lanes per survivor (lane j owns the reference's accumulator j, then the lanes are summed in
+// order). The next chunk's rows and snapshots are fetched into registers while the current chunk
+// is screened.
 constexpr uint32_t kNrThreads = 512;  // 8 waves: 2 strips of the chunk x 4 groups of S[q]'s tiles
-constexpr uint32_t kNrChunk = 32;     // reverse-neighbour rows per pass
+constexpr uint32_t kNrChunk = 32;     // reverse-neighbour rows per chunk
 constexpr uint32_t kNrMaxK4 = 32;     // ld / 4 for ld <= 128 (larger ld takes the direct kernel)
+constexpr uint32_t kNrIds = 512;      // reverse-neighbour ids staged at a time
+constexpr uint32_t kNrQueue = 320;    // per-wave survivor queue; flushed before it could overflow
+constexpr uint32_t kNrPfRows = (kNrChunk * (kNrMaxK4 + 1) + kNrThreads - 1) / kNrThreads;  // float4 per thread (padded stride <= 132 floats)
+constexpr uint32_t kNrPfSnap = kNrChunk * 128 / kNrThreads;        // snapshot ids per thread
 
 __host__ __device__ inline uint32_t nr_row_stride(uint32_t ld) { return ld % 8 == 0 ? ld + 4 : ld; }
 
 struct NrLds {
-    uint32_t kb16, ldp, mw;
-    uint32_t off_bid, off_bnrm, off_brow;
-    uint32_t off_aid, off_abnd, off_anrm, off_arow, off_asnap, off_mask, off_queue, total;
+    uint32_t kb16, ldp;
+    uint32_t off_bid, off_bnrm, off_brow, off_hkey, off_hval, off_rsid;
+    uint32_t off_aid, off_abnd, off_anrm, off_arow, off_mask, off_queue, total;
 };
 
-constexpr uint32_t kNrQueue = 256;  // per-wave survivor queue (one tile yields at most 256)
+constexpr uint32_t kNrHash = 256;  // open-addressing table of S(q)'s ids (<= 128 keys)
 
 __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
     NrLds L;
@@ -175,14 +182,15 @@ __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
     L.off_bnrm = o; o += L.kb16 * 4;
     o = (o + 15) & ~15u;
     L.off_brow = o; o += L.kb16 * L.ldp * 4;
+    L.off_hkey = o; o += kNrHash * 4;
+    L.off_hval = o; o += kNrHash * 4;
+    L.off_rsid = o; o += kNrIds * 4;
     L.off_aid = o;  o += kNrChunk * 4;
     L.off_abnd = o; o += kNrChunk * 4;
     L.off_anrm = o; o += kNrChunk * 4;
     o = (o + 15) & ~15u;
     L.off_arow = o; o += kNrChunk * L.ldp * 4;
-    L.off_asnap = o; o += kNrChunk * k * 4;
-    L.mw = (k + 31) / 32;
-    L.off_mask = o;  o += kNrChunk * L.mw * 4;
+    L.off_mask = o;  o += kNrChunk * 4 * 4;
     L.off_queue = o; o += (kNrThreads / 64) * kNrQueue * 2;
     L.total = (o + 15) & ~15u;
     return L;
@@ -190,20 +198,61 @@ __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
 
 size_t non_reverse_lds(uint32_t k, uint32_t ld) { return nr_layout(k, ld).total; }
 
-// rows[r] <- X[ids[r]] for r < nr, zero rows up to nrpad; all threads, float4 copies.
-__device__ inline void bulk_rows(const float* __restrict__ X, uint32_t ld, uint32_t ldp,
-                                 const uint32_t* __restrict__ ids, uint32_t nr, uint32_t nrpad,
-                                 float* __restrict__ rows) {
-    const uint32_t ld4 = ld / 4, ldp4 = ldp / 4;
-    for (uint32_t t = threadIdx.x; t < nrpad * ldp4; t += blockDim.x) {
-        const uint32_t r = t / ldp4, q4 = t - r * ldp4;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (r < nr && q4 < ld4) v = reinterpret_cast<const float4*>(X + (size_t)ids[r] * ld)[q4];
-        reinterpret_cast<float4*>(rows)[t] = v;
+__device__ inline uint32_t nr_hash(uint32_t id) { return (id * 0x9E3779B1u) >> 24; }
+
+// Reference-order distance of two rows, computed by the 8 lanes of a lane group: lane j of the
+// group accumulates dims j, j+8, ... below floor(d/8)*8 with fused multiply-add; the 8 partial sums
+// are added in lane order into 0.0f; the tail dims are added with fused multiply-add. Every lane
+// of the group returns the same value, equal to dist_ref(a, b, d).
+__device__ inline float dist_ref_lane8(const float* __restrict__ a, const float* __restrict__ b,
+                                       uint32_t d, uint32_t lane) {
+    const uint32_t main = d & ~7u, l8 = lane & 7u, base = lane & ~7u;
+    float acc = 0.f;
+    for (uint32_t i = l8; i < main; i += 8) {
+        const float t = a[i] - b[i];
+        acc = __builtin_fmaf(t, t, acc);
     }
+    float s = 0.0f;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) s = s + __shfl(acc, base + j);
+    for (uint32_t i = main; i < d; ++i) {
+        const float t = a[i] - b[i];
+        s = __builtin_fmaf(t, t, s);
+    }
+    return s;
 }
 
-__global__ __launch_bounds__(kNrThreads) void k_non_reverse(NonRevArgs a) {
+// FP32 Gram of a 16x16 tile over K = 4 KS dims, four independent accumulation chains (the
+// screen's error bound holds for any summation order). KS is a compile-time constant for the
+// common widths; KS = 0 is the generic runtime loop.
+template <uint32_t KS>
+__device__ inline float4_t gram_tile_f32(const float* __restrict__ arow,
+                                         const float* __restrict__ brow, uint32_t ksteps) {
+    float4_t c0 = {0.f, 0.f, 0.f, 0.f}, c1 = c0, c2 = c0, c3 = c0;
+    if constexpr (KS > 0) {
+        float av[KS], bv[KS];
+#pragma unroll
+        for (uint32_t kk = 0; kk < KS; ++kk) {
+            av[kk] = arow[4 * kk];
+            bv[kk] = brow[4 * kk];
+        }
+#pragma unroll
+        for (uint32_t kk = 0; kk < KS; ++kk) {
+            if (kk % 4 == 0) c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], bv[kk], c0, 0, 0, 0);
+            if (kk % 4 == 1) c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], bv[kk], c1, 0, 0, 0);
+            if (kk % 4 == 2) c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], bv[kk], c2, 0, 0, 0);
+            if (kk % 4 == 3) c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], bv[kk], c3, 0, 0, 0);
+        }
+    } else {
+        for (uint32_t kk = 0; kk < ksteps; ++kk)
+            c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * kk], brow[4 * kk], c0, 0, 0, 0);
+    }
+    return (c0 + c1) + (c2 + c3);
+}
+
+template <uint32_t KS>
+__global__ __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_non_reverse(NonRevArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t k = a.k, d = a.d, ld = a.ld;
     // XCD-aware order: the blocks an XCD runs take one contiguous stretch of this launch's q list
@@ -214,26 +263,114 @@ __global__ __launch_bounds__(kNrThreads) void k_non_reverse(NonRevArgs a) {
     const uint64_t a0 = a.roff[q], a1 = a.roff[q + 1];
     if (a0 == a1) return;
     const NrLds L = nr_layout(k, ld);
-    const uint32_t ldp = L.ldp;
+    const uint32_t ldp = L.ldp, ld4 = ld / 4, ldp4 = ldp / 4;
     uint32_t* s_bid = reinterpret_cast<uint32_t*>(lds + L.off_bid);
     float* s_bnrm = reinterpret_cast<float*>(lds + L.off_bnrm);
     float* s_brow = reinterpret_cast<float*>(lds + L.off_brow);
+    uint32_t* s_hkey = reinterpret_cast<uint32_t*>(lds + L.off_hkey);
+    uint32_t* s_hval = reinterpret_cast<uint32_t*>(lds + L.off_hval);
+    uint32_t* s_rsid = reinterpret_cast<uint32_t*>(lds + L.off_rsid);
     uint32_t* s_aid = reinterpret_cast<uint32_t*>(lds + L.off_aid);
     float* s_abnd = reinterpret_cast<float*>(lds + L.off_abnd);
     float* s_anrm = reinterpret_cast<float*>(lds + L.off_anrm);
     float* s_arow = reinterpret_cast<float*>(lds + L.off_arow);
-    uint32_t* s_asnap = reinterpret_cast<uint32_t*>(lds + L.off_asnap);
     uint32_t* s_mask = reinterpret_cast<uint32_t*>(lds + L.off_mask);
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint16_t* s_queue = reinterpret_cast<uint16_t*>(lds + L.off_queue) + wave * kNrQueue;
+    const uint32_t nrs = (uint32_t)(a1 - a0);
+    // staging map: 16 threads per chunk row
+    const uint32_t sr = tid >> 4, sc = tid & 15;
 
+    // ---- S(q): ids, norms, hash table id -> slot, rows ---------------------------------------
+    for (uint32_t t = tid; t < kNrHash; t += kNrThreads) s_hkey[t] = kNoId;
     for (uint32_t t = tid; t < L.kb16; t += kNrThreads) {
         const uint32_t r = t < k ? a.snap[(size_t)q * k + t] : kNoId;
         s_bid[t] = r;
         s_bnrm[t] = t < k ? a.norms[r] : 0.f;
     }
+    uint32_t g0 = 0;  // first reverse entry of the staged ids group
+    for (uint32_t t = tid; t < min(nrs, kNrIds); t += kNrThreads) s_rsid[t] = a.rs[a0 + t];
     __syncthreads();
-    bulk_rows(a.X, ld, ldp, s_bid, k, L.kb16, s_brow);
+    // Id 0 may be the padding of an unfilled list rather than a member, so it is never hashed:
+    // offers of id 0 always go through (the list's id check settles them).
+    if (tid < k) {
+        const uint32_t id = s_bid[tid];
+        if (id != 0) {
+            uint32_t h = nr_hash(id);
+            while (atomicCAS(&s_hkey[h], kNoId, id) != kNoId) h = (h + 1) & (kNrHash - 1);
+            s_hval[h] = tid;
+        }
+    }
+    for (uint32_t t = tid; t < L.kb16 * ldp4; t += kNrThreads) {
+        const uint32_t r = t / ldp4, q4 = t - r * ldp4;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (r < k && q4 < ld4) v = reinterpret_cast<const float4*>(a.X + (size_t)s_bid[r] * ld)[q4];
+        reinterpret_cast<float4*>(s_brow)[t] = v;
+    }
+
+    // ---- chunk staging: global -> registers (fetch), registers -> LDS + membership (commit) --
+    float4 pf_row[kNrPfRows];
+    uint32_t pf_snap[kNrPfSnap];
+    uint32_t pf_id = kNoId, pf_na = 0;
+    float pf_bnd = 0.f, pf_nrm = 0.f;
+    auto fetch = [&](uint32_t c_off, uint32_t na) {
+        pf_na = na;
+        pf_id = sr < na ? s_rsid[c_off + sr] : kNoId;
+        if (sc == 0 && sr < na) {
+            pf_bnd = a.bound[pf_id];
+            pf_nrm = a.norms[pf_id];
+        }
+        const float* xr = a.X + (size_t)(sr < na ? pf_id : 0) * ld;
+#pragma unroll
+        for (uint32_t u = 0; u < kNrPfRows; ++u) {
+            const uint32_t q4 = sc + 16 * u;
+            pf_row[u] = (sr < na && q4 < ld4) ? reinterpret_cast<const float4*>(xr)[q4]
+                                              : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const uint32_t* sp = a.snap + (size_t)(sr < na ? pf_id : 0) * k;
+#pragma unroll
+        for (uint32_t u = 0; u < kNrPfSnap; ++u) {
+            const uint32_t j = sc + 16 * u;
+            pf_snap[u] = (sr < na && j < k) ? sp[j] : 0u;
+        }
+    };
+    // needs the mask words zeroed (by the caller, one barrier earlier)
+    auto commit = [&]() {
+        if (sc == 0) {
+            s_aid[sr] = pf_id;
+            s_abnd[sr] = sr < pf_na ? pf_bnd : 0.f;
+            s_anrm[sr] = sr < pf_na ? pf_nrm : 0.f;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kNrPfRows; ++u) {
+            const uint32_t q4 = sc + 16 * u;
+            if (q4 < ldp4) reinterpret_cast<float4*>(s_arow)[sr * ldp4 + q4] = pf_row[u];
+        }
+        // member[sr][j]: S(q)[j] is in S(p): the list's id check would reject it
+        if (sr < pf_na) {
+#pragma unroll
+            for (uint32_t u = 0; u < kNrPfSnap; ++u) {
+                const uint32_t e = pf_snap[u];
+                if (e == 0) continue;
+                uint32_t h = nr_hash(e);
+                for (;;) {
+                    const uint32_t key = s_hkey[h];
+                    if (key == kNoId) break;
+                    if (key == e) {
+                        const uint32_t j = s_hval[h];
+                        atomicOr(&s_mask[sr * 4 + (j >> 5)], 1u << (j & 31));
+                        break;
+                    }
+                    h = (h + 1) & (kNrHash - 1);
+                }
+            }
+        }
+    };
+    fetch(0, min(nrs, kNrChunk));
+    for (uint32_t t = tid; t < kNrChunk * 4; t += kNrThreads) s_mask[t] = 0;
+    __syncthreads();
+    commit();
+    __syncthreads();
 
     // |approx - exact| <= (K + 8) 2^-23 (n_p + n_r) for the FP32 Gram over K = ld terms
     const float margin_c = (float)(ld + 8) * (1.0f / 8388608.0f);
@@ -244,72 +381,61 @@ __global__ __launch_bounds__(kNrThreads) void k_non_reverse(NonRevArgs a) {
     const uint32_t t_lo = th * t_per, t_hi = min(ntile, t_lo + t_per);
     const uint32_t lrow = lane & 15, lk = lane >> 4;
     uint32_t n_scr = 0, n_app = 0;
-    for (uint64_t c0 = a0; c0 < a1; c0 += kNrChunk) {
-        const uint32_t na = (uint32_t)min<uint64_t>(kNrChunk, a1 - c0);
-        __syncthreads();
-        for (uint32_t t = tid; t < kNrChunk; t += kNrThreads) {
-            const uint32_t p = t < na ? a.rs[c0 + t] : kNoId;
-            s_aid[t] = p;
-            s_abnd[t] = t < na ? a.bound[p] : 0.f;
-            s_anrm[t] = t < na ? a.norms[p] : 0.f;
-        }
-        __syncthreads();
-        bulk_rows(a.X, ld, ldp, s_aid, na, kNrChunk, s_arow);
-        for (uint32_t t = tid; t < na * k; t += kNrThreads) {
-            const uint32_t r = t / k, j = t - r * k;
-            s_asnap[t] = a.snap[(size_t)s_aid[r] * k + j];
-        }
-        for (uint32_t t = tid; t < kNrChunk * L.mw; t += kNrThreads) s_mask[t] = 0;
-        __syncthreads();
-        // member[i][j]: S[q][j] is already in S[p_i] (or is p_i itself): those offers would be
-        // rejected by the list's id check. Id 0 may be the padding of an unfilled list rather than
-        // a member, so it is never marked.
-        for (uint32_t t = tid; t < na * k; t += kNrThreads) {
-            const uint32_t i = t / k, j = t - i * k;
-            const uint32_t rid = s_bid[j];
-            const uint32_t* sp = s_asnap + i * k;
-            uint32_t lo = 0, hi = k;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (sp[mid] < rid) lo = mid + 1;
-                else hi = mid;
+
+    for (uint32_t c0 = 0; c0 < nrs; c0 += kNrChunk) {
+        const uint32_t na = min(kNrChunk, nrs - c0);
+        // the next chunk's loads fly while this chunk is screened
+        const uint32_t n0 = c0 + kNrChunk;
+        const bool next_in_group = n0 < nrs && n0 + kNrChunk <= g0 + kNrIds;
+        if (next_in_group) fetch(n0 - g0, min(kNrChunk, nrs - n0));
+
+        uint32_t qn = 0;
+        auto flush = [&]() {
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t e0 = 0; e0 < qn; e0 += 8) {
+                const uint32_t e = e0 + (lane >> 3);
+                const uint32_t ent = e < qn ? s_queue[e] : 0u;
+                const uint32_t i = ent >> 7, cc = ent & 127;
+                const float ex = dist_ref_lane8(s_brow + cc * ldp, s_arow + i * ldp, d, lane);
+                if (e < qn && (lane & 7) == 0 && ex < s_abnd[i]) {
+                    const uint32_t pid = s_aid[i];
+                    const uint32_t slot = atomicAdd(&a.ncnt[pid], 1u);
+                    if (slot < a.cap) {
+                        a.nbuf_d[(size_t)pid * a.cap + slot] = ex;
+                        a.nbuf_id[(size_t)pid * a.cap + slot] = s_bid[cc];
+                    }
+                    ++n_app;
+                }
             }
-            const bool mem = (rid != 0 && lo < k && sp[lo] == rid) || rid == s_aid[i];
-            if (mem) atomicOr(&s_mask[i * L.mw + (j >> 5)], 1u << (j & 31));
-        }
-        __syncthreads();
+            __builtin_amdgcn_wave_barrier();
+            qn = 0;
+        };
         if (16 * strip < na) {
             float rn[4], thr0[4];
-            uint32_t rr[4];
+            uint32_t rr[4], pid[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 rr[i] = 16 * strip + (lane >> 4) * 4 + i;
                 rn[i] = s_anrm[rr[i]];
+                pid[i] = s_aid[rr[i]];
                 thr0[i] = s_abnd[rr[i]] * (1.0f + 1.0f / 65536.0f);
             }
             const float* arow = s_arow + (16 * strip + lrow) * ldp + lk;
-            float af[kNrMaxK4];
-#pragma unroll
-            for (uint32_t kk = 0; kk < kNrMaxK4; ++kk) af[kk] = kk < ksteps ? arow[4 * kk] : 0.f;
             for (uint32_t t = t_lo; t < t_hi; ++t) {
-                float4_t acc = {0.f, 0.f, 0.f, 0.f};
+                if (qn + 256 > kNrQueue) flush();
                 const float* brow = s_brow + (16 * t + lrow) * ldp + lk;
-                float bf[kNrMaxK4];
-#pragma unroll
-                for (uint32_t kk = 0; kk < kNrMaxK4; ++kk) bf[kk] = kk < ksteps ? brow[4 * kk] : 0.f;
-#pragma unroll
-                for (uint32_t kk = 0; kk < kNrMaxK4; ++kk)
-                    if (kk < ksteps) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[kk], bf[kk], acc, 0, 0, 0);
+                const float4_t acc = gram_tile_f32<KS>(arow, brow, ksteps);
                 const uint32_t c = 16 * t + lrow;
                 const float cn = c < k ? s_bnrm[c] : 0.f;
+                const uint32_t rid = s_bid[c];
                 // compact the survivors of this tile into the wave's queue as (row << 7 | col)
-                uint32_t qn = 0;
+                const uint32_t qn_before = qn;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float nsum = rn[i] + cn;
                     const float approx = nsum - 2.0f * acc[i];
-                    bool ok = rr[i] < na && c < k && approx < thr0[i] + margin_c * nsum;
-                    if (ok) ok = !((s_mask[rr[i] * L.mw + (c >> 5)] >> (c & 31)) & 1u);
+                    bool ok = rr[i] < na && c < k && rid != pid[i] && approx < thr0[i] + margin_c * nsum;
+                    if (ok) ok = !((s_mask[rr[i] * 4 + (c >> 5)] >> (c & 31)) & 1u);
                     const uint64_t bal = __ballot(ok);
                     if (ok) {
                         const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
@@ -317,27 +443,23 @@ __global__ __launch_bounds__(kNrThreads) void k_non_reverse(NonRevArgs a) {
                     }
                     qn += __popcll(bal);
                 }
-                n_scr += qn;
-                __builtin_amdgcn_wave_barrier();
-                // exact re-rank of the queue, one survivor per lane
-                for (uint32_t e0 = 0; e0 < qn; e0 += 64) {
-                    if (e0 + lane < qn) {
-                        const uint32_t ent = s_queue[e0 + lane];
-                        const uint32_t i = ent >> 7, cc = ent & 127;
-                        const float ex = dist_ref(s_brow + cc * ldp, s_arow + i * ldp, d);
-                        if (ex < s_abnd[i]) {
-                            const uint32_t pid = s_aid[i];
-                            const uint32_t slot = atomicAdd(&a.ncnt[pid], 1u);
-                            if (slot < a.cap) {
-                                a.nbuf_d[(size_t)pid * a.cap + slot] = ex;
-                                a.nbuf_id[(size_t)pid * a.cap + slot] = s_bid[cc];
-                            }
-                            ++n_app;
-                        }
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
+                n_scr += qn - qn_before;
             }
+        }
+        if (qn) flush();
+        __syncthreads();
+        if (n0 < nrs) {
+            for (uint32_t t = tid; t < kNrChunk * 4; t += kNrThreads) s_mask[t] = 0;
+            if (!next_in_group) {  // next ids group, then a direct fetch
+                g0 = n0;
+                for (uint32_t t = tid; t < min(nrs - g0, kNrIds); t += kNrThreads)
+                    s_rsid[t] = a.rs[a0 + g0 + t];
+                __syncthreads();
+                fetch(0, min(kNrChunk, nrs - n0));
+            }
+            __syncthreads();
+            commit();
+            __syncthreads();
         }
     }
     if (a.counters) {
@@ -463,7 +585,7 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint64_t* out, uint64
                            hipStream_t st) {
     const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
     if (nb == 0) {
-        hipMemsetAsync(out, 0, sizeof(uint64_t), st);
+        (void)hipMemsetAsync(out, 0, sizeof(uint64_t), st);
         return;
     }
     k_scan_sums<<<nb, kScanThreads, 0, st>>>(in, n, scratch);
@@ -478,8 +600,8 @@ void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t
     const size_t tot = (size_t)n * k;
     if (!tot) return;
     const unsigned g = (unsigned)((tot + 255) / 256);
-    hipMemsetAsync(rcnt, 0, (size_t)n * 4, st);
-    hipMemsetAsync(rfill, 0, (size_t)n * 4, st);
+    (void)hipMemsetAsync(rcnt, 0, (size_t)n * 4, st);
+    (void)hipMemsetAsync(rfill, 0, (size_t)n * 4, st);
     k_rs_count<<<g, 256, 0, st>>>(snap, n, k, rcnt);
     launch_exclusive_scan(rcnt, n, roff, scratch, st);
     k_rs_fill<<<g, 256, 0, st>>>(snap, n, k, roff, rfill, rs);
@@ -489,9 +611,17 @@ void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {
     if (!a.nq) return;
     const size_t lds = non_reverse_lds(a.k, a.ld);
     const uint32_t grid = 8 * ((a.nq + 7) / 8);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_non_reverse),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    k_non_reverse<<<grid, kNrThreads, lds, st>>>(a);
+    auto go = [&](auto kernel) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        kernel<<<grid, kNrThreads, lds, st>>>(a);
+    };
+    switch (a.ld / 4) {
+        case 16: go(&k_non_reverse<16>); break;
+        case 25: go(&k_non_reverse<25>); break;
+        case 32: go(&k_non_reverse<32>); break;
+        default: go(&k_non_reverse<0>); break;
+    }
 }
 
 void launch_non_merge(const NonRevArgs& a, uint32_t* ovf_cnt, uint32_t* ovf_list, uint32_t* ovf_flag,
