@@ -512,7 +512,7 @@ struct Builder {
         a.nbuf_id = nbuf_id.p;
         a.cap = kNonCap;
         a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
-        a.counters = counters.p;
+        a.counters = verbose ? counters.p : nullptr;  // same-address atomics: diagnostics only
         KNNG_TRY(norms32.ensure(n));
         launch_row_norms(X, ld, d, n, norms32.p, st);
         a.norms = norms32.p;

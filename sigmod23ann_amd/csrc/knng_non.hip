@@ -154,7 +154,9 @@ __global__ void k_scan_total(const uint32_t* __restrict__ in, uint32_t n, uint64
 // a per-chunk membership bitmap; the survivors queue up per wave and are re-ranked exactly, eight
 // lanes per survivor (lane j owns the reference's accumulator j, then the lanes are summed in
 // order). The next chunk's rows and snapshots are fetched into registers while the current chunk
-// is screened.
+// is screened. Accepted offers collect in an LDS list and go to the candidate buffers once, after
+// the last chunk: an atomic that returns a value waits for every load issued before it, so
+// appending inside the chunk loop would stall on the next chunk's fetch.
 constexpr uint32_t kNrThreads = 512;  // 8 waves: 2 strips of the chunk x 4 groups of S[q]'s tiles
 constexpr uint32_t kNrChunk = 32;     // reverse-neighbour rows per chunk
 constexpr uint32_t kNrMaxK4 = 32;     // ld / 4 for ld <= 128 (larger ld takes the direct kernel)
@@ -162,13 +164,15 @@ constexpr uint32_t kNrIds = 512;      // reverse-neighbour ids staged at a time
 constexpr uint32_t kNrQueue = 320;    // per-wave survivor queue; flushed before it could overflow
 constexpr uint32_t kNrPfRows = (kNrChunk * (kNrMaxK4 + 1) + kNrThreads - 1) / kNrThreads;  // float4 per thread (padded stride <= 132 floats)
 constexpr uint32_t kNrPfSnap = kNrChunk * 128 / kNrThreads;        // snapshot ids per thread
+constexpr uint32_t kNrSurv = 512;     // accepted offers held in LDS per workgroup before appending
 
 __host__ __device__ inline uint32_t nr_row_stride(uint32_t ld) { return ld % 8 == 0 ? ld + 4 : ld; }
 
 struct NrLds {
     uint32_t kb16, ldp;
     uint32_t off_bid, off_bnrm, off_brow, off_hkey, off_hval, off_rsid;
-    uint32_t off_aid, off_abnd, off_anrm, off_arow, off_mask, off_queue, total;
+    uint32_t off_aid, off_abnd, off_anrm, off_arow, off_mask, off_queue;
+    uint32_t off_spid, off_srid, off_sd, off_cnt, total;
 };
 
 constexpr uint32_t kNrHash = 256;  // open-addressing table of S(q)'s ids (<= 128 keys)
@@ -192,6 +196,11 @@ __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
     L.off_arow = o; o += kNrChunk * L.ldp * 4;
     L.off_mask = o;  o += kNrChunk * 4 * 4;
     L.off_queue = o; o += (kNrThreads / 64) * kNrQueue * 2;
+    o = (o + 15) & ~15u;
+    L.off_spid = o; o += kNrSurv * 4;
+    L.off_srid = o; o += kNrSurv * 4;
+    L.off_sd = o;   o += kNrSurv * 4;
+    L.off_cnt = o;  o += 16;
     L.total = (o + 15) & ~15u;
     return L;
 }
@@ -275,6 +284,10 @@ void k_non_reverse(NonRevArgs a) {
     float* s_anrm = reinterpret_cast<float*>(lds + L.off_anrm);
     float* s_arow = reinterpret_cast<float*>(lds + L.off_arow);
     uint32_t* s_mask = reinterpret_cast<uint32_t*>(lds + L.off_mask);
+    uint32_t* s_spid = reinterpret_cast<uint32_t*>(lds + L.off_spid);
+    uint32_t* s_srid = reinterpret_cast<uint32_t*>(lds + L.off_srid);
+    float* s_sd = reinterpret_cast<float*>(lds + L.off_sd);
+    uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds + L.off_cnt);  // [0] accepted, [1] screened
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint16_t* s_queue = reinterpret_cast<uint16_t*>(lds + L.off_queue) + wave * kNrQueue;
     const uint32_t nrs = (uint32_t)(a1 - a0);
@@ -283,6 +296,7 @@ void k_non_reverse(NonRevArgs a) {
 
     // ---- S(q): ids, norms, hash table id -> slot, rows ---------------------------------------
     for (uint32_t t = tid; t < kNrHash; t += kNrThreads) s_hkey[t] = kNoId;
+    if (tid < 2) s_cnt[tid] = 0;
     for (uint32_t t = tid; t < L.kb16; t += kNrThreads) {
         const uint32_t r = t < k ? a.snap[(size_t)q * k + t] : kNoId;
         s_bid[t] = r;
@@ -380,7 +394,7 @@ void k_non_reverse(NonRevArgs a) {
     const uint32_t t_per = (ntile + 3) / 4;
     const uint32_t t_lo = th * t_per, t_hi = min(ntile, t_lo + t_per);
     const uint32_t lrow = lane & 15, lk = lane >> 4;
-    uint32_t n_scr = 0, n_app = 0;
+    uint32_t n_scr = 0;
 
     for (uint32_t c0 = 0; c0 < nrs; c0 += kNrChunk) {
         const uint32_t na = min(kNrChunk, nrs - c0);
@@ -397,14 +411,27 @@ void k_non_reverse(NonRevArgs a) {
                 const uint32_t ent = e < qn ? s_queue[e] : 0u;
                 const uint32_t i = ent >> 7, cc = ent & 127;
                 const float ex = dist_ref_lane8(s_brow + cc * ldp, s_arow + i * ldp, d, lane);
-                if (e < qn && (lane & 7) == 0 && ex < s_abnd[i]) {
-                    const uint32_t pid = s_aid[i];
-                    const uint32_t slot = atomicAdd(&a.ncnt[pid], 1u);
-                    if (slot < a.cap) {
-                        a.nbuf_d[(size_t)pid * a.cap + slot] = ex;
-                        a.nbuf_id[(size_t)pid * a.cap + slot] = s_bid[cc];
+                const bool keep = e < qn && (lane & 7) == 0 && ex < s_abnd[i];
+                const uint64_t bal = __ballot(keep);
+                if (bal) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&s_cnt[0], (uint32_t)__popcll(bal));
+                    base = __shfl(base, 0);
+                    if (keep) {
+                        const uint32_t at = base + __popcll(bal & ((1ull << lane) - 1ull));
+                        const uint32_t pid = s_aid[i];
+                        if (at < kNrSurv) {
+                            s_spid[at] = pid;
+                            s_srid[at] = s_bid[cc];
+                            s_sd[at] = ex;
+                        } else {  // list full: append directly
+                            const uint32_t slot = atomicAdd(&a.ncnt[pid], 1u);
+                            if (slot < a.cap) {
+                                a.nbuf_d[(size_t)pid * a.cap + slot] = ex;
+                                a.nbuf_id[(size_t)pid * a.cap + slot] = s_bid[cc];
+                            }
+                        }
                     }
-                    ++n_app;
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -462,11 +489,23 @@ void k_non_reverse(NonRevArgs a) {
             __syncthreads();
         }
     }
-    if (a.counters) {
-        // n_scr is wave-uniform, n_app per lane
-        if (lane == 0)
-            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)n_scr);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[1]), (unsigned long long)n_app);
+    // the chunk loop ends with a barrier: the list is complete
+    const uint32_t nsurv = s_cnt[0];
+    for (uint32_t t = tid; t < min(nsurv, kNrSurv); t += kNrThreads) {
+        const uint32_t pid = s_spid[t];
+        const uint32_t slot = atomicAdd(&a.ncnt[pid], 1u);
+        if (slot < a.cap) {
+            a.nbuf_d[(size_t)pid * a.cap + slot] = s_sd[t];
+            a.nbuf_id[(size_t)pid * a.cap + slot] = s_srid[t];
+        }
+    }
+    if (a.counters) {  // diagnostics only: two same-address atomics per workgroup
+        if (lane == 0 && n_scr) atomicAdd(&s_cnt[1], n_scr);
+        __syncthreads();
+        if (tid == 0) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)s_cnt[1]);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[1]), (unsigned long long)nsurv);
+        }
     }
 }
 
