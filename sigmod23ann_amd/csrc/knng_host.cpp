@@ -18,6 +18,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/knng.h"
@@ -167,8 +168,16 @@ struct Builder {
     // generator scratch
     DevBuf<float> gen_a, gen_b, gen_c;
 
+    // pinned staging for host <-> device copies: two buffers, so that the CPU fills (or drains)
+    // one while the DMA engine moves the other
+    static constexpr size_t kStageBytes = 32u << 20;
+    PinBuf<unsigned char> stage[2];
+    hipEvent_t stage_ev[2] = {};
+
     ~Builder() {
         for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : stage_ev)
             if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -182,6 +191,80 @@ struct Builder {
         prm = p;
         KNNG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         for (auto& e : ev) KNNG_HIP(hipEventCreate(&e));
+        for (auto& e : stage_ev) KNNG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return KNNG_OK;
+    }
+
+    // Runs fn(lo, hi) over [0, count) on up to 4 host threads.
+    template <class F>
+    static void parallel_rows(size_t count, F fn) {
+        const size_t nt = count >= 4096 ? 4 : 1;
+        if (nt == 1) {
+            fn((size_t)0, count);
+            return;
+        }
+        std::vector<std::thread> th;
+        for (size_t t = 1; t < nt; ++t)
+            th.emplace_back(fn, count * t / nt, count * (t + 1) / nt);
+        fn((size_t)0, count / nt);
+        for (auto& x : th) x.join();
+    }
+
+    // Host rows (any stride) -> device rows of ld floats, zero-padded, through pinned staging.
+    int upload_rows(float* dst, const float* src, size_t stride, uint32_t rows) {
+        const size_t rb = (size_t)ld * sizeof(float);
+        const size_t per = std::max<size_t>(1, kStageBytes / rb);
+        for (int b = 0; b < 2; ++b) KNNG_TRY(stage[b].ensure(per * rb));
+        bool used[2] = {false, false};
+        for (size_t i0 = 0, blk = 0; i0 < rows; i0 += per, ++blk) {
+            const int b = (int)(blk & 1);
+            if (used[b]) KNNG_HIP(hipEventSynchronize(stage_ev[b]));
+            const size_t m = std::min<size_t>(per, rows - i0);
+            float* buf = reinterpret_cast<float*>(stage[b].p);
+            const uint32_t dd = d, ldd = ld;
+            parallel_rows(m, [=](size_t lo, size_t hi) {
+                for (size_t r = lo; r < hi; ++r) {
+                    std::memcpy(buf + r * ldd, src + (i0 + r) * stride, dd * sizeof(float));
+                    for (uint32_t j = dd; j < ldd; ++j) buf[r * ldd + j] = 0.f;
+                }
+            });
+            KNNG_HIP(hipMemcpyAsync(dst + i0 * ld, buf, m * rb, hipMemcpyHostToDevice, st));
+            KNNG_HIP(hipEventRecord(stage_ev[b], st));
+            used[b] = true;
+        }
+        KNNG_HIP(hipStreamSynchronize(st));
+        return KNNG_OK;
+    }
+
+    // Device bytes -> pageable host memory through pinned staging.
+    int download(void* dst, const void* src, size_t bytes) {
+        for (int b = 0; b < 2; ++b) KNNG_TRY(stage[b].ensure(kStageBytes));
+        const unsigned char* s8 = static_cast<const unsigned char*>(src);
+        unsigned char* d8 = static_cast<unsigned char*>(dst);
+        size_t pend_off[2] = {0, 0}, pend_len[2] = {0, 0};
+        auto drain = [&](int b) -> int {
+            if (!pend_len[b]) return KNNG_OK;
+            KNNG_HIP(hipEventSynchronize(stage_ev[b]));
+            const unsigned char* from = stage[b].p;
+            unsigned char* to = d8 + pend_off[b];
+            parallel_rows(pend_len[b] / 4096 + 1, [=](size_t lo, size_t hi) {
+                const size_t a0 = std::min(lo * 4096, pend_len[b]), a1 = std::min(hi * 4096, pend_len[b]);
+                if (a1 > a0) std::memcpy(to + a0, from + a0, a1 - a0);
+            });
+            pend_len[b] = 0;
+            return KNNG_OK;
+        };
+        for (size_t o = 0, blk = 0; o < bytes; o += kStageBytes, ++blk) {
+            const int b = (int)(blk & 1);
+            KNNG_TRY(drain(b));
+            const size_t m = std::min(kStageBytes, bytes - o);
+            KNNG_HIP(hipMemcpyAsync(stage[b].p, s8 + o, m, hipMemcpyDeviceToHost, st));
+            KNNG_HIP(hipEventRecord(stage_ev[b], st));
+            pend_off[b] = o;
+            pend_len[b] = m;
+        }
+        KNNG_TRY(drain(0));
+        KNNG_TRY(drain(1));
         return KNNG_OK;
     }
 
@@ -204,11 +287,12 @@ struct Builder {
         }
         ld = (dd + 3) & ~3u;
         KNNG_TRY(xbuf.ensure((size_t)nn * ld));
-        if (nn) {
+        if (nn && on_device) {
             KNNG_HIP(hipMemsetAsync(xbuf.p, 0, (size_t)nn * ld * sizeof(float), st));
             KNNG_HIP(hipMemcpy2DAsync(xbuf.p, ld * sizeof(float), pts, stride * sizeof(float),
-                                      dd * sizeof(float), nn,
-                                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+                                      dd * sizeof(float), nn, hipMemcpyDeviceToDevice, st));
+        } else if (nn) {
+            KNNG_TRY(upload_rows(xbuf.p, pts, stride, nn));
         }
         X = xbuf.p;
         return KNNG_OK;
@@ -761,8 +845,8 @@ int knng_build(knng_handle* h, const float* points, uint32_t n, uint32_t d, size
     KNNG_TRY(oid.ensure(tot));
     if (out_dists) KNNG_TRY(od.ensure(tot));
     KNNG_TRY(b.build(oid.p, out_dists ? od.p : nullptr));
-    KNNG_HIP(hipMemcpy(out_ids, oid.p, tot * 4, hipMemcpyDeviceToHost));
-    if (out_dists) KNNG_HIP(hipMemcpy(out_dists, od.p, tot * 4, hipMemcpyDeviceToHost));
+    KNNG_TRY(b.download(out_ids, oid.p, tot * 4));
+    if (out_dists) KNNG_TRY(b.download(out_dists, od.p, tot * 4));
     return KNNG_OK;
 }
 

@@ -151,7 +151,8 @@ __global__ void k_scan_total(const uint32_t* __restrict__ in, uint32_t n, uint64
 // ---- reverse screening kernel --------------------------------------------------------------------
 // One workgroup per q. FP32 rows of S[q] and 32-row chunks of RS[q] sit in LDS; an FP32 MFMA Gram
 // (16x16x4) screens every pair with a rigorous margin; pairs with r already in S[p] are masked by
-// a per-chunk membership bitmap; the survivors queue up per wave and are re-ranked exactly, eight
+// a per-chunk membership bitmap (S(q)'s ids sit in a bucketed hash table: one lookup is two
+// 16-byte LDS reads, with no probe loop for the lanes of a wave to diverge on); the survivors queue up per wave and are re-ranked exactly, eight
 // lanes per survivor (lane j owns the reference's accumulator j, then the lanes are summed in
 // order). The next chunk's rows and snapshots are fetched into registers while the current chunk
 // is screened. Accepted offers collect in an LDS list and go to the candidate buffers once, after
@@ -170,12 +171,18 @@ __host__ __device__ inline uint32_t nr_row_stride(uint32_t ld) { return ld % 8 =
 
 struct NrLds {
     uint32_t kb16, ldp;
-    uint32_t off_bid, off_bnrm, off_brow, off_hkey, off_hval, off_rsid;
+    uint32_t off_bid, off_bnrm, off_brow, off_hkey, off_hval, off_hcnt, off_rsid;
     uint32_t off_aid, off_abnd, off_anrm, off_arow, off_mask, off_queue;
     uint32_t off_spid, off_srid, off_sd, off_cnt, total;
 };
 
-constexpr uint32_t kNrHash = 256;  // open-addressing table of S(q)'s ids (<= 128 keys)
+// S(q)'s ids (<= 128) hashed into 64 buckets of 8 slots. A key whose bucket is full is left out:
+// membership then reads "absent", the offer goes through, and the list's id check rejects it.
+constexpr uint32_t kNrBuckets = 64;
+constexpr uint32_t kNrSlots = 8;
+constexpr uint32_t kNrHash = kNrBuckets * kNrSlots;
+// S(q) rows staged per thread (k <= 128 rows of <= 33 float4)
+constexpr uint32_t kNrBRows = (128 * (kNrMaxK4 + 1) + kNrThreads - 1) / kNrThreads;
 
 __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
     NrLds L;
@@ -187,7 +194,8 @@ __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
     o = (o + 15) & ~15u;
     L.off_brow = o; o += L.kb16 * L.ldp * 4;
     L.off_hkey = o; o += kNrHash * 4;
-    L.off_hval = o; o += kNrHash * 4;
+    L.off_hval = o; o += kNrHash;
+    L.off_hcnt = o; o += kNrBuckets * 4;
     L.off_rsid = o; o += kNrIds * 4;
     L.off_aid = o;  o += kNrChunk * 4;
     L.off_abnd = o; o += kNrChunk * 4;
@@ -207,7 +215,7 @@ __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
 
 size_t non_reverse_lds(uint32_t k, uint32_t ld) { return nr_layout(k, ld).total; }
 
-__device__ inline uint32_t nr_hash(uint32_t id) { return (id * 0x9E3779B1u) >> 24; }
+__device__ inline uint32_t nr_bucket(uint32_t id) { return (id * 0x9E3779B1u) >> 26; }
 
 // Reference-order distance of two rows, computed by the 8 lanes of a lane group: lane j of the
 // group accumulates dims j, j+8, ... below floor(d/8)*8 with fused multiply-add; the 8 partial sums
@@ -277,7 +285,8 @@ void k_non_reverse(NonRevArgs a) {
     float* s_bnrm = reinterpret_cast<float*>(lds + L.off_bnrm);
     float* s_brow = reinterpret_cast<float*>(lds + L.off_brow);
     uint32_t* s_hkey = reinterpret_cast<uint32_t*>(lds + L.off_hkey);
-    uint32_t* s_hval = reinterpret_cast<uint32_t*>(lds + L.off_hval);
+    uint8_t* s_hval = lds + L.off_hval;
+    uint32_t* s_hcnt = reinterpret_cast<uint32_t*>(lds + L.off_hcnt);
     uint32_t* s_rsid = reinterpret_cast<uint32_t*>(lds + L.off_rsid);
     uint32_t* s_aid = reinterpret_cast<uint32_t*>(lds + L.off_aid);
     float* s_abnd = reinterpret_cast<float*>(lds + L.off_abnd);
@@ -296,6 +305,7 @@ void k_non_reverse(NonRevArgs a) {
 
     // ---- S(q): ids, norms, hash table id -> slot, rows ---------------------------------------
     for (uint32_t t = tid; t < kNrHash; t += kNrThreads) s_hkey[t] = kNoId;
+    if (tid < kNrBuckets) s_hcnt[tid] = 0;
     if (tid < 2) s_cnt[tid] = 0;
     for (uint32_t t = tid; t < L.kb16; t += kNrThreads) {
         const uint32_t r = t < k ? a.snap[(size_t)q * k + t] : kNoId;
@@ -310,16 +320,30 @@ void k_non_reverse(NonRevArgs a) {
     if (tid < k) {
         const uint32_t id = s_bid[tid];
         if (id != 0) {
-            uint32_t h = nr_hash(id);
-            while (atomicCAS(&s_hkey[h], kNoId, id) != kNoId) h = (h + 1) & (kNrHash - 1);
-            s_hval[h] = tid;
+            const uint32_t b = nr_bucket(id);
+            const uint32_t slot = atomicAdd(&s_hcnt[b], 1u);
+            if (slot < kNrSlots) {
+                s_hkey[b * kNrSlots + slot] = id;
+                s_hval[b * kNrSlots + slot] = (uint8_t)tid;
+            }
         }
     }
-    for (uint32_t t = tid; t < L.kb16 * ldp4; t += kNrThreads) {
-        const uint32_t r = t / ldp4, q4 = t - r * ldp4;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (r < k && q4 < ld4) v = reinterpret_cast<const float4*>(a.X + (size_t)s_bid[r] * ld)[q4];
-        reinterpret_cast<float4*>(s_brow)[t] = v;
+    {  // every S(q) row load in flight at once
+        const uint32_t tot = L.kb16 * ldp4;
+        float4 v[kNrBRows];
+#pragma unroll
+        for (uint32_t u = 0; u < kNrBRows; ++u) {
+            const uint32_t t = tid + u * kNrThreads;
+            const uint32_t r = t / ldp4, q4 = t - r * ldp4;
+            v[u] = (t < tot && r < k && q4 < ld4)
+                       ? reinterpret_cast<const float4*>(a.X + (size_t)s_bid[r] * ld)[q4]
+                       : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kNrBRows; ++u) {
+            const uint32_t t = tid + u * kNrThreads;
+            if (t < tot) reinterpret_cast<float4*>(s_brow)[t] = v[u];
+        }
     }
 
     // ---- chunk staging: global -> registers (fetch), registers -> LDS + membership (commit) --
@@ -366,16 +390,21 @@ void k_non_reverse(NonRevArgs a) {
             for (uint32_t u = 0; u < kNrPfSnap; ++u) {
                 const uint32_t e = pf_snap[u];
                 if (e == 0) continue;
-                uint32_t h = nr_hash(e);
-                for (;;) {
-                    const uint32_t key = s_hkey[h];
-                    if (key == kNoId) break;
-                    if (key == e) {
-                        const uint32_t j = s_hval[h];
-                        atomicOr(&s_mask[sr * 4 + (j >> 5)], 1u << (j & 31));
-                        break;
-                    }
-                    h = (h + 1) & (kNrHash - 1);
+                const uint32_t b = nr_bucket(e);
+                const uint4 k0 = reinterpret_cast<const uint4*>(s_hkey)[2 * b];
+                const uint4 k1 = reinterpret_cast<const uint4*>(s_hkey)[2 * b + 1];
+                int f = -1;
+                f = k0.x == e ? 0 : f;
+                f = k0.y == e ? 1 : f;
+                f = k0.z == e ? 2 : f;
+                f = k0.w == e ? 3 : f;
+                f = k1.x == e ? 4 : f;
+                f = k1.y == e ? 5 : f;
+                f = k1.z == e ? 6 : f;
+                f = k1.w == e ? 7 : f;
+                if (f >= 0) {
+                    const uint32_t j = s_hval[b * kNrSlots + f];
+                    atomicOr(&s_mask[sr * 4 + (j >> 5)], 1u << (j & 31));
                 }
             }
         }
