@@ -125,48 +125,67 @@ __device__ inline bool pair_less(float da, uint32_t ia, float db, uint32_t ib) {
     return da < db || (da == db && ia < ib);
 }
 
-// A point's top-k list held by one wave: lane l owns slots l and l + 64 (k <= 128), sorted
-// ascending by (dist, id), empty slots (FLT_MAX, kNoId). `bound` is the k-th distance.
-struct WaveList {
-    float d0, d1;
-    uint32_t i0, i1;
-    float bound;
+// Lane `src` (wave-uniform) of v, in every lane: v_readlane, a register read.
+__device__ inline float lane_bcast(float v, uint32_t src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)src));
+}
+__device__ inline uint32_t lane_bcast(uint32_t v, uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)src);
+}
+// Lane - 1 of v (lane 0 receives `fill`): DPP wave_shr:1, a register move with no LDS round trip.
+__device__ inline uint32_t wave_shr1(uint32_t v, uint32_t fill) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138, 0xF, 0xF, false);
+}
+__device__ inline float wave_shr1(float v, float fill) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x138,
+                                                      0xF, 0xF, false));
+}
 
-    __device__ inline void load(const uint32_t* gid, const float* gd, uint32_t k, uint32_t lane) {
-        i0 = lane < k ? gid[lane] : kNoId;
-        d0 = lane < k ? gd[lane] : kFltMax;
-        i1 = lane + 64 < k ? gid[lane + 64] : kNoId;
-        d1 = lane + 64 < k ? gd[lane + 64] : kFltMax;
+// A point's top-k list held by one wave: lane l owns slots l and l + 64 (k <= 128), sorted
+// ascending by (dist, id); slots past the last filled one, and all slots >= k, hold the empty key.
+// A slot is one 64-bit key, the distance's bits above the id: distances are non-negative, so
+// unsigned key order is the (dist, id) order of pair_less and one 64-bit compare decides it. An
+// insert is a few VALU, SALU, DPP and readlane operations with scalar branches only.
+struct WaveList {
+    static constexpr uint64_t kEmpty = ((uint64_t)0x7F7FFFFFu << 32) | 0xFFFFFFFFu;  // (FLT_MAX, kNoId)
+    uint64_t k0, k1;
+    float bound;  // k-th distance, wave-uniform
+
+    __device__ static inline uint64_t key(float d, uint32_t id) {
+        return ((uint64_t)__float_as_uint(d) << 32) | id;
+    }
+    __device__ inline void set_bound(uint32_t k) {
         const uint32_t kl = k - 1;
-        bound = __shfl(kl >= 64 ? d1 : d0, kl & 63);
+        const uint64_t kk = kl >= 64 ? k1 : k0;
+        bound = __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(kk >> 32), (int)(kl & 63)));
+    }
+    __device__ inline void load(const uint32_t* gid, const float* gd, uint32_t k, uint32_t lane) {
+        k0 = lane < k ? key(gd[lane], gid[lane]) : kEmpty;
+        k1 = lane + 64 < k ? key(gd[lane + 64], gid[lane + 64]) : kEmpty;
+        set_bound(k);
     }
     __device__ inline void store(uint32_t* gid, float* gd, uint32_t k, uint32_t lane) const {
-        if (lane < k) { gid[lane] = i0; gd[lane] = d0; }
-        if (lane + 64 < k) { gid[lane + 64] = i1; gd[lane + 64] = d1; }
+        if (lane < k) { gid[lane] = (uint32_t)k0; gd[lane] = __uint_as_float((uint32_t)(k0 >> 32)); }
+        if (lane + 64 < k) { gid[lane + 64] = (uint32_t)k1; gd[lane + 64] = __uint_as_float((uint32_t)(k1 >> 32)); }
     }
     // Offer (nd, nid) to the list; wave-uniform arguments. The reference's filter is strict
-    // `dist < bound` (KnnSets.hpp:152-157) and an id already present is rejected (:49, :59-72).
+    // `dist < bound` (KnnSets.hpp:152-157) and an id already present is rejected (:49, :59-72);
+    // a present id carries the same distance, so its key equals the offered one.
     __device__ inline bool insert(float nd, uint32_t nid, uint32_t k, uint32_t lane) {
-        if (!(nd < bound)) return false;
-        if (__ballot((lane < k && i0 == nid) || (lane + 64 < k && i1 == nid))) return false;
-        const bool lt0 = lane < k && pair_less(d0, i0, nd, nid);
-        const bool lt1 = lane + 64 < k && pair_less(d1, i1, nd, nid);
-        const uint32_t pos = __popcll(__ballot(lt0)) + __popcll(__ballot(lt1));
-        const float p0d = __shfl_up(d0, 1);
-        const uint32_t p0i = __shfl_up(i0, 1);
-        const float p1d_up = __shfl_up(d1, 1);
-        const uint32_t p1i_up = __shfl_up(i1, 1);
-        const float l63d = __shfl(d0, 63);
-        const uint32_t l63i = __shfl(i0, 63);
-        const float p1d = lane == 0 ? l63d : p1d_up;
-        const uint32_t p1i = lane == 0 ? l63i : p1i_up;
-        const uint32_t e0 = lane, e1 = lane + 64;
-        if (e0 == pos) { d0 = nd; i0 = nid; }
-        else if (e0 > pos) { d0 = p0d; i0 = p0i; }
-        if (e1 == pos) { d1 = nd; i1 = nid; }
-        else if (e1 > pos) { d1 = p1d; i1 = p1i; }
-        const uint32_t kl = k - 1;
-        bound = __shfl(kl >= 64 ? d1 : d0, kl & 63);
+        if (!__builtin_amdgcn_readfirstlane(nd < bound)) return false;
+        const uint64_t nk = key(nd, nid);
+        if (__ballot(k0 == nk || k1 == nk)) return false;
+        const uint32_t pos = __popcll(__ballot(k0 < nk)) + __popcll(__ballot(k1 < nk));
+        // slot e takes slot e - 1's key for e > pos; slot 64 (lane 0's second) takes slot 63
+        const uint32_t l63lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k0, 63);
+        const uint32_t l63hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(k0 >> 32), 63);
+        const uint64_t s0 = ((uint64_t)wave_shr1((uint32_t)(k0 >> 32), 0u) << 32) | wave_shr1((uint32_t)k0, 0u);
+        const uint64_t s1 = ((uint64_t)wave_shr1((uint32_t)(k1 >> 32), l63hi) << 32) | wave_shr1((uint32_t)k1, l63lo);
+        const uint32_t e1 = lane + 64;
+        k0 = lane == pos ? nk : (lane > pos ? s0 : k0);
+        k1 = e1 >= k ? kEmpty : (e1 == pos ? nk : (e1 > pos ? s1 : k1));
+        if (lane >= k) k0 = kEmpty;
+        set_bound(k);
         return true;
     }
 };

@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
             }
             const uint32_t cnt = __popc(w);
             const uint32_t base = wave_excl_scan(cnt, lane);
-            const uint32_t S = __shfl(base + cnt, 63);
+            const uint32_t S = lane_bcast(base + cnt, 63);
             if (S == 0) continue;
             uint32_t o = base;
             while (w) {
@@ -233,8 +233,8 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
                 while (acc) {
                     const uint32_t src = __ffsll((unsigned long long)acc) - 1;
                     acc &= acc - 1;
-                    const float nd = __shfl(dd, src);
-                    const uint32_t nid = __shfl(cid, src);
+                    const float nd = lane_bcast(dd, src);
+                    const uint32_t nid = lane_bcast(cid, src);
                     if (lst.insert(nd, nid, k, lane)) {
                         changed = true;
                         ++n_ins;

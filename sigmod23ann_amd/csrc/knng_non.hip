@@ -239,6 +239,29 @@ __device__ inline float dist_ref_lane8(const float* __restrict__ a, const float*
     return s;
 }
 
+// dist_ref_lane8 for a compile-time width D: every load of the loops is issued at once.
+template <uint32_t D>
+__device__ inline float dist_ref_lane8_fixed(const float* __restrict__ a, const float* __restrict__ b,
+                                             uint32_t lane) {
+    constexpr uint32_t kMain = D & ~7u;
+    const uint32_t l8 = lane & 7u, base = lane & ~7u;
+    float acc = 0.f;
+#pragma unroll
+    for (uint32_t i = 0; i < kMain; i += 8) {
+        const float t = a[i + l8] - b[i + l8];
+        acc = __builtin_fmaf(t, t, acc);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) s = s + __shfl(acc, base + j);
+#pragma unroll
+    for (uint32_t i = kMain; i < D; ++i) {
+        const float t = a[i] - b[i];
+        s = __builtin_fmaf(t, t, s);
+    }
+    return s;
+}
+
 // FP32 Gram of a 16x16 tile over K = 4 KS dims, four independent accumulation chains (the
 // screen's error bound holds for any summation order). KS is a compile-time constant for the
 // common widths; KS = 0 is the generic runtime loop.
@@ -439,13 +462,21 @@ void k_non_reverse(NonRevArgs a) {
                 const uint32_t e = e0 + (lane >> 3);
                 const uint32_t ent = e < qn ? s_queue[e] : 0u;
                 const uint32_t i = ent >> 7, cc = ent & 127;
-                const float ex = dist_ref_lane8(s_brow + cc * ldp, s_arow + i * ldp, d, lane);
+                float ex;
+                if constexpr (KS > 0) {
+                    if (d == 4 * KS)
+                        ex = dist_ref_lane8_fixed<4 * KS>(s_brow + cc * ldp, s_arow + i * ldp, lane);
+                    else
+                        ex = dist_ref_lane8(s_brow + cc * ldp, s_arow + i * ldp, d, lane);
+                } else {
+                    ex = dist_ref_lane8(s_brow + cc * ldp, s_arow + i * ldp, d, lane);
+                }
                 const bool keep = e < qn && (lane & 7) == 0 && ex < s_abnd[i];
                 const uint64_t bal = __ballot(keep);
                 if (bal) {
                     uint32_t base = 0;
                     if (lane == 0) base = atomicAdd(&s_cnt[0], (uint32_t)__popcll(bal));
-                    base = __shfl(base, 0);
+                    base = lane_bcast(base, 0);
                     if (keep) {
                         const uint32_t at = base + __popcll(bal & ((1ull << lane) - 1ull));
                         const uint32_t pid = s_aid[i];
@@ -569,7 +600,7 @@ __global__ __launch_bounds__(256) void k_non_merge(NonRevArgs a, uint32_t* __res
         while (acc) {
             const uint32_t src = __ffsll((unsigned long long)acc) - 1;
             acc &= acc - 1;
-            changed |= lst.insert(__shfl(dd, src), __shfl(cid, src), k, lane);
+            changed |= lst.insert(lane_bcast(dd, src), lane_bcast(cid, src), k, lane);
         }
     }
     if (changed) {
@@ -612,7 +643,7 @@ __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a, const 
             while (acc) {
                 const uint32_t src = __ffsll((unsigned long long)acc) - 1;
                 acc &= acc - 1;
-                changed |= lst.insert(__shfl(dd, src), __shfl(cid, src), k, lane);
+                changed |= lst.insert(lane_bcast(dd, src), lane_bcast(cid, src), k, lane);
             }
         }
     }
