@@ -45,10 +45,43 @@ def parse():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"])
     ap.add_argument("--seed", type=int, default=2023)
     ap.add_argument("--recall-queries", type=int, default=100_000)
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-sample", type=int, default=250_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="before the build, run one plain device copy of this many MiB (a known "
+                         "read/write volume for scaling FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_non_reverse_traffic_lowrank10M.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary, if it covers it."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            vals = dict(line.strip()[2:].split(": ", 1) for line in f
+                        if line.startswith("- ") and ": " in line)
+    except OSError:
+        return None
+    if vals.get("kernel") != kernel:
+        return None
+    try:
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def copy_calibration(mib):
+    """One elementwise copy of `mib` MiB on the device: mib MiB read and mib MiB written."""
+    import torch
+    x = torch.ones(mib * (1 << 18), dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: {mib} MiB read, {mib} MiB written")
 
 
 def main():
@@ -58,6 +91,9 @@ def main():
               "shard across GPUs in this version", file=sys.stderr)
         sys.exit(2)
     import sigmod23ann_amd as K
+
+    if a.copy_calibration_mib > 0:
+        copy_calibration(a.copy_calibration_mib)
 
     prec = {"fp32": K.PREC_FP32, "fp16": K.PREC_FP16, "bf16": K.PREC_BF16}[a.precision]
     prm = K.params(k=a.k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
@@ -141,6 +177,7 @@ def main():
                 "peak": MFMA_FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / MFMA_FP16_PEAK_TFLOPS, 5), "traffic": None,
                 "flops_per_launch": work / max(1, launches)}
+    roof["traffic"] = measured_traffic(name)
     roof["launches"] = launches
     roof["avg_launch_ms"] = round(ms / max(1, launches), 4)
     roof["share_of_build"] = round(ms / st["total_ms"], 3) if st["total_ms"] > 0 else None
