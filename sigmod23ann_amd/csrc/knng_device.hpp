@@ -141,6 +141,61 @@ __device__ inline float wave_shr1(float v, float fill) {
                                                       0xF, 0xF, false));
 }
 
+// v from lane ^ J (J a power of two below 64). J = 1, 2: DPP quad_perm; J = 4: DPP row_shl:4 /
+// row_shr:4 selected by lane bit 2; J = 8: DPP row_ror:8; J = 16: ds_swizzle xor mode within 32
+// lanes; J = 32: ds_bpermute.
+template <uint32_t J>
+__device__ inline uint32_t lane_xor(uint32_t v, uint32_t lane) {
+    if constexpr (J == 1) {
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false);
+    } else if constexpr (J == 2) {
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false);
+    } else if constexpr (J == 4) {
+        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x104, 0xF, 0xF, false);
+        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x114, 0xF, 0xF, false);
+        return (lane & 4) ? dn : up;
+    } else if constexpr (J == 8) {
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x128, 0xF, 0xF, false);
+    } else if constexpr (J == 16) {
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
+    } else {
+        static_assert(J == 32, "lane_xor: J must be a power of two below 64");
+        return (uint32_t)__shfl_xor((int)v, 32);
+    }
+}
+template <uint32_t J>
+__device__ inline uint64_t lane_xor64(uint64_t v, uint32_t lane) {
+    return ((uint64_t)lane_xor<J>((uint32_t)(v >> 32), lane) << 32) | lane_xor<J>((uint32_t)v, lane);
+}
+// One compare-exchange of a bitonic network across lanes at distance J: the pair's lower lane
+// keeps the minimum when `asc`, the maximum otherwise.
+template <uint32_t J>
+__device__ inline uint64_t bitonic_step(uint64_t x, uint32_t lane, bool asc) {
+    const uint64_t p = lane_xor64<J>(x, lane);
+    const bool take_min = ((lane & J) == 0) == asc;
+    return ((x < p) == take_min) ? x : p;
+}
+template <uint32_t SIZE>
+__device__ inline uint64_t bitonic_phase_desc(uint64_t x, uint32_t lane) {
+    // merge runs of SIZE / 2 into runs of SIZE, the run holding `lane` descending overall
+    const bool asc = (lane & SIZE) != 0;
+    if constexpr (SIZE >= 64) x = bitonic_step<32>(x, lane, asc);
+    if constexpr (SIZE >= 32) x = bitonic_step<16>(x, lane, asc);
+    if constexpr (SIZE >= 16) x = bitonic_step<8>(x, lane, asc);
+    if constexpr (SIZE >= 8) x = bitonic_step<4>(x, lane, asc);
+    if constexpr (SIZE >= 4) x = bitonic_step<2>(x, lane, asc);
+    return bitonic_step<1>(x, lane, asc);
+}
+// 64 keys, one per lane, sorted descending (lane 0 the largest).
+__device__ inline uint64_t sort64_desc(uint64_t x, uint32_t lane) {
+    x = bitonic_phase_desc<2>(x, lane);
+    x = bitonic_phase_desc<4>(x, lane);
+    x = bitonic_phase_desc<8>(x, lane);
+    x = bitonic_phase_desc<16>(x, lane);
+    x = bitonic_phase_desc<32>(x, lane);
+    return bitonic_phase_desc<64>(x, lane);
+}
+
 // A point's top-k list held by one wave: lane l owns slots l and l + 64 (k <= 128), sorted
 // ascending by (dist, id); slots past the last filled one, and all slots >= k, hold the empty key.
 // A slot is one 64-bit key, the distance's bits above the id: distances are non-negative, so
@@ -187,6 +242,78 @@ struct WaveList {
         if (lane >= k) k0 = kEmpty;
         set_bound(k);
         return true;
+    }
+
+    // Offer up to 64 keys at once, one per lane (kEmpty where a lane has none); `scratch` is 128
+    // keys of this wave's LDS. Same set semantics as inserting them one by one: only keys below
+    // the bound and not already present enter, and the list keeps the k smallest. With ties at the
+    // k-th distance the batch keeps the smaller ids, where one-by-one insertion keeps whichever
+    // arrived first. Few offers go through insert(); more go through a bitonic sort of the
+    // batch, a half-cleaner against the list and a bitonic merge of the 128 smallest, then
+    // adjacent equal keys (an id offered again) are dropped. Returns whether the list changed.
+    __device__ inline bool offer64(uint64_t x, uint32_t k, uint32_t lane, uint64_t* scratch) {
+        if (!((uint32_t)(x >> 32) < __float_as_uint(bound))) x = kEmpty;
+        uint64_t m = __ballot(x != kEmpty);
+        if (!m) return false;
+        if (__popcll(m) <= 4) {
+            bool changed = false;
+            while (m) {
+                const uint32_t src = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)src);
+                const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), (int)src);
+                changed |= insert(__uint_as_float(hi), lo, k, lane);
+            }
+            return changed;
+        }
+        const uint64_t o0 = k0, o1 = k1;
+        const uint64_t b = sort64_desc(x, lane);
+        // list (ascending) ++ 64 empty ++ batch (descending) is bitonic. Its half-cleaner leaves
+        // the 128 smallest, as a bitonic sequence, in (r0, r1); the next 64 are max(list slots
+        // 64..127, batch) as a bitonic sequence, in r2. Duplicates are dropped only after the
+        // merge, so the 192 sorted slots are kept: they hold every distinct key, and at most 64
+        // of them are duplicates, which leaves at least 128 distinct ones.
+        uint64_t r0 = k0, r1 = b < k1 ? b : k1, r2 = b < k1 ? k1 : b;
+        const bool lt = r0 < r1;
+        const uint64_t n0 = lt ? r0 : r1, n1 = lt ? r1 : r0;
+        r0 = n0;
+        r1 = n1;
+        r0 = bitonic_step<32>(r0, lane, true); r1 = bitonic_step<32>(r1, lane, true); r2 = bitonic_step<32>(r2, lane, true);
+        r0 = bitonic_step<16>(r0, lane, true); r1 = bitonic_step<16>(r1, lane, true); r2 = bitonic_step<16>(r2, lane, true);
+        r0 = bitonic_step<8>(r0, lane, true);  r1 = bitonic_step<8>(r1, lane, true);  r2 = bitonic_step<8>(r2, lane, true);
+        r0 = bitonic_step<4>(r0, lane, true);  r1 = bitonic_step<4>(r1, lane, true);  r2 = bitonic_step<4>(r2, lane, true);
+        r0 = bitonic_step<2>(r0, lane, true);  r1 = bitonic_step<2>(r1, lane, true);  r2 = bitonic_step<2>(r2, lane, true);
+        r0 = bitonic_step<1>(r0, lane, true);  r1 = bitonic_step<1>(r1, lane, true);  r2 = bitonic_step<1>(r2, lane, true);
+        // drop a key equal to its predecessor (an id already in the list, or offered twice)
+        auto prev = [&](uint64_t v, uint64_t before) {
+            const uint32_t flo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)before, 63);
+            const uint32_t fhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(before >> 32), 63);
+            return ((uint64_t)wave_shr1((uint32_t)(v >> 32), fhi) << 32) | wave_shr1((uint32_t)v, flo);
+        };
+        const uint64_t p0 = prev(r0, kEmpty), p1 = prev(r1, r0), p2 = prev(r2, r1);
+        const bool dup0 = lane > 0 && r0 == p0 && r0 != kEmpty;
+        const bool dup1 = r1 == p1 && r1 != kEmpty;
+        const bool dup2 = r2 == p2 && r2 != kEmpty;
+        const uint64_t dm0 = __ballot(dup0), dm1 = __ballot(dup1), dm2 = __ballot(dup2);
+        if (dm0 | dm1) {
+            const uint64_t below = (1ull << lane) - 1ull;
+            const uint32_t d0 = __popcll(dm0), d1 = __popcll(dm1);
+            const uint32_t q0 = lane - __popcll(dm0 & below);
+            const uint32_t q1 = lane + 64 - d0 - __popcll(dm1 & below);
+            const uint32_t q2 = lane + 128 - d0 - d1 - __popcll(dm2 & below);
+            __builtin_amdgcn_wave_barrier();
+            if (!dup0) scratch[q0] = r0;
+            if (!dup1 && q1 < 128) scratch[q1] = r1;
+            if (!dup2 && q2 < 128) scratch[q2] = r2;
+            __builtin_amdgcn_wave_barrier();
+            r0 = scratch[lane];
+            r1 = scratch[lane + 64];
+            __builtin_amdgcn_wave_barrier();
+        }
+        k0 = lane < k ? r0 : kEmpty;
+        k1 = lane + 64 < k ? r1 : kEmpty;
+        set_bound(k);
+        return __ballot(k0 != o0 || k1 != o1) != 0;
     }
 };
 

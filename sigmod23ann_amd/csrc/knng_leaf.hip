@@ -25,7 +25,8 @@ constexpr uint32_t kLeafWaves = kLeafThreads / 64;
 
 struct LeafLds {
     uint32_t b16, kp, rs, w16;
-    uint32_t off_ids, off_bnd, off_nrm, off_rinv, off_mask, off_rows, off_rowbuf, off_cand, total;
+    uint32_t off_ids, off_bnd, off_nrm, off_rinv, off_mask, off_rows, off_rowbuf, off_cand, off_scr;
+    uint32_t total;
     uint32_t part_rows;  // FP32 rows that fit the row region at once (phase 2 re-rank)
 };
 
@@ -47,6 +48,8 @@ __host__ __device__ inline LeafLds leaf_lds_layout(uint32_t max_leaf, uint32_t d
     L.part_rows = (L.b16 * L.rs * 2) / (ld * 4);
     L.off_rowbuf = o; o += kLeafWaves * ld * 4;
     L.off_cand = o;   o += kLeafWaves * L.b16 * 2;
+    o = (o + 15) & ~15u;
+    L.off_scr = o;    o += kLeafWaves * 128 * 8;  // per-wave list scratch of the batched offer
     L.total = (o + 15) & ~15u;
     return L;
 }
@@ -72,6 +75,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     float* s_rowbuf = reinterpret_cast<float*>(lds + L.off_rowbuf) + wave * ld;
     uint16_t* s_cand = reinterpret_cast<uint16_t*>(lds + L.off_cand) + wave * L.b16;
+    uint64_t* s_scr = reinterpret_cast<uint64_t*>(lds + L.off_scr) + wave * 128;
 
     // ---- phase 0: gather, centre, scale ---------------------------------------------------------
     for (uint32_t t = tid; t < b16; t += kLeafThreads) {
@@ -222,24 +226,13 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
             bool changed = false;
             n_surv += S;
             for (uint32_t c0 = 0; c0 < S; c0 += 64) {
-                float dd = kFltMax;
-                uint32_t cid = kNoId;
+                uint64_t x = WaveList::kEmpty;
                 if (c0 + lane < S) {
                     const uint32_t c = s_cand[c0 + lane];
-                    cid = s_ids[c];
-                    dd = dist_ref(s_rowbuf, s_f32 + (size_t)(c - c_lo) * ld, d);
+                    x = WaveList::key(dist_ref(s_rowbuf, s_f32 + (size_t)(c - c_lo) * ld, d), s_ids[c]);
                 }
-                uint64_t acc = __ballot(dd < lst.bound);
-                while (acc) {
-                    const uint32_t src = __ffsll((unsigned long long)acc) - 1;
-                    acc &= acc - 1;
-                    const float nd = lane_bcast(dd, src);
-                    const uint32_t nid = lane_bcast(cid, src);
-                    if (lst.insert(nd, nid, k, lane)) {
-                        changed = true;
-                        ++n_ins;
-                    }
-                }
+                n_ins += __popcll(__ballot((uint32_t)(x >> 32) < __float_as_uint(lst.bound)));
+                changed |= lst.offer64(x, k, lane, s_scr);
             }
             if (changed) {
                 lst.store(a.knn_ids + (size_t)id_r * k, a.knn_d + (size_t)id_r * k, k, lane);

@@ -586,22 +586,16 @@ __global__ __launch_bounds__(256) void k_non_merge(NonRevArgs a, uint32_t* __res
     const uint32_t k = a.k;
     uint32_t* gid = a.knn_ids + (size_t)p * k;
     float* gd = a.knn_d + (size_t)p * k;
+    __shared__ uint64_t scr[4][128];
     WaveList lst;
     lst.load(gid, gd, k, lane);
     bool changed = false;
     for (uint32_t c0 = 0; c0 < m; c0 += 64) {
-        float dd = kFltMax;
-        uint32_t cid = kNoId;
-        if (c0 + lane < m) {
-            dd = a.nbuf_d[(size_t)p * a.cap + c0 + lane];
-            cid = a.nbuf_id[(size_t)p * a.cap + c0 + lane];
-        }
-        uint64_t acc = __ballot(dd < lst.bound);
-        while (acc) {
-            const uint32_t src = __ffsll((unsigned long long)acc) - 1;
-            acc &= acc - 1;
-            changed |= lst.insert(lane_bcast(dd, src), lane_bcast(cid, src), k, lane);
-        }
+        uint64_t x = WaveList::kEmpty;
+        if (c0 + lane < m)
+            x = WaveList::key(a.nbuf_d[(size_t)p * a.cap + c0 + lane],
+                              a.nbuf_id[(size_t)p * a.cap + c0 + lane]);
+        changed |= lst.offer64(x, k, lane, scr[threadIdx.x >> 6]);
     }
     if (changed) {
         lst.store(gid, gd, k, lane);
@@ -615,6 +609,7 @@ constexpr uint32_t kNonWaves = 4;
 __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a, const uint32_t* plist,
                                                                uint32_t np) {
     __shared__ __align__(16) float rowbuf[kNonWaves][kMaxDim];
+    __shared__ uint64_t scr[kNonWaves][128];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t w = blockIdx.x * kNonWaves + wave;
     if (w >= np) return;
@@ -633,18 +628,12 @@ __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a, const 
         if (a2 > 0 && sp[a2 - 1] == q) continue;  // repeated id 0 of an unfilled list
         const uint32_t* sq = a.snap + (size_t)q * k;
         for (uint32_t c0 = 0; c0 < k; c0 += 64) {
-            float dd = kFltMax;
-            uint32_t cid = kNoId;
+            uint64_t x = WaveList::kEmpty;
             if (c0 + lane < k) {
-                cid = sq[c0 + lane];
-                if (cid != p) dd = dist_ref(a.X + (size_t)cid * ld, xr, d);
+                const uint32_t cid = sq[c0 + lane];
+                if (cid != p) x = WaveList::key(dist_ref(a.X + (size_t)cid * ld, xr, d), cid);
             }
-            uint64_t acc = __ballot(dd < lst.bound);
-            while (acc) {
-                const uint32_t src = __ffsll((unsigned long long)acc) - 1;
-                acc &= acc - 1;
-                changed |= lst.insert(lane_bcast(dd, src), lane_bcast(cid, src), k, lane);
-            }
+            changed |= lst.offer64(x, k, lane, scr[wave]);
         }
     }
     if (changed) {
@@ -653,19 +642,31 @@ __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a, const 
     }
 }
 
-__global__ void k_row_norms(const float* __restrict__ X, uint32_t ld, uint32_t d, uint32_t n,
-                            float* __restrict__ out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* x = X + (size_t)i * ld;
+// FP32 squared norms for the screen (any summation order: the screen's margin covers it). Sixteen
+// lanes per row, float4 loads, so a wave reads four whole rows at a time.
+__global__ __launch_bounds__(256) void k_row_norms(const float* __restrict__ X, uint32_t ld,
+                                                   uint32_t d, uint32_t n, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 16 + (threadIdx.x >> 4), sc = threadIdx.x & 15;
+    const uint32_t ld4 = ld / 4;
     float s = 0.f;
-    for (uint32_t j = 0; j < d; ++j) s = __builtin_fmaf(x[j], x[j], s);
-    out[i] = s;
+    if (i < n) {
+        const float4* x = reinterpret_cast<const float4*>(X + (size_t)i * ld);
+        for (uint32_t q = sc; q < ld4; q += 16) {
+            const float4 v = x[q];
+            const uint32_t j = 4 * q;
+            if (j + 0 < d) s = __builtin_fmaf(v.x, v.x, s);
+            if (j + 1 < d) s = __builtin_fmaf(v.y, v.y, s);
+            if (j + 2 < d) s = __builtin_fmaf(v.z, v.z, s);
+            if (j + 3 < d) s = __builtin_fmaf(v.w, v.w, s);
+        }
+    }
+    for (uint32_t o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (i < n && sc == 0) out[i] = s;
 }
 
 void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float* out,
                       hipStream_t st) {
-    if (n) k_row_norms<<<(n + 255) / 256, 256, 0, st>>>(X, ld, d, n, out);
+    if (n) k_row_norms<<<(n + 15) / 16, 256, 0, st>>>(X, ld, d, n, out);
 }
 
 // ---- launchers ----------------------------------------------------------------------------------------
