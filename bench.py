@@ -48,6 +48,11 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=250_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--compare-fp32", action="store_true",
+                    help="with --precision bf16/fp16: also build the FP32 form at the same rounds and "
+                         "report the recall delta and the speed ratio")
+    ap.add_argument("--error-queries", type=int, default=10_000,
+                    help="sampled points whose returned distances are checked against FP64")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the build, run one plain device copy of this many MiB (a known "
                          "read/write volume for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -107,33 +112,31 @@ def main():
     dIds = K.DeviceArray(n * k * 4)
     dD = K.DeviceArray(n * k * 4)
 
-    for w in range(a.warmup):
-        t0 = time.perf_counter()
-        b.build_device(dX.ptr, n, d, ld, dIds.ptr, dD.ptr)
-        log(f"[bench] warmup {w}: {time.perf_counter() - t0:.3f} s")
-    times, stats = [], []
-    for s in range(a.steps):
-        K.lib().knng_synchronize()
-        t0 = time.perf_counter()
-        b.build_device(dX.ptr, n, d, ld, dIds.ptr, dD.ptr)
-        K.lib().knng_synchronize()
-        dt = time.perf_counter() - t0
-        times.append(dt)
-        st = b.stats()
-        stats.append(st)
-        log(f"[bench] step {s}: {dt:.3f} s  split {st['split_ms']:.1f} ms (side {st['side_ms']:.1f}) "
-            f"leaf {st['leaf_ms']:.1f} ms non {st['non_ms']:.1f} ms")
-    mean_t = float(np.mean(times))
-    value = n / mean_t
-    st = stats[-1]
+    def run_builds(builder, tag):
+        for w in range(a.warmup):
+            t0 = time.perf_counter()
+            builder.build_device(dX.ptr, n, d, ld, dIds.ptr, dD.ptr)
+            log(f"[bench] {tag} warmup {w}: {time.perf_counter() - t0:.3f} s")
+        times, stats = [], []
+        for s in range(a.steps):
+            K.lib().knng_synchronize()
+            t0 = time.perf_counter()
+            builder.build_device(dX.ptr, n, d, ld, dIds.ptr, dD.ptr)
+            K.lib().knng_synchronize()
+            dt = time.perf_counter() - t0
+            times.append(dt)
+            st = builder.stats()
+            stats.append(st)
+            log(f"[bench] {tag} step {s}: {dt:.3f} s  split {st['split_ms']:.1f} ms "
+                f"(side {st['side_ms']:.1f}) leaf {st['leaf_ms']:.1f} ms non {st['non_ms']:.1f} ms")
+        return times, stats
 
-    # ---- recall against exact FP64 k-NN of sampled points ---------------------------------------
-    rec = None
+    rng = np.random.default_rng(a.seed + 1)
     nq = min(a.recall_queries, n)
-    if nq > 0:
+    q = np.sort(rng.choice(n, size=nq, replace=False)).astype(np.uint32) if nq > 0 else None
+
+    def measure_recall(builder):
         t0 = time.perf_counter()
-        rng = np.random.default_rng(a.seed + 1)
-        q = np.sort(rng.choice(n, size=nq, replace=False)).astype(np.uint32)
         ids = dIds.download((n, k), np.uint32)
         got = np.ascontiguousarray(ids[q])
         dq = K.DeviceArray(nq * 4)
@@ -141,14 +144,71 @@ def main():
         dl = K.DeviceArray(nq * k * 4)
         dl.upload(got)
         dub = K.DeviceArray(nq * 8)
-        b.exact_upper_from_lists(dX.ptr, n, d, ld, dq.ptr, nq, dl.ptr, k, dub.ptr)
+        builder.exact_upper_from_lists(dX.ptr, n, d, ld, dq.ptr, nq, dl.ptr, k, dub.ptr)
         dti = K.DeviceArray(nq * k * 4)
         dtd = K.DeviceArray(nq * k * 8)
-        b.exact_knn_device(dX.ptr, n, d, ld, dq.ptr, nq, k, dub.ptr, 4 * k + 256, dti.ptr, dtd.ptr)
+        builder.exact_knn_device(dX.ptr, n, d, ld, dq.ptr, nq, k, dub.ptr, 4 * k + 256, dti.ptr,
+                                 dtd.ptr)
         truth = dti.download((nq, k), np.uint32)
         rec = K.recall(got, truth)
         log(f"[bench] recall@{k} on {nq} sampled points vs FP64: {rec:.4f} "
             f"({time.perf_counter() - t0:.1f} s)")
+        return rec, got
+
+    cmp32 = None
+    if a.compare_fp32 and a.precision != "fp32" and nq > 0:
+        prm32 = K.params(k=a.k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
+                         rounds=a.rounds, non_passes=a.non, precision=K.PREC_FP32, seed=a.seed)
+        b32 = K.Builder(prm32)
+        t32, _ = run_builds(b32, "fp32")
+        r32, _ = measure_recall(b32)
+        cmp32 = {"fp32_points_per_s": round(n / float(np.mean(t32)), 1), "fp32_recall": round(r32, 4)}
+        del b32
+
+    times, stats = run_builds(b, a.precision)
+    mean_t = float(np.mean(times))
+    value = n / mean_t
+    st = stats[-1]
+
+    # ---- recall against exact FP64 k-NN of sampled points ---------------------------------------
+    rec, prec_block = None, None
+    if nq > 0:
+        rec, got = measure_recall(b)
+        # returned distances against FP64 recomputation of the same pairs
+        ne = min(a.error_queries, nq)
+        if ne > 0:
+            Xh = dX.download((n, ld), np.float32)
+            ds_all = dD.download((n, k), np.float32)
+            qe = q[:ne]
+            ids_e = got[:ne].astype(np.int64)
+            errs, rels = [], []
+            for c0 in range(0, ne, 1000):
+                sl = slice(c0, min(ne, c0 + 1000))
+                xq = Xh[qe[sl], :d].astype(np.float64)
+                xn = Xh[ids_e[sl], :d].astype(np.float64)
+                d64 = ((xn - xq[:, None, :]) ** 2).sum(-1)
+                d32 = ds_all[qe[sl]].astype(np.float64)
+                m = d32 < 3e38
+                errs.append(np.abs(d32 - d64)[m])
+                rels.append((np.abs(d32 - d64) / np.maximum(d64, 1e-30))[m])
+            err = np.concatenate(errs)
+            relerr = np.concatenate(rels)
+            del Xh, ds_all
+            prec_block = {"precision": a.precision,
+                          "screen": {"fp32": "fp16 Gram with exact error margin",
+                                     "fp16": "fp16 Gram, no margin",
+                                     "bf16": "bf16 Gram, no margin"}[a.precision],
+                          "rerank": "exact reference-order FP32 for every screen survivor",
+                          "dist_abs_err_vs_fp64": {"max": float(err.max()), "mean": float(err.mean())},
+                          "dist_rel_err_vs_fp64": {"max": float(relerr.max()),
+                                                   "mean": float(relerr.mean())},
+                          "error_queries": ne}
+            if cmp32:
+                prec_block.update(cmp32)
+                prec_block["recall_delta_vs_fp32"] = round(rec - cmp32["fp32_recall"], 4)
+                prec_block["speedup_vs_fp32"] = round((n / mean_t) / cmp32["fp32_points_per_s"], 3)
+            log(f"[bench] distance error vs FP64 on {ne} points: max {err.max():.3e} "
+                f"mean {err.mean():.3e} (relative max {relerr.max():.3e})")
 
     # ---- roofline of the dominant kernel -----------------------------------------------------
     # Algorithmic bytes/flops per launch over the HIP-event launch time:
@@ -219,6 +279,7 @@ def main():
         "stages_ms": {"split": round(st["split_ms"], 2), "side": round(st["side_ms"], 2),
                       "leaf": round(st["leaf_ms"], 2), "non": round(st["non_ms"], 2),
                       "total": round(st["total_ms"], 2)},
+        "precision_form": prec_block,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
