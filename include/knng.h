@@ -34,13 +34,14 @@ extern "C" {
 #define KNNG_PREC_BF16 2 /* bf16 MFMA screen, no margin, FP32 re-rank of the survivors */
 
 #define KNNG_FLAG_DIRECT_NON 1u /* neighbours-of-neighbours by the per-point direct kernel */
+#define KNNG_FLAG_BIG_LEAF 2u   /* every leaf through the big-bucket kernel (the path of leaves > 512) */
 
 /* Build parameters. The reference hard-codes k = dims = 100 (src/Constants.hpp:23-24),
  * maxGroupSize 200/400 alternating (src/SolutionKmeans.hpp:582-587), knnIterations = 1 (:591),
  * the 3000-point small-branch threshold (:156) and a wall-clock round budget (:562). */
 typedef struct knng_params {
     uint32_t k;            /* neighbours per point, 1..128 */
-    uint32_t leaf_max;     /* largest leaf bucket, inclusive (reference maxGroupSize - 1) */
+    uint32_t leaf_max;     /* largest leaf bucket, inclusive (reference maxGroupSize - 1); <= 65536 */
     uint32_t leaf_max_odd; /* leaf_max on odd rounds; 0 = leaf_max */
     uint32_t small_max;    /* nodes below this size take the small branch, <= 4096 */
     uint32_t iterations;   /* Lloyd iterations per split */

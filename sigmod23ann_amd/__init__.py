@@ -33,6 +33,7 @@ PREC_FP16 = 1
 PREC_BF16 = 2
 
 FLAG_DIRECT_NON = 1
+FLAG_BIG_LEAF = 2
 
 GEN_LOWRANK = 0
 GEN_MIXTURE = 1

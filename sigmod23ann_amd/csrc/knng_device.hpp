@@ -248,14 +248,15 @@ struct WaveList {
     // keys of this wave's LDS. Same set semantics as inserting them one by one: only keys below
     // the bound and not already present enter, and the list keeps the k smallest. With ties at the
     // k-th distance the batch keeps the smaller ids, where one-by-one insertion keeps whichever
-    // arrived first. Few offers go through insert(); more go through a bitonic sort of the
+    // arrived first. Up to `one_by_one` offers go through insert(); more go through a bitonic sort of the
     // batch, a half-cleaner against the list and a bitonic merge of the 128 smallest, then
     // adjacent equal keys (an id offered again) are dropped. Returns whether the list changed.
-    __device__ inline bool offer64(uint64_t x, uint32_t k, uint32_t lane, uint64_t* scratch) {
+    __device__ inline bool offer64(uint64_t x, uint32_t k, uint32_t lane, uint64_t* scratch,
+                                   uint32_t one_by_one = 4) {
         if (!((uint32_t)(x >> 32) < __float_as_uint(bound))) x = kEmpty;
         uint64_t m = __ballot(x != kEmpty);
         if (!m) return false;
-        if (__popcll(m) <= 4) {
+        if ((uint32_t)__popcll(m) <= one_by_one) {
             bool changed = false;
             while (m) {
                 const uint32_t src = __ffsll((unsigned long long)m) - 1;
@@ -386,7 +387,7 @@ __device__ inline void screen_vals(float v0, float v1, uint32_t d, uint32_t rs,
         h0 = to_bf16_rne(v0);
         h1 = to_bf16_rne(v1);
     }
-    hrow[lane] = lane < d ? h0 : 0;
+    if (lane < rs) hrow[lane] = lane < d ? h0 : 0;  // rs < 64 when d < 56
     if (lane + 64 < rs) hrow[lane + 64] = lane + 64 < d ? h1 : 0;
     rinv = ldexpf(1.0f, -sh);
 }

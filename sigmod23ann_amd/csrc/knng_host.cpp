@@ -114,8 +114,8 @@ struct HostNode {
 
 bool valid_params(const knng_params& p, std::string& why) {
     if (p.k < 1 || p.k > kMaxK) { why = "k must be in [1, 128]"; return false; }
-    if (p.leaf_max < 1 || p.leaf_max > kMaxLeaf) { why = "leaf_max must be in [1, 512]"; return false; }
-    if (p.leaf_max_odd > kMaxLeaf) { why = "leaf_max_odd must be <= 512"; return false; }
+    if (p.leaf_max < 1 || p.leaf_max > kMaxLeafAny) { why = "leaf_max must be in [1, 65536]"; return false; }
+    if (p.leaf_max_odd > kMaxLeafAny) { why = "leaf_max_odd must be <= 65536"; return false; }
     if (p.small_max > kMaxSmallNode) { why = "small_max must be <= 4096"; return false; }
     if (p.precision > KNNG_PREC_BF16) { why = "unknown precision"; return false; }
     return true;
@@ -146,11 +146,13 @@ struct Builder {
     PinBuf<ChunkDesc> h_chunks;
     PinBuf<TileDesc> h_tiles;
     PinBuf<LeafDesc> h_leaves;
+    PinBuf<BigBlock> h_big;
 
     // top-k state
     DevBuf<uint32_t> knn_ids, snap;
     DevBuf<float> knn_d, bound;
     DevBuf<LeafDesc> leaves;
+    DevBuf<BigBlock> bigblocks;
     DevBuf<uint64_t> counters;
 
     // exact k-NN scratch
@@ -481,47 +483,86 @@ struct Builder {
         return KNNG_OK;
     }
 
+    // Leaves are binned by size: each bin of the LDS kernel gets LDS sized for its own largest
+    // leaf (small leaves then fit several workgroups per CU), and leaves beyond it (or all of them
+    // with KNNG_FLAG_BIG_LEAF) go to the big-bucket kernel as 64-row blocks.
     int leaf_stage(const std::vector<LeafDesc>& lv) {
         if (lv.empty()) return KNNG_OK;
         if (d > 128) return fail(KNNG_ERR_UNSUPPORTED, "leaf stage supports d <= 128");
-        uint32_t mx = 0;
+        static constexpr uint32_t kEdges[4] = {64, 128, 256, kMaxLeaf};
+        std::vector<LeafDesc> bin[4];
+        std::vector<BigBlock> big;
         uint64_t pairs = 0;
+        const bool force_big = (prm.flags & KNNG_FLAG_BIG_LEAF) != 0;
         for (auto& l : lv) {
-            mx = std::max(mx, l.size);
             pairs += (uint64_t)l.size * (l.size - 1) / 2;
+            if (l.size < 2) continue;
+            int bi = 0;
+            while (bi < 4 && l.size > kEdges[bi]) ++bi;
+            if (force_big || (bi < 4 && leaf_lds_bytes(kEdges[bi], d, ld) > 160 * 1024)) bi = 4;
+            if (bi < 4) {
+                bin[bi].push_back(l);
+            } else {
+                for (uint32_t r0 = 0; r0 < l.size; r0 += kLeafBigRows) big.push_back({l.start, l.size, r0});
+            }
         }
-        const size_t lds = leaf_lds_bytes(mx, d, ld);
-        if (lds > 160 * 1024)
-            return fail(KNNG_ERR_UNSUPPORTED, "leaf bucket too large for LDS at this d");
-        KNNG_TRY(h_leaves.ensure(lv.size()));
-        std::memcpy(h_leaves.p, lv.data(), lv.size() * sizeof(LeafDesc));
-        KNNG_TRY(leaves.ensure(lv.size()));
-        KNNG_HIP(hipMemcpyAsync(leaves.p, h_leaves.p, lv.size() * sizeof(LeafDesc), hipMemcpyHostToDevice, st));
+        if (!big.empty() && leaf_big_lds_bytes(d, ld) > 160 * 1024)
+            return fail(KNNG_ERR_UNSUPPORTED, "big-bucket leaf kernel does not fit LDS at this d");
+        size_t nl = 0;
+        for (auto& v : bin) nl += v.size();
+        KNNG_TRY(h_leaves.ensure(std::max<size_t>(nl, 1)));
+        KNNG_TRY(leaves.ensure(std::max<size_t>(nl, 1)));
+        size_t off = 0;
+        for (auto& v : bin) {
+            if (!v.empty()) std::memcpy(h_leaves.p + off, v.data(), v.size() * sizeof(LeafDesc));
+            off += v.size();
+        }
+        if (nl) KNNG_HIP(hipMemcpyAsync(leaves.p, h_leaves.p, nl * sizeof(LeafDesc), hipMemcpyHostToDevice, st));
+        if (!big.empty()) {
+            KNNG_TRY(h_big.ensure(big.size()));
+            std::memcpy(h_big.p, big.data(), big.size() * sizeof(BigBlock));
+            KNNG_TRY(bigblocks.ensure(big.size()));
+            KNNG_HIP(hipMemcpyAsync(bigblocks.p, h_big.p, big.size() * sizeof(BigBlock), hipMemcpyHostToDevice, st));
+        }
         LeafArgs a{};
         a.X = X;
         a.ld = ld;
         a.d = d;
         a.k = prm.k;
         a.perm = perm.p;
-        a.leaves = leaves.p;
-        a.nleaves = (uint32_t)lv.size();
         a.knn_ids = knn_ids.p;
         a.knn_d = knn_d.p;
         a.bound = bound.p;
         a.screen = prm.precision == KNNG_PREC_FP32 ? kScreenExact : kScreenApprox;
         a.rerank_cap = prm.rerank;
         a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
-        a.max_leaf = mx;
         a.counters = counters.p;
         KNNG_HIP(hipEventRecord(ev[4], st));
-        launch_leaf(a, st);
+        off = 0;
+        for (auto& v : bin) {
+            if (!v.empty()) {
+                uint32_t mx = 0;
+                for (auto& l : v) mx = std::max(mx, l.size);
+                a.leaves = leaves.p + off;
+                a.nleaves = (uint32_t)v.size();
+                a.max_leaf = mx;
+                launch_leaf(a, st);
+                stats.leaf_launches++;
+            }
+            off += v.size();
+        }
+        if (!big.empty()) {
+            a.leaves = nullptr;
+            a.nleaves = 0;
+            launch_leaf_big(a, bigblocks.p, (uint32_t)big.size(), st);
+            stats.leaf_launches++;
+        }
         KNNG_HIP(hipGetLastError());
         KNNG_HIP(hipEventRecord(ev[5], st));
         KNNG_HIP(hipEventSynchronize(ev[5]));
         float ms = 0.f;
         KNNG_HIP(hipEventElapsedTime(&ms, ev[4], ev[5]));
         stats.leaf_ms += ms;
-        stats.leaf_launches++;
         stats.leaves += lv.size();
         stats.leaf_pairs += pairs;
         return KNNG_OK;
@@ -932,7 +973,7 @@ int knng_leaf_stage(knng_handle* h, const float* points, uint32_t n, uint32_t d,
     std::vector<knng::LeafDesc> lv(n_leaves);
     for (uint32_t i = 0; i < n_leaves; ++i) {
         if ((uint64_t)leaf_start[i] + leaf_size[i] > n) return fail(KNNG_ERR_INVALID, "leaf outside [0, n)");
-        if (leaf_size[i] > knng::kMaxLeaf) return fail(KNNG_ERR_UNSUPPORTED, "leaf larger than 512");
+        if (leaf_size[i] > knng::kMaxLeafAny) return fail(KNNG_ERR_UNSUPPORTED, "leaf larger than 65536");
         lv[i] = {leaf_start[i], leaf_size[i]};
     }
     KNNG_TRY(b.perm.ensure(n));

@@ -9,6 +9,8 @@ namespace knng {
 constexpr uint32_t kTileSize = 1024;       // partition positions per block
 constexpr uint32_t kMaxSmallNode = 4096;   // small-branch nodes are summed by one block
 constexpr uint32_t kMaxLeaf = 512;         // largest leaf bucket handled by the LDS leaf kernel
+constexpr uint32_t kMaxLeafAny = 65536;    // largest leaf bucket (beyond kMaxLeaf: k_leaf_big)
+constexpr uint32_t kLeafBigRows = 64;      // rows of a bucket per k_leaf_big workgroup
 constexpr uint32_t kMaxK = 128;            // neighbours per point
 
 enum : uint32_t { kNodeActive = 0, kNodeFailed = 1, kNodeInjected = 2 };
@@ -83,6 +85,13 @@ struct LeafArgs {
     uint64_t* counters;    // [0] survivors screened, [1] exact distances, [2] inserts (nullable)
 };
 void launch_leaf(const LeafArgs& a, hipStream_t st);
+
+// Buckets beyond the LDS leaf kernel: one workgroup per kLeafBigRows-row block of a bucket.
+struct BigBlock {
+    uint32_t start, size, row0;  // bucket range in perm, first row of this block
+};
+size_t leaf_big_lds_bytes(uint32_t d, uint32_t ld);
+void launch_leaf_big(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks, hipStream_t st);
 
 void launch_init_knn(uint32_t* ids, float* dists, float* bound, uint32_t n, uint32_t k,
                      hipStream_t st);

@@ -247,6 +247,281 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
     }
 }
 
+// ---- buckets too large for LDS ------------------------------------------------------------------
+// One workgroup (16 waves) per 64-row block of a bucket. Each wave owns 4 of the rows and keeps
+// their sorted lists in registers for the whole kernel (a row's FP32 values are re-read from L2
+// for the column blocks where it has survivors). The bucket's rows stream
+// through LDS 64 at a time (16-bit screening rows, FP32 rows); for each such column block the
+// workgroup screens its 64 x 64 pairs on MFMA (one 16 x 16 tile per wave) (same scaling, centre and margin as k_leaf), then
+// each wave re-ranks its rows' survivors exactly and offers them in one batch. Every row's list
+// belongs to one workgroup, so there are no races; a pair (i, j) is computed once from each side,
+// with the same distance bits both times.
+constexpr uint32_t kBigThreads = 1024;
+constexpr uint32_t kBigWaves = kBigThreads / 64;
+constexpr uint32_t kBigRows = kLeafBigRows;
+constexpr uint32_t kBigCols = 64;
+constexpr uint32_t kBigRowsPerWave = kBigRows / kBigWaves;  // 4
+
+struct BigLds {
+    uint32_t rs;
+    uint32_t off_arow, off_anrm, off_arinv, off_abnd, off_crow, off_cnrm, off_crinv, off_cid;
+    uint32_t off_cf32, off_mask, off_rowbuf, off_scr, total;
+};
+
+__host__ __device__ inline BigLds big_lds_layout(uint32_t d, uint32_t ld) {
+    BigLds L;
+    L.rs = leaf_row_stride(d);
+    uint32_t o = 0;
+    L.off_arow = o;  o += kBigRows * L.rs * 2;
+    o = (o + 15) & ~15u;
+    L.off_anrm = o;  o += kBigRows * 4;
+    L.off_arinv = o; o += kBigRows * 4;
+    L.off_abnd = o;  o += kBigRows * 4;
+    L.off_crow = o;  o += kBigCols * L.rs * 2;
+    o = (o + 15) & ~15u;
+    L.off_cnrm = o;  o += kBigCols * 4;
+    L.off_crinv = o; o += kBigCols * 4;
+    L.off_cid = o;   o += kBigCols * 4;
+    o = (o + 15) & ~15u;
+    L.off_cf32 = o;  o += kBigCols * ld * 4;
+    L.off_mask = o;  o += kBigRows * 8;
+    L.off_rowbuf = o; o += kBigWaves * ld * 4;
+    L.off_scr = o;   o += kBigWaves * 128 * 8;
+    L.total = (o + 15) & ~15u;
+    return L;
+}
+
+size_t leaf_big_lds_bytes(uint32_t d, uint32_t ld) { return big_lds_layout(d, ld).total; }
+
+template <uint32_t HALF, uint32_t KS>
+__global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigBlock* __restrict__ blocks) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const BigBlock bb = blocks[blockIdx.x];
+    const uint32_t b = bb.size, r0 = bb.row0;
+    const uint32_t nrows = min(kBigRows, b - r0);
+    const uint32_t d = a.d, ld = a.ld, k = a.k;
+    const BigLds L = big_lds_layout(d, ld);
+    uint16_t* s_arow = reinterpret_cast<uint16_t*>(lds + L.off_arow);
+    float* s_anrm = reinterpret_cast<float*>(lds + L.off_anrm);
+    float* s_arinv = reinterpret_cast<float*>(lds + L.off_arinv);
+    float* s_abnd = reinterpret_cast<float*>(lds + L.off_abnd);
+    uint16_t* s_crow = reinterpret_cast<uint16_t*>(lds + L.off_crow);
+    float* s_cnrm = reinterpret_cast<float*>(lds + L.off_cnrm);
+    float* s_crinv = reinterpret_cast<float*>(lds + L.off_crinv);
+    uint32_t* s_cid = reinterpret_cast<uint32_t*>(lds + L.off_cid);
+    float* s_cf32 = reinterpret_cast<float*>(lds + L.off_cf32);
+    uint16_t* s_mask = reinterpret_cast<uint16_t*>(lds + L.off_mask);  // 4 x 16 bits per row
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float* s_rowbuf = reinterpret_cast<float*>(lds + L.off_rowbuf) + wave * ld;
+    uint64_t* s_scr = reinterpret_cast<uint64_t*>(lds + L.off_scr) + wave * 128;
+    const uint32_t* pm = a.perm + bb.start;
+    const float* mu = a.X + (size_t)pm[0] * ld;  // centre: the bucket's first row, as in k_leaf
+    const float mu0 = lane < d ? mu[lane] : 0.f;
+    const float mu1 = lane + 64 < d ? mu[lane + 64] : 0.f;
+
+    // ---- this wave's rows: lists in registers, 16-bit screening rows in LDS --------------------
+    WaveList lst[kBigRowsPerWave];
+    uint32_t pid[kBigRowsPerWave];
+#pragma unroll
+    for (uint32_t j = 0; j < kBigRowsPerWave; ++j) {
+        const uint32_t rr = kBigRowsPerWave * wave + j;
+        const bool valid = rr < nrows;
+        pid[j] = valid ? __builtin_amdgcn_readfirstlane(pm[r0 + rr]) : kNoId;
+        lst[j].k0 = lst[j].k1 = WaveList::kEmpty;
+        lst[j].bound = 0.f;
+        if (valid) lst[j].load(a.knn_ids + (size_t)pid[j] * k, a.knn_d + (size_t)pid[j] * k, k, lane);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kBigRowsPerWave; ++j) {
+        const uint32_t rr = kBigRowsPerWave * wave + j;
+        uint16_t* hrow = s_arow + rr * L.rs;
+        if (rr < nrows) {
+            const float* x = a.X + (size_t)pid[j] * ld;
+            const float x0 = lane < d ? x[lane] : 0.f, x1 = lane + 64 < d ? x[lane + 64] : 0.f;
+            float nrm, rinv;
+            screen_vals<HALF>(lane < d ? x0 - mu0 : 0.f, lane + 64 < d ? x1 - mu1 : 0.f, d,
+                              L.rs, hrow, lane, nrm, rinv);
+            if (lane == 0) {
+                s_anrm[rr] = nrm;
+                s_arinv[rr] = rinv;
+                s_abnd[rr] = lst[j].bound;
+            }
+        } else {
+            for (uint32_t t = lane; t < L.rs; t += 64) hrow[t] = 0;
+            if (lane == 0) {
+                s_anrm[rr] = 0.f;
+                s_arinv[rr] = 0.f;
+                s_abnd[rr] = 0.f;
+            }
+        }
+    }
+
+    const bool exact = a.screen == kScreenExact;
+    const float margin_c = HalfTraits<HALF>::kMargin;
+    const uint32_t lrow = lane & 15, lk = (lane >> 4) * 4;
+    const uint32_t strip = wave >> 2, tile = wave & 3;  // phase 1: 4 strips x 4 tiles
+    bool changed[kBigRowsPerWave];
+#pragma unroll
+    for (uint32_t j = 0; j < kBigRowsPerWave; ++j) changed[j] = false;
+    uint32_t n_surv = 0, n_ins = 0;
+
+    for (uint32_t c0 = 0; c0 < b; c0 += kBigCols) {
+        const uint32_t nc = min(kBigCols, b - c0);
+        __syncthreads();  // the previous block's phase 2 is done with the column buffers
+        for (uint32_t cc = wave; cc < kBigCols; cc += kBigWaves) {
+            uint16_t* hrow = s_crow + cc * L.rs;
+            float* frow = s_cf32 + cc * ld;
+            if (cc < nc) {
+                const uint32_t id = pm[c0 + cc];
+                const float* x = a.X + (size_t)id * ld;
+                const float x0 = lane < ld ? x[lane] : 0.f;
+                const float x1 = lane + 64 < ld ? x[lane + 64] : 0.f;
+                if (lane < ld) frow[lane] = x0;
+                if (lane + 64 < ld) frow[lane + 64] = x1;
+                float nrm, rinv;
+                screen_vals<HALF>(lane < d ? x0 - mu0 : 0.f, lane + 64 < d ? x1 - mu1 : 0.f, d, L.rs,
+                                  hrow, lane, nrm, rinv);
+                if (lane == 0) {
+                    s_cid[cc] = id;
+                    s_cnrm[cc] = nrm;
+                    s_crinv[cc] = rinv;
+                }
+            } else {
+                for (uint32_t t = lane; t < L.rs; t += 64) hrow[t] = 0;
+                if (lane == 0) {
+                    s_cid[cc] = kNoId;
+                    s_cnrm[cc] = 0.f;
+                    s_crinv[cc] = 0.f;
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- phase 1: screen this wave's tile ------------------------------------------------
+        {
+            half4_t af[KS];
+#pragma unroll
+            for (uint32_t kk = 0; kk < KS; ++kk)
+                af[kk] = *reinterpret_cast<const half4_t*>(s_arow + (16 * strip + lrow) * L.rs + kk * 16 + lk);
+            float rn[4], rinv[4], thr_base[4];
+            uint32_t rr[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                rr[i] = 16 * strip + (lane >> 4) * 4 + i;
+                rn[i] = s_anrm[rr[i]];
+                rinv[i] = s_arinv[rr[i]];
+                const float bnd = s_abnd[rr[i]];
+                thr_base[i] = exact ? bnd * (1.0f + 1.0f / 65536.0f) : bnd;
+            }
+            {
+                const uint32_t t = tile;
+                half4_t bf[KS];
+#pragma unroll
+                for (uint32_t kk = 0; kk < KS; ++kk)
+                    bf[kk] = *reinterpret_cast<const half4_t*>(s_crow + (16 * t + lrow) * L.rs + kk * 16 + lk);
+                float4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (uint32_t kk = 0; kk < KS; ++kk) acc = mfma16(af[kk], bf[kk], acc, HALF);
+                const uint32_t c = 16 * t + lrow;
+                const float cn = s_cnrm[c], cinv = s_crinv[c];
+                uint64_t bits[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float g = acc[i] * (rinv[i] * cinv);
+                    const float nsum = rn[i] + cn;
+                    const float approx = nsum - 2.0f * g;
+                    const float thr = exact ? thr_base[i] + margin_c * nsum : thr_base[i];
+                    const bool ok = c < nc && rr[i] < nrows && r0 + rr[i] != c0 + c && approx < thr;
+                    bits[i] = __ballot(ok);
+                }
+                if (lane < 16) {
+                    const uint32_t g = lane >> 2, i = lane & 3;
+                    const uint64_t bw = i == 0 ? bits[0] : i == 1 ? bits[1] : i == 2 ? bits[2] : bits[3];
+                    s_mask[(16 * strip + lane) * 4 + t] = (uint16_t)(bw >> (16 * g));
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- phase 2: exact re-rank of each owned row's survivors, one batched offer ----------
+        for (uint32_t j = 0; j < kBigRowsPerWave; ++j) {
+            const uint32_t rr = kBigRowsPerWave * wave + j;
+            if (rr >= nrows) break;
+            const uint64_t w64 = *reinterpret_cast<const uint64_t*>(s_mask + rr * 4);
+            if (!w64) continue;
+            // this row's state into fixed registers (constant-index selects keep the arrays in VGPRs)
+            WaveList cur = lst[0];
+            uint32_t id = pid[0];
+#pragma unroll
+            for (uint32_t jj = 1; jj < kBigRowsPerWave; ++jj)
+                if (j == jj) {
+                    cur = lst[jj];
+                    id = pid[jj];
+                }
+            const float* xr = a.X + (size_t)id * ld;
+            if (lane < ld) s_rowbuf[lane] = xr[lane];
+            if (lane + 64 < ld) s_rowbuf[lane + 64] = xr[lane + 64];
+            __builtin_amdgcn_wave_barrier();
+            uint64_t x = WaveList::kEmpty;
+            if ((w64 >> lane) & 1ull)
+                x = WaveList::key(dist_ref(s_rowbuf, s_cf32 + (size_t)lane * ld, d), s_cid[lane]);
+            n_surv += __popcll(w64);
+            n_ins += __popcll(__ballot((uint32_t)(x >> 32) < __float_as_uint(cur.bound)));
+            const bool ch = cur.offer64(x, k, lane, s_scr);
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (uint32_t jj = 0; jj < kBigRowsPerWave; ++jj)
+                if (j == jj) {
+                    lst[jj] = cur;
+                    changed[jj] |= ch;
+                }
+            if (lane == 0) s_abnd[rr] = cur.bound;
+        }
+    }
+
+#pragma unroll
+    for (uint32_t j = 0; j < kBigRowsPerWave; ++j) {
+        if (changed[j]) {
+            lst[j].store(a.knn_ids + (size_t)pid[j] * k, a.knn_d + (size_t)pid[j] * k, k, lane);
+            if (lane == 0) a.bound[pid[j]] = lst[j].bound;
+        }
+    }
+    if (a.counters && lane == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)n_surv);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[2]), (unsigned long long)n_ins);
+    }
+}
+
+template <uint32_t HALF, uint32_t KS>
+static void launch_leaf_big_t(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks, size_t lds,
+                              hipStream_t st) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf_big<HALF, KS>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_leaf_big<HALF, KS><<<nblocks, kBigThreads, lds, st>>>(a, blocks);
+}
+
+template <uint32_t HALF>
+static void launch_leaf_big_h(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks, size_t lds,
+                              hipStream_t st) {
+    switch ((a.d + 15) / 16) {
+        case 1: launch_leaf_big_t<HALF, 1>(a, blocks, nblocks, lds, st); break;
+        case 2: launch_leaf_big_t<HALF, 2>(a, blocks, nblocks, lds, st); break;
+        case 3: launch_leaf_big_t<HALF, 3>(a, blocks, nblocks, lds, st); break;
+        case 4: launch_leaf_big_t<HALF, 4>(a, blocks, nblocks, lds, st); break;
+        case 5: launch_leaf_big_t<HALF, 5>(a, blocks, nblocks, lds, st); break;
+        case 6: launch_leaf_big_t<HALF, 6>(a, blocks, nblocks, lds, st); break;
+        case 7: launch_leaf_big_t<HALF, 7>(a, blocks, nblocks, lds, st); break;
+        default: launch_leaf_big_t<HALF, 8>(a, blocks, nblocks, lds, st); break;
+    }
+}
+
+void launch_leaf_big(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks, hipStream_t st) {
+    if (!nblocks) return;
+    const size_t lds = leaf_big_lds_bytes(a.d, a.ld);
+    if (a.half_kind == 0) launch_leaf_big_h<0>(a, blocks, nblocks, lds, st);
+    else launch_leaf_big_h<1>(a, blocks, nblocks, lds, st);
+}
+
 size_t leaf_lds_bytes(uint32_t max_leaf, uint32_t d, uint32_t ld) {
     return leaf_lds_layout(max_leaf, d, ld).total;
 }

@@ -134,6 +134,49 @@ def test_leaf_stage_matches_oracle(k, leaf, d):
     assert_lists_equal_up_to_ties(ids2, ds2, o2[0], o2[1])
 
 
+@pytest.mark.parametrize("k,leaf,d", [(100, 256, 100), (10, 400, 100), (32, 128, 64), (100, 400, 128)])
+def test_leaf_stage_big_kernel_matches_oracle(k, leaf, d):
+    # the big-bucket kernel (64-row blocks, streamed column blocks) forced onto ordinary leaves
+    X = mixture(12000, d, 40, 21)
+    kw = dict(k=k, leaf_max=leaf, small_max=3000, iterations=1, seed=9)
+    o_prm = oracle.make_params(**kw)
+    b = K.Builder(K.params(flags=K.FLAG_BIG_LEAF, **kw))
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    ids, ds = b.leaf_stage(X, perm, ls, lz)
+    o_ids, o_ds, o_sz = oracle.leaf_stage(X, k, perm, ls, lz)
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    perm2, ls2, lz2 = oracle.split_round(X, o_prm, 1)
+    ids2, ds2 = b.leaf_stage(X, perm2, ls2, lz2, state=(ids, ds))
+    o2 = oracle.leaf_stage(X, k, perm2, ls2, lz2, state=(o_ids, o_ds, o_sz))
+    assert_lists_equal_up_to_ties(ids2, ds2, o2[0], o2[1])
+
+
+@pytest.mark.parametrize("d,leaf", [(100, 2000), (128, 1500), (37, 3000)])
+def test_leaf_stage_large_buckets(d, leaf):
+    # buckets beyond the LDS kernel (up to `leaf` points) against brute force
+    X = mixture(10000, d, 6, 23)
+    kw = dict(k=100, leaf_max=leaf, small_max=3000, iterations=1, seed=5)
+    o_prm = oracle.make_params(**kw)
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    assert lz.max() > 512
+    b = K.Builder(K.params(**kw))
+    ids, ds = b.leaf_stage(X, perm, ls, lz)
+    o_ids, o_ds, o_sz = oracle.leaf_stage(X, 100, perm, ls, lz)
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    perm2, ls2, lz2 = oracle.split_round(X, o_prm, 1)
+    ids2, ds2 = b.leaf_stage(X, perm2, ls2, lz2, state=(ids, ds))
+    o2 = oracle.leaf_stage(X, 100, perm2, ls2, lz2, state=(o_ids, o_ds, o_sz))
+    assert_lists_equal_up_to_ties(ids2, ds2, o2[0], o2[1])
+
+
+def test_build_large_leaves_matches_oracle():
+    X = mixture(12000, 100, 30, 43)
+    kw = dict(k=50, leaf_max=1500, small_max=1000, iterations=1, rounds=2, non_passes=1, seed=19)
+    ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
+    o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(**kw))
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+
+
 def test_leaf_stage_lowrank_unit_norm():
     # close pairs on the unit sphere: the screen's margin must keep every true candidate
     X = lowrank(20000, 100, 8)
