@@ -33,6 +33,9 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="lowrank", choices=["lowrank", "zipf128"],
+                    help="lowrank: the headline 10M x 100 build; zipf128: one round (split + leaf "
+                         "stage) over 10M x 128 Zipf-mixture points with buckets up to --leaf")
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=10_000_000)
@@ -89,6 +92,54 @@ def copy_calibration(mib):
     log(f"[bench] copy calibration: {mib} MiB read, {mib} MiB written")
 
 
+def zipf_round(a, K):
+    """One round of the batching form: 2-means split + every bucket's all-pairs stage."""
+    n, d, k = a.n, (a.d if a.d != 100 else 128), a.k
+    leaf = a.leaf if a.leaf != 400 else 4096
+    ld = (d + 3) & ~3
+    prm = K.params(k=k, leaf_max=leaf, small_max=3000, iterations=a.iterations, rounds=1,
+                   non_passes=0, precision=K.PREC_FP32, seed=a.seed)
+    b = K.Builder(prm)
+    log(f"[bench] generating {n}x{d} Zipf-mixture points (20,000 clusters) on the device")
+    dX = K.DeviceArray(n * ld * 4)
+    b.generate_device(K.GEN_ZIPF, n, d, ld, a.seed, dX.ptr)
+    dIds = K.DeviceArray(n * k * 4)
+    dD = K.DeviceArray(n * k * 4)
+    times, stats = [], []
+    for s in range(a.warmup + a.steps):
+        K.lib().knng_synchronize()
+        t0 = time.perf_counter()
+        b.build_device(dX.ptr, n, d, ld, dIds.ptr, dD.ptr)
+        K.lib().knng_synchronize()
+        dt = time.perf_counter() - t0
+        st = b.stats()
+        log(f"[bench] zipf round {s}: {dt:.3f} s  split {st['split_ms']:.1f} ms (side {st['side_ms']:.1f}) "
+            f"leaf {st['leaf_ms']:.1f} ms, {st['leaves']} buckets")
+        if s >= a.warmup:
+            times.append(dt)
+            stats.append(st)
+    st = stats[-1]
+    mean_t = float(np.mean(times))
+    side_bytes = st["side_rows"] * (4 * d + 4 + 1)
+    gram_flops = 2.0 * d * st["leaf_pairs"]
+    out = {
+        "metric": "one round (split + all buckets) of the batching form, 10M x 128 Zipf mixture",
+        "value": round(mean_t * 1e3, 2), "unit": "ms per round", "higher_is_better": False,
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"Zipf(1.1) mixture {n}x{d}, 20000 clusters, sigma U[0.3,2], "
+                               f"leaf<={leaf}, k={k}", "n": n, "d": d, "k": k, "leaf_max": leaf},
+        "stages_ms": {"split": round(st["split_ms"], 2), "side": round(st["side_ms"], 2),
+                      "leaf": round(st["leaf_ms"], 2), "total": round(st["total_ms"], 2)},
+        "buckets": int(st["leaves"]), "leaf_pairs": int(st["leaf_pairs"]),
+        "split_side_hbm": {"achieved": round(side_bytes / (st["side_ms"] * 1e-3) / 1e9, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s"},
+        "leaf_gram_mfma": {"achieved": round(gram_flops / (st["leaf_ms"] * 1e-3) / 1e12, 2),
+                           "peak": MFMA_FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(gram_flops / (st["leaf_ms"] * 1e-3) / 1e12 / MFMA_FP16_PEAK_TFLOPS, 5)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -96,6 +147,10 @@ def main():
               "shard across GPUs in this version", file=sys.stderr)
         sys.exit(2)
     import sigmod23ann_amd as K
+
+    if a.workload == "zipf128":
+        zipf_round(a, K)
+        return
 
     if a.copy_calibration_mib > 0:
         copy_calibration(a.copy_calibration_mib)

@@ -46,6 +46,7 @@ __device__ inline float dot_ref(const float* __restrict__ a, const float* __rest
     const uint32_t main = d & ~7u;
     float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f, acc4 = 0.f, acc5 = 0.f, acc6 = 0.f,
           acc7 = 0.f;
+#pragma unroll 4
     for (uint32_t i = 0; i < main; i += 8) {
         const float4 a0 = *reinterpret_cast<const float4*>(a + i);
         const float4 a1 = *reinterpret_cast<const float4*>(a + i + 4);
@@ -72,6 +73,7 @@ __device__ inline float dist_ref(const float* __restrict__ a, const float* __res
     const uint32_t main = d & ~7u;
     float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f, acc4 = 0.f, acc5 = 0.f, acc6 = 0.f,
           acc7 = 0.f;
+#pragma unroll 4
     for (uint32_t i = 0; i < main; i += 8) {
         const float4 a0 = *reinterpret_cast<const float4*>(a + i);
         const float4 a1 = *reinterpret_cast<const float4*>(a + i + 4);

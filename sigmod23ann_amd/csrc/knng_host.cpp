@@ -499,7 +499,8 @@ struct Builder {
             if (l.size < 2) continue;
             int bi = 0;
             while (bi < 4 && l.size > kEdges[bi]) ++bi;
-            if (force_big || (bi < 4 && leaf_lds_bytes(kEdges[bi], d, ld) > 160 * 1024)) bi = 4;
+            // a bin's LDS is sized by its largest leaf, so the test is on the leaf's own size
+            if (force_big || (bi < 4 && leaf_lds_bytes(l.size, d, ld) > 160 * 1024)) bi = 4;
             if (bi < 4) {
                 bin[bi].push_back(l);
             } else {

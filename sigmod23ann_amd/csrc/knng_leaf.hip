@@ -249,9 +249,9 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
 
 // ---- buckets too large for LDS ------------------------------------------------------------------
 // One workgroup (16 waves) per 64-row block of a bucket. Each wave owns 4 of the rows and keeps
-// their sorted lists in registers for the whole kernel (a row's FP32 values are re-read from L2
-// for the column blocks where it has survivors). The bucket's rows stream
-// through LDS 64 at a time (16-bit screening rows, FP32 rows); for each such column block the
+// their sorted lists in registers for the whole kernel; the block's FP32 and 16-bit rows stay in
+// LDS. The bucket's rows stream through LDS 64 at a time (16-bit screening rows, FP32 rows), the
+// next column block's loads in flight while the current one is screened; for each column block the
 // workgroup screens its 64 x 64 pairs on MFMA (one 16 x 16 tile per wave) (same scaling, centre and margin as k_leaf), then
 // each wave re-ranks its rows' survivors exactly and offers them in one batch. Every row's list
 // belongs to one workgroup, so there are no races; a pair (i, j) is computed once from each side,
@@ -263,14 +263,16 @@ constexpr uint32_t kBigCols = 64;
 constexpr uint32_t kBigRowsPerWave = kBigRows / kBigWaves;  // 4
 
 struct BigLds {
-    uint32_t rs;
+    uint32_t rs, fs;  // 16-bit row stride (elements), FP32 row stride (floats)
     uint32_t off_arow, off_anrm, off_arinv, off_abnd, off_crow, off_cnrm, off_crinv, off_cid;
-    uint32_t off_cf32, off_mask, off_rowbuf, off_scr, total;
+    uint32_t off_af32, off_cf32, off_mask, off_scr, total;
 };
 
 __host__ __device__ inline BigLds big_lds_layout(uint32_t d, uint32_t ld) {
     BigLds L;
     L.rs = leaf_row_stride(d);
+    // fs / 4 odd: the lanes of a phase-2 re-rank read consecutive rows without bank conflicts
+    L.fs = ld % 8 == 0 ? ld + 4 : ld;
     uint32_t o = 0;
     L.off_arow = o;  o += kBigRows * L.rs * 2;
     o = (o + 15) & ~15u;
@@ -283,9 +285,9 @@ __host__ __device__ inline BigLds big_lds_layout(uint32_t d, uint32_t ld) {
     L.off_crinv = o; o += kBigCols * 4;
     L.off_cid = o;   o += kBigCols * 4;
     o = (o + 15) & ~15u;
-    L.off_cf32 = o;  o += kBigCols * ld * 4;
+    L.off_af32 = o;  o += kBigRows * L.fs * 4;
+    L.off_cf32 = o;  o += kBigCols * L.fs * 4;
     L.off_mask = o;  o += kBigRows * 8;
-    L.off_rowbuf = o; o += kBigWaves * ld * 4;
     L.off_scr = o;   o += kBigWaves * 128 * 8;
     L.total = (o + 15) & ~15u;
     return L;
@@ -310,9 +312,9 @@ __global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigB
     float* s_crinv = reinterpret_cast<float*>(lds + L.off_crinv);
     uint32_t* s_cid = reinterpret_cast<uint32_t*>(lds + L.off_cid);
     float* s_cf32 = reinterpret_cast<float*>(lds + L.off_cf32);
+    float* s_af32 = reinterpret_cast<float*>(lds + L.off_af32);
     uint16_t* s_mask = reinterpret_cast<uint16_t*>(lds + L.off_mask);  // 4 x 16 bits per row
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    float* s_rowbuf = reinterpret_cast<float*>(lds + L.off_rowbuf) + wave * ld;
     uint64_t* s_scr = reinterpret_cast<uint64_t*>(lds + L.off_scr) + wave * 128;
     const uint32_t* pm = a.perm + bb.start;
     const float* mu = a.X + (size_t)pm[0] * ld;  // centre: the bucket's first row, as in k_leaf
@@ -335,9 +337,12 @@ __global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigB
     for (uint32_t j = 0; j < kBigRowsPerWave; ++j) {
         const uint32_t rr = kBigRowsPerWave * wave + j;
         uint16_t* hrow = s_arow + rr * L.rs;
+        float* frow = s_af32 + rr * L.fs;
         if (rr < nrows) {
             const float* x = a.X + (size_t)pid[j] * ld;
-            const float x0 = lane < d ? x[lane] : 0.f, x1 = lane + 64 < d ? x[lane + 64] : 0.f;
+            const float x0 = lane < ld ? x[lane] : 0.f, x1 = lane + 64 < ld ? x[lane + 64] : 0.f;
+            if (lane < ld) frow[lane] = x0;
+            if (lane + 64 < ld) frow[lane + 64] = x1;
             float nrm, rinv;
             screen_vals<HALF>(lane < d ? x0 - mu0 : 0.f, lane + 64 < d ? x1 - mu1 : 0.f, d,
                               L.rs, hrow, lane, nrm, rinv);
@@ -348,6 +353,7 @@ __global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigB
             }
         } else {
             for (uint32_t t = lane; t < L.rs; t += 64) hrow[t] = 0;
+            for (uint32_t t = lane; t < L.fs; t += 64) frow[t] = 0.f;
             if (lane == 0) {
                 s_anrm[rr] = 0.f;
                 s_arinv[rr] = 0.f;
@@ -365,37 +371,56 @@ __global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigB
     for (uint32_t j = 0; j < kBigRowsPerWave; ++j) changed[j] = false;
     uint32_t n_surv = 0, n_ins = 0;
 
-    for (uint32_t c0 = 0; c0 < b; c0 += kBigCols) {
-        const uint32_t nc = min(kBigCols, b - c0);
-        __syncthreads();  // the previous block's phase 2 is done with the column buffers
-        for (uint32_t cc = wave; cc < kBigCols; cc += kBigWaves) {
+    // column staging: each wave fetches 4 rows of the next block into registers (fetch), then
+    // writes them to LDS with their 16-bit screening rows (commit)
+    constexpr uint32_t kColsPerWave = kBigCols / kBigWaves;  // 4
+    float cx0[kColsPerWave], cx1[kColsPerWave];
+    uint32_t cid[kColsPerWave];
+    auto fetch = [&](uint32_t cb) {
+#pragma unroll
+        for (uint32_t u = 0; u < kColsPerWave; ++u) {
+            const uint32_t cc = wave + kBigWaves * u;
+            const bool valid = cb + cc < b;
+            cid[u] = valid ? __builtin_amdgcn_readfirstlane(pm[cb + cc]) : kNoId;
+            const float* x = a.X + (size_t)(valid ? cid[u] : pm[0]) * ld;
+            cx0[u] = valid && lane < ld ? x[lane] : 0.f;
+            cx1[u] = valid && lane + 64 < ld ? x[lane + 64] : 0.f;
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (uint32_t u = 0; u < kColsPerWave; ++u) {
+            const uint32_t cc = wave + kBigWaves * u;
             uint16_t* hrow = s_crow + cc * L.rs;
-            float* frow = s_cf32 + cc * ld;
-            if (cc < nc) {
-                const uint32_t id = pm[c0 + cc];
-                const float* x = a.X + (size_t)id * ld;
-                const float x0 = lane < ld ? x[lane] : 0.f;
-                const float x1 = lane + 64 < ld ? x[lane + 64] : 0.f;
-                if (lane < ld) frow[lane] = x0;
-                if (lane + 64 < ld) frow[lane + 64] = x1;
+            float* frow = s_cf32 + cc * L.fs;
+            if (lane < ld) frow[lane] = cx0[u];
+            if (lane + 64 < ld) frow[lane + 64] = cx1[u];
+            if (cid[u] != kNoId) {
                 float nrm, rinv;
-                screen_vals<HALF>(lane < d ? x0 - mu0 : 0.f, lane + 64 < d ? x1 - mu1 : 0.f, d, L.rs,
-                                  hrow, lane, nrm, rinv);
+                screen_vals<HALF>(lane < d ? cx0[u] - mu0 : 0.f, lane + 64 < d ? cx1[u] - mu1 : 0.f, d,
+                                  L.rs, hrow, lane, nrm, rinv);
                 if (lane == 0) {
-                    s_cid[cc] = id;
                     s_cnrm[cc] = nrm;
                     s_crinv[cc] = rinv;
                 }
             } else {
                 for (uint32_t t = lane; t < L.rs; t += 64) hrow[t] = 0;
                 if (lane == 0) {
-                    s_cid[cc] = kNoId;
                     s_cnrm[cc] = 0.f;
                     s_crinv[cc] = 0.f;
                 }
             }
+            if (lane == 0) s_cid[cc] = cid[u];
         }
+    };
+    fetch(0);
+
+    for (uint32_t c0 = 0; c0 < b; c0 += kBigCols) {
+        const uint32_t nc = min(kBigCols, b - c0);
+        __syncthreads();  // the previous block's phase 2 is done with the column buffers
+        commit();
         __syncthreads();
+        if (c0 + kBigCols < b) fetch(c0 + kBigCols);
 
         // ---- phase 1: screen this wave's tile ------------------------------------------------
         {
@@ -451,20 +476,13 @@ __global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigB
             if (!w64) continue;
             // this row's state into fixed registers (constant-index selects keep the arrays in VGPRs)
             WaveList cur = lst[0];
-            uint32_t id = pid[0];
 #pragma unroll
             for (uint32_t jj = 1; jj < kBigRowsPerWave; ++jj)
-                if (j == jj) {
-                    cur = lst[jj];
-                    id = pid[jj];
-                }
-            const float* xr = a.X + (size_t)id * ld;
-            if (lane < ld) s_rowbuf[lane] = xr[lane];
-            if (lane + 64 < ld) s_rowbuf[lane + 64] = xr[lane + 64];
-            __builtin_amdgcn_wave_barrier();
+                if (j == jj) cur = lst[jj];
             uint64_t x = WaveList::kEmpty;
             if ((w64 >> lane) & 1ull)
-                x = WaveList::key(dist_ref(s_rowbuf, s_cf32 + (size_t)lane * ld, d), s_cid[lane]);
+                x = WaveList::key(dist_ref(s_af32 + (size_t)rr * L.fs, s_cf32 + (size_t)lane * L.fs, d),
+                                  s_cid[lane]);
             n_surv += __popcll(w64);
             n_ins += __popcll(__ballot((uint32_t)(x >> 32) < __float_as_uint(cur.bound)));
             const bool ch = cur.offer64(x, k, lane, s_scr);
