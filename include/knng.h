@@ -164,6 +164,45 @@ int knng_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int knng_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int knng_synchronize(void);
 
+/* ---- serving: continuous batching of build jobs ---------------------------------------------
+ * No counterpart in the reference (a one-shot CLI, src/main.cpp:39-71). A server owns `workers`
+ * builders, each with its own HIP stream and device buffers reused across jobs. Jobs are admitted
+ * in arrival order as soon as a worker is free and the sum of the running jobs' device footprints
+ * (knng_job_footprint) stays within hbm_budget bytes; admitted jobs run concurrently on the GPU.
+ * A build is deterministic, so a job's result does not depend on what else is running. */
+typedef struct knng_server knng_server;
+
+typedef struct {
+    const float* points;        /* n rows of row_stride floats (host, or device if points_on_device) */
+    uint32_t n, d;
+    size_t row_stride;
+    uint32_t points_on_device;  /* 1: device memory, read in place until the job completes */
+    knng_params params;
+    uint32_t* out_ids;          /* host, n * params.k, rows padded with id 1 */
+    float* out_dists;           /* host, n * params.k, nullable */
+} knng_job;
+
+typedef struct {
+    int status;                 /* KNNG_OK or the job's error code */
+    double queue_ms;            /* submit -> start */
+    double run_ms;              /* start -> results in the caller's buffers */
+    double latency_ms;          /* submit -> results in the caller's buffers */
+    uint64_t footprint;         /* bytes the admission accounted for */
+    uint32_t worker;
+} knng_job_result;
+
+/* Device bytes a job needs: points, top-k state, NoN scratch, outputs. */
+uint64_t knng_job_footprint(uint32_t n, uint32_t d, const knng_params* params);
+int knng_server_create(uint32_t workers, uint64_t hbm_budget, knng_server** out);
+/* Queues a job; buffers must stay valid until knng_server_wait returns for it. A job whose
+ * footprint exceeds the budget is rejected with KNNG_ERR_NOMEM. */
+int knng_server_submit(knng_server* s, const knng_job* job, uint64_t* job_id);
+int knng_server_poll(knng_server* s, uint64_t job_id, int* done);
+/* Blocks until the job is done, fills `out` (nullable) and forgets the job. */
+int knng_server_wait(knng_server* s, uint64_t job_id, knng_job_result* out);
+/* Runs every queued job to completion, then stops the workers. */
+void knng_server_destroy(knng_server* s);
+
 #ifdef __cplusplus
 }
 #endif

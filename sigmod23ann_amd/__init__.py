@@ -46,7 +46,8 @@ EXPORTS = [
     "knng_partition_injected", "knng_leaf_stage", "knng_non", "knng_exact_knn_device",
     "knng_exact_upper_from_lists", "knng_recall", "knng_generate_device", "knng_device_alloc",
     "knng_device_free", "knng_memcpy_h2d", "knng_memcpy_d2h", "knng_memcpy_d2d",
-    "knng_synchronize",
+    "knng_synchronize", "knng_job_footprint", "knng_server_create", "knng_server_submit",
+    "knng_server_poll", "knng_server_wait", "knng_server_destroy",
 ]
 
 
@@ -95,6 +96,33 @@ class Stats(C.Structure):
         ("non_screen_ms", C.c_double),
         ("non_screen_launches", C.c_uint64),
         ("non_screen_bytes", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Job(C.Structure):
+    _fields_ = [
+        ("points", C.c_void_p),
+        ("n", C.c_uint32),
+        ("d", C.c_uint32),
+        ("row_stride", C.c_size_t),
+        ("points_on_device", C.c_uint32),
+        ("params", Params),
+        ("out_ids", C.c_void_p),
+        ("out_dists", C.c_void_p),
+    ]
+
+
+class JobResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int),
+        ("queue_ms", C.c_double),
+        ("run_ms", C.c_double),
+        ("latency_ms", C.c_double),
+        ("footprint", C.c_uint64),
+        ("worker", C.c_uint32),
     ]
 
     def as_dict(self):
@@ -151,6 +179,14 @@ def lib():
         L.knng_memcpy_h2d.argtypes = [_vp, _vp, _sz]
         L.knng_memcpy_d2h.argtypes = [_vp, _vp, _sz]
         L.knng_memcpy_d2d.argtypes = [_vp, _vp, _sz]
+        L.knng_job_footprint.argtypes = [_u32, _u32, _P(Params)]
+        L.knng_job_footprint.restype = C.c_uint64
+        L.knng_server_create.argtypes = [_u32, C.c_uint64, _P(_vp)]
+        L.knng_server_submit.argtypes = [_vp, _P(Job), _P(C.c_uint64)]
+        L.knng_server_poll.argtypes = [_vp, C.c_uint64, _P(C.c_int)]
+        L.knng_server_wait.argtypes = [_vp, C.c_uint64, _P(JobResult)]
+        L.knng_server_destroy.argtypes = [_vp]
+        L.knng_server_destroy.restype = None
         _lib = L
     return _lib
 
@@ -350,3 +386,63 @@ def construct_result(points, k=100, **kw):
         return b.build(points)
     finally:
         b.close()
+
+
+def job_footprint(n, d, prm: Params) -> int:
+    """Device bytes the server's admission accounts for a job."""
+    return int(lib().knng_job_footprint(n, d, C.byref(prm)))
+
+
+class Ticket:
+    """A submitted job: its id and the buffers that must outlive it."""
+
+    def __init__(self, jid, ids, dists, keep):
+        self.id, self.ids, self.dists, self._keep = jid, ids, dists, keep
+
+
+class Server:
+    """Continuous batching of build jobs (knng_server_*): `workers` builders on their own HIP
+    streams, jobs admitted in arrival order while their device footprints fit `hbm_budget`."""
+
+    def __init__(self, workers=4, hbm_budget=250 << 30):
+        h = _vp()
+        _check(lib().knng_server_create(workers, hbm_budget, C.byref(h)))
+        self.h = h.value
+
+    def submit(self, prm: Params, X=None, device_ptr=None, n=None, d=None, row_stride=None,
+               return_dists=False) -> Ticket:
+        """Host points X (n, d), or device rows at device_ptr (n rows of row_stride floats)."""
+        if X is not None:
+            X, n, d = _points(X)
+            ptr, on_dev, stride, keep = X.ctypes.data, 0, d, X
+        else:
+            ptr, on_dev, stride, keep = device_ptr, 1, row_stride or d, None
+        ids = np.empty((n, prm.k), np.uint32)
+        ds = np.empty((n, prm.k), np.float32) if return_dists else None
+        job = Job(ptr, n, d, stride, on_dev, prm, ids.ctypes.data,
+                  ds.ctypes.data if ds is not None else None)
+        jid = C.c_uint64()
+        _check(lib().knng_server_submit(self.h, C.byref(job), C.byref(jid)))
+        return Ticket(jid.value, ids, ds, keep)
+
+    def done(self, t: Ticket) -> bool:
+        flag = C.c_int()
+        _check(lib().knng_server_poll(self.h, t.id, C.byref(flag)))
+        return bool(flag.value)
+
+    def wait(self, t: Ticket):
+        """(ids, dists or None, result dict); raises if the job failed."""
+        r = JobResult()
+        _check(lib().knng_server_wait(self.h, t.id, C.byref(r)))
+        return t.ids, t.dists, r.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().knng_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

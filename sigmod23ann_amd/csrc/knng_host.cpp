@@ -17,8 +17,14 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/knng.h"
@@ -1174,3 +1180,167 @@ int knng_synchronize(void) {
 }
 
 }  // extern "C"
+
+// ---- serving -------------------------------------------------------------------------------------
+namespace knng {
+
+struct ServerJob {
+    knng_job job{};
+    uint64_t footprint = 0;
+    std::chrono::steady_clock::time_point t_submit, t_start, t_done;
+    bool done = false;
+    int status = KNNG_OK;
+    uint32_t worker = 0;
+};
+
+}  // namespace knng
+
+struct knng_server {
+    uint64_t budget = 0, in_use = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<knng::ServerJob*> queue;
+    std::unordered_map<uint64_t, std::unique_ptr<knng::ServerJob>> jobs;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::vector<std::thread> workers;
+};
+
+namespace {
+
+// One worker: a builder (stream, device buffers) reused across jobs.
+void server_worker(knng_server* s, uint32_t w) {
+    knng_params p;
+    knng_default_params(&p);
+    Builder b;
+    const int init_rc = b.init(p);
+    knng::DevBuf<uint32_t> oid;
+    knng::DevBuf<float> od;
+    for (;;) {
+        knng::ServerJob* j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] {
+                return (s->stop && s->queue.empty()) ||
+                       (!s->queue.empty() && s->in_use + s->queue.front()->footprint <= s->budget);
+            });
+            if (s->queue.empty()) return;  // stopping
+            j = s->queue.front();
+            s->queue.pop_front();
+            s->in_use += j->footprint;
+            j->t_start = std::chrono::steady_clock::now();
+            j->worker = w;
+        }
+        int rc = init_rc;
+        if (rc == KNNG_OK) {
+            const knng_job& jb = j->job;
+            const size_t tot = (size_t)jb.n * jb.params.k;
+            b.prm = jb.params;
+            rc = b.bind_points(jb.points, jb.points_on_device != 0, jb.n, jb.d, jb.row_stride);
+            if (rc == KNNG_OK) rc = oid.ensure(std::max<size_t>(tot, 1));
+            if (rc == KNNG_OK && jb.out_dists) rc = od.ensure(std::max<size_t>(tot, 1));
+            if (rc == KNNG_OK) rc = b.build(oid.p, jb.out_dists ? od.p : nullptr);
+            if (rc == KNNG_OK) rc = b.download(jb.out_ids, oid.p, tot * 4);
+            if (rc == KNNG_OK && jb.out_dists) rc = b.download(jb.out_dists, od.p, tot * 4);
+        }
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->in_use -= j->footprint;
+            j->status = rc;
+            j->done = true;
+            j->t_done = std::chrono::steady_clock::now();
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+}  // namespace
+
+uint64_t knng_job_footprint(uint32_t n, uint32_t d, const knng_params* params) {
+    if (!params) return 0;
+    const uint64_t ld = (d + 3) & ~3u, k = params->k;
+    // points copy, top-k ids/dists, outputs, snapshot + reverse lists, NoN candidate buffers,
+    // per-point scalars (bounds, permutations, flags, offsets, counters, norms)
+    const uint64_t per_point = 4 * ld + 8 * k + 8 * k + 8 * k + 8 * Builder::kNonCap + 64;
+    return (uint64_t)n * per_point + (64ull << 20);
+}
+
+int knng_server_create(uint32_t workers, uint64_t hbm_budget, knng_server** out) {
+    if (!out || workers == 0 || workers > 64) return fail(KNNG_ERR_INVALID, "workers must be in [1, 64]");
+    auto* s = new knng_server();
+    s->budget = hbm_budget;
+    for (uint32_t w = 0; w < workers; ++w) s->workers.emplace_back(server_worker, s, w);
+    *out = s;
+    return KNNG_OK;
+}
+
+int knng_server_submit(knng_server* s, const knng_job* job, uint64_t* job_id) {
+    if (!s || !job || !job_id) return fail(KNNG_ERR_INVALID, "null argument");
+    std::string why;
+    if (!knng::valid_params(job->params, why)) return fail(KNNG_ERR_INVALID, why);
+    if (job->n && (!job->points || !job->out_ids)) return fail(KNNG_ERR_INVALID, "null points or output");
+    if (job->d < 1 || job->d > knng::kMaxDim || job->row_stride < job->d)
+        return fail(KNNG_ERR_INVALID, "bad d or row_stride");
+    auto j = std::make_unique<knng::ServerJob>();
+    j->job = *job;
+    j->footprint = knng_job_footprint(job->n, job->d, &job->params);
+    if (j->footprint > s->budget) return fail(KNNG_ERR_NOMEM, "job footprint exceeds the server's HBM budget");
+    j->t_submit = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return fail(KNNG_ERR_INVALID, "server is stopping");
+        *job_id = s->next_id++;
+        s->queue.push_back(j.get());
+        s->jobs.emplace(*job_id, std::move(j));
+    }
+    s->cv_work.notify_all();
+    return KNNG_OK;
+}
+
+int knng_server_poll(knng_server* s, uint64_t job_id, int* done) {
+    if (!s || !done) return fail(KNNG_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(job_id);
+    if (it == s->jobs.end()) return fail(KNNG_ERR_INVALID, "unknown job id");
+    *done = it->second->done ? 1 : 0;
+    return KNNG_OK;
+}
+
+int knng_server_wait(knng_server* s, uint64_t job_id, knng_job_result* out) {
+    if (!s) return fail(KNNG_ERR_INVALID, "null server");
+    std::unique_ptr<knng::ServerJob> j;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(job_id);
+        if (it == s->jobs.end()) return fail(KNNG_ERR_INVALID, "unknown job id");
+        knng::ServerJob* jp = it->second.get();
+        s->cv_done.wait(lk, [&] { return jp->done; });
+        j = std::move(it->second);
+        s->jobs.erase(it);
+    }
+    if (out) {
+        out->status = j->status;
+        out->queue_ms = ms_between(j->t_submit, j->t_start);
+        out->run_ms = ms_between(j->t_start, j->t_done);
+        out->latency_ms = ms_between(j->t_submit, j->t_done);
+        out->footprint = j->footprint;
+        out->worker = j->worker;
+    }
+    return j->status == KNNG_OK ? KNNG_OK : fail(j->status, "job failed");
+}
+
+void knng_server_destroy(knng_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->workers) t.join();
+    delete s;
+}

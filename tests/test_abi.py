@@ -48,3 +48,28 @@ def test_recall_grader_semantics():
     truth = np.array([[1, 2, 3, 4], [5, 6, 7, 8]], np.uint32)
     res = np.array([[4, 3, 9, 9], [8, 7, 6, 5]], np.uint32)   # 2/4 and 4/4
     assert K.recall(res, truth) == 0.75
+
+
+def test_server_admission_and_validation_without_gpu():
+    import numpy as np
+    if not os.path.exists(K.LIB_PATH):
+        pytest.skip("libknng.so not built")
+    prm = K.params(k=100)
+    f1 = K.job_footprint(100_000, 100, prm)
+    f2 = K.job_footprint(1_000_000, 100, prm)
+    f3 = K.job_footprint(1_000_000, 100, K.params(k=10))
+    assert f2 > f1 > 0 and f3 < f2
+    # per point: points (4 ld) + lists, outputs, snapshot/reverse lists (8k each) + NoN buffers
+    assert f2 - f1 == 900_000 * (4 * 100 + 24 * 100 + 8 * 192 + 64)
+    # a job larger than the budget is refused before anything is queued
+    srv = K.Server(workers=2, hbm_budget=f1)
+    X = np.zeros((1000, 100), np.float32)
+    with pytest.raises(K.KnngError) as e:
+        srv.submit(K.params(k=100), X=np.zeros((200_000, 100), np.float32))
+    assert e.value.code == K.ERR_NOMEM
+    with pytest.raises(K.KnngError) as e:
+        srv.submit(K.params(k=0), X=X)
+    assert e.value.code == K.ERR_INVALID
+    srv.close()
+    h = ctypes.c_void_p()
+    assert K.lib().knng_server_create(0, 1 << 30, ctypes.byref(h)) == K.ERR_INVALID
