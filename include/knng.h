@@ -50,6 +50,9 @@ typedef struct knng_params {
     uint32_t precision;    /* KNNG_PREC_* */
     uint32_t rerank;       /* reserved: per-point re-rank cap of the approximate screens */
     uint32_t flags;        /* KNNG_FLAG_* */
+    uint32_t search_k;     /* list width while building, <= 128 (0 = k); the output keeps each
+                              list's first k. Small k search poorly through neighbours of
+                              neighbours, whose candidate count is width^2. */
     uint64_t seed;
 } knng_params;
 

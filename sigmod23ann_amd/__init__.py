@@ -72,6 +72,7 @@ class Params(C.Structure):
         ("precision", C.c_uint32),
         ("rerank", C.c_uint32),
         ("flags", C.c_uint32),
+        ("search_k", C.c_uint32),
         ("seed", C.c_uint64),
     ]
 
@@ -197,9 +198,9 @@ def _check(code):
 
 
 def params(k=100, leaf_max=400, leaf_max_odd=0, small_max=3000, iterations=1, rounds=8,
-           non_passes=1, precision=PREC_FP32, rerank=160, seed=12345, flags=0) -> Params:
+           non_passes=1, precision=PREC_FP32, rerank=160, seed=12345, flags=0, search_k=0) -> Params:
     return Params(k, leaf_max, leaf_max_odd, small_max, iterations, rounds, non_passes, precision,
-                  rerank, flags, seed)
+                  rerank, flags, search_k, seed)
 
 
 def _f(a):

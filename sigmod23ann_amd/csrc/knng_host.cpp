@@ -120,6 +120,7 @@ struct HostNode {
 
 bool valid_params(const knng_params& p, std::string& why) {
     if (p.k < 1 || p.k > kMaxK) { why = "k must be in [1, 128]"; return false; }
+    if (p.search_k > kMaxK) { why = "search_k must be <= 128"; return false; }
     if (p.leaf_max < 1 || p.leaf_max > kMaxLeafAny) { why = "leaf_max must be in [1, 65536]"; return false; }
     if (p.leaf_max_odd > kMaxLeafAny) { why = "leaf_max_odd must be <= 65536"; return false; }
     if (p.small_max > kMaxSmallNode) { why = "small_max must be <= 4096"; return false; }
@@ -275,6 +276,9 @@ struct Builder {
         KNNG_TRY(drain(1));
         return KNNG_OK;
     }
+
+    // list width while building; the output keeps each list's first prm.k
+    uint32_t kl() const { return std::max(prm.k, prm.search_k); }
 
     uint32_t leaf_max_for(uint32_t round) const {
         return (round % 2 == 1 && prm.leaf_max_odd) ? prm.leaf_max_odd : prm.leaf_max;
@@ -482,8 +486,8 @@ struct Builder {
     }
 
     int ensure_state() {
-        KNNG_TRY(knn_ids.ensure((size_t)n * prm.k));
-        KNNG_TRY(knn_d.ensure((size_t)n * prm.k));
+        KNNG_TRY(knn_ids.ensure((size_t)n * kl()));
+        KNNG_TRY(knn_d.ensure((size_t)n * kl()));
         KNNG_TRY(bound.ensure(n));
         KNNG_TRY(counters.ensure(4));
         return KNNG_OK;
@@ -535,7 +539,7 @@ struct Builder {
         a.X = X;
         a.ld = ld;
         a.d = d;
-        a.k = prm.k;
+        a.k = kl();
         a.perm = perm.p;
         a.knn_ids = knn_ids.p;
         a.knn_d = knn_d.p;
@@ -579,7 +583,7 @@ struct Builder {
     bool perm_is_tree = false;
 
     int non_pass() {
-        const uint32_t k = prm.k;
+        const uint32_t k = kl();
         const size_t nk = (size_t)n * k;
         KNNG_TRY(snap.ensure(nk));
         launch_non_snapshot(knn_ids.p, snap.p, n, k, st);
@@ -732,7 +736,7 @@ struct Builder {
         KNNG_TRY(ensure_state());
         KNNG_HIP(hipMemsetAsync(counters.p, 0, 4 * sizeof(uint64_t), st));
         KNNG_HIP(hipEventRecord(ev[0], st));
-        launch_init_knn(knn_ids.p, knn_d.p, bound.p, n, prm.k, st);
+        launch_init_knn(knn_ids.p, knn_d.p, bound.p, n, kl(), st);
         std::vector<LeafDesc> lv;
         for (uint32_t r = 0; r < prm.rounds; ++r) {
             KNNG_HIP(hipEventRecord(ev[6], st));
@@ -766,7 +770,7 @@ struct Builder {
                          (unsigned long long)c[0], (unsigned long long)c[1],
                          (unsigned long long)stats.non_overflow);
         }
-        launch_finalize(knn_ids.p, knn_d.p, n, prm.k, out_ids, out_d, st);
+        launch_finalize(knn_ids.p, knn_d.p, n, kl(), prm.k, out_ids, out_d, st);
         KNNG_HIP(hipGetLastError());
         KNNG_HIP(hipEventRecord(ev[1], st));
         KNNG_HIP(hipEventSynchronize(ev[1]));
@@ -794,26 +798,26 @@ int check_handle(const knng_handle* h) {
     return h ? KNNG_OK : fail(KNNG_ERR_INVALID, "null handle");
 }
 
-// Host state (n*k) <-> device state.
+// Host state (n * list width) <-> device state.
 int upload_state(Builder& b, const uint32_t* ids, const float* dists, bool empty) {
-    const size_t tot = (size_t)b.n * b.prm.k;
+    const size_t tot = (size_t)b.n * b.kl();
     KNNG_TRY(b.ensure_state());
     if (empty) {
-        knng::launch_init_knn(b.knn_ids.p, b.knn_d.p, b.bound.p, b.n, b.prm.k, b.st);
+        knng::launch_init_knn(b.knn_ids.p, b.knn_d.p, b.bound.p, b.n, b.kl(), b.st);
         return KNNG_OK;
     }
     KNNG_HIP(hipMemcpyAsync(b.knn_ids.p, ids, tot * 4, hipMemcpyHostToDevice, b.st));
     KNNG_HIP(hipMemcpyAsync(b.knn_d.p, dists, tot * 4, hipMemcpyHostToDevice, b.st));
     // bound = distance in slot k-1
     std::vector<float> bnd(b.n);
-    for (uint32_t p = 0; p < b.n; ++p) bnd[p] = dists[(size_t)p * b.prm.k + b.prm.k - 1];
+    for (uint32_t p = 0; p < b.n; ++p) bnd[p] = dists[(size_t)p * b.kl() + b.kl() - 1];
     KNNG_HIP(hipMemcpyAsync(b.bound.p, bnd.data(), b.n * 4, hipMemcpyHostToDevice, b.st));
     KNNG_HIP(hipStreamSynchronize(b.st));
     return KNNG_OK;
 }
 
 int download_state(Builder& b, uint32_t* ids, float* dists) {
-    const size_t tot = (size_t)b.n * b.prm.k;
+    const size_t tot = (size_t)b.n * b.kl();
     KNNG_HIP(hipMemcpyAsync(ids, b.knn_ids.p, tot * 4, hipMemcpyDeviceToHost, b.st));
     KNNG_HIP(hipMemcpyAsync(dists, b.knn_d.p, tot * 4, hipMemcpyDeviceToHost, b.st));
     KNNG_HIP(hipStreamSynchronize(b.st));
@@ -1263,7 +1267,7 @@ double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_c
 
 uint64_t knng_job_footprint(uint32_t n, uint32_t d, const knng_params* params) {
     if (!params) return 0;
-    const uint64_t ld = (d + 3) & ~3u, k = params->k;
+    const uint64_t ld = (d + 3) & ~3u, k = std::max(params->k, params->search_k);
     // points copy, top-k ids/dists, outputs, snapshot + reverse lists, NoN candidate buffers,
     // per-point scalars (bounds, permutations, flags, offsets, counters, norms)
     const uint64_t per_point = 4 * ld + 8 * k + 8 * k + 8 * k + 8 * Builder::kNonCap + 64;

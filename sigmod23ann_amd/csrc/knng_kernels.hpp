@@ -95,8 +95,8 @@ void launch_leaf_big(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks
 
 void launch_init_knn(uint32_t* ids, float* dists, float* bound, uint32_t n, uint32_t k,
                      hipStream_t st);
-// ids -> out (padding id 1 for empty slots), dists -> out_d (nullable).
-void launch_finalize(const uint32_t* ids, const float* dists, uint32_t n, uint32_t k,
+// First k of each width-kl list -> out (padding id 1 for empty slots), dists -> out_d (nullable).
+void launch_finalize(const uint32_t* ids, const float* dists, uint32_t n, uint32_t kl, uint32_t k,
                      uint32_t* out_ids, float* out_d, hipStream_t st);
 
 // ---- neighbours of neighbours ------------------------------------------------------------------

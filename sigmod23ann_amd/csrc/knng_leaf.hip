@@ -590,19 +590,21 @@ void launch_init_knn(uint32_t* ids, float* dists, float* bound, uint32_t n, uint
 
 // finalize + padResult (KnnSets.hpp:113-126, :334-344): lists are already sorted by (dist, id);
 // empty slots become id 1.
-__global__ void k_finalize(const uint32_t* ids, const float* dists, uint32_t n, uint32_t k,
-                           uint32_t* out_ids, float* out_d) {
+// Lists of width kl -> outputs of width k <= kl (each list's first k: the k nearest found).
+__global__ void k_finalize(const uint32_t* ids, const float* dists, uint32_t n, uint32_t kl,
+                           uint32_t k, uint32_t* out_ids, float* out_d) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)n * k) return;
-    const uint32_t id = ids[i];
+    const size_t src = (i / k) * kl + i % k;
+    const uint32_t id = ids[src];
     out_ids[i] = id == kNoId ? 1u : id;
-    if (out_d) out_d[i] = dists[i];
+    if (out_d) out_d[i] = dists[src];
 }
 
-void launch_finalize(const uint32_t* ids, const float* dists, uint32_t n, uint32_t k,
+void launch_finalize(const uint32_t* ids, const float* dists, uint32_t n, uint32_t kl, uint32_t k,
                      uint32_t* out_ids, float* out_d, hipStream_t st) {
     const size_t tot = (size_t)n * k;
-    if (tot) k_finalize<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(ids, dists, n, k, out_ids, out_d);
+    if (tot) k_finalize<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(ids, dists, n, kl, k, out_ids, out_d);
 }
 
 }  // namespace knng

@@ -306,3 +306,13 @@ def test_generators_on_device():
     assert np.abs(Y).max() < 20 and np.abs(Y).max() > 8
     b.generate_device(K.GEN_LOWRANK, n, d, d, 5, dX.ptr)
     np.testing.assert_array_equal(dX.download((n, d), np.float32), X)     # seeded
+
+
+def test_search_width_outputs_first_k_of_wider_lists():
+    # k = 10 searched with 100-wide lists is the first 10 columns of the k = 100 build
+    X = lowrank(20000, 64, 31)
+    kw = dict(leaf_max=256, small_max=3000, iterations=1, rounds=3, non_passes=1, seed=29)
+    ids10, ds10 = K.Builder(K.params(k=10, search_k=100, **kw)).build(X, return_dists=True)
+    ids100, ds100 = K.Builder(K.params(k=100, **kw)).build(X, return_dists=True)
+    np.testing.assert_array_equal(ids10, ids100[:, :10])
+    np.testing.assert_array_equal(ds10, ds100[:, :10])
