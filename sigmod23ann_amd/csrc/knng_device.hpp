@@ -206,7 +206,8 @@ __device__ inline uint64_t sort64_desc(uint64_t x, uint32_t lane) {
 struct WaveList {
     static constexpr uint64_t kEmpty = ((uint64_t)0x7F7FFFFFu << 32) | 0xFFFFFFFFu;  // (FLT_MAX, kNoId)
     uint64_t k0, k1;
-    float bound;  // k-th distance, wave-uniform
+    uint64_t kth;  // key in slot k-1, wave-uniform
+    float bound;   // its distance
 
     __device__ static inline uint64_t key(float d, uint32_t id) {
         return ((uint64_t)__float_as_uint(d) << 32) | id;
@@ -214,7 +215,10 @@ struct WaveList {
     __device__ inline void set_bound(uint32_t k) {
         const uint32_t kl = k - 1;
         const uint64_t kk = kl >= 64 ? k1 : k0;
-        bound = __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(kk >> 32), (int)(kl & 63)));
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(kk >> 32), (int)(kl & 63));
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)kk, (int)(kl & 63));
+        kth = ((uint64_t)hi << 32) | lo;
+        bound = __uint_as_float(hi);
     }
     __device__ inline void load(const uint32_t* gid, const float* gd, uint32_t k, uint32_t lane) {
         k0 = lane < k ? key(gd[lane], gid[lane]) : kEmpty;
@@ -225,12 +229,14 @@ struct WaveList {
         if (lane < k) { gid[lane] = (uint32_t)k0; gd[lane] = __uint_as_float((uint32_t)(k0 >> 32)); }
         if (lane + 64 < k) { gid[lane + 64] = (uint32_t)k1; gd[lane + 64] = __uint_as_float((uint32_t)(k1 >> 32)); }
     }
-    // Offer (nd, nid) to the list; wave-uniform arguments. The reference's filter is strict
-    // `dist < bound` (KnnSets.hpp:152-157) and an id already present is rejected (:49, :59-72);
-    // a present id carries the same distance, so its key equals the offered one.
+    // Offer (nd, nid) to the list; wave-uniform arguments. The list keeps the k smallest keys
+    // offered, with an id already present rejected (KnnSets.hpp:49, :59-72; a present id carries
+    // the same distance, so its key equals the offered one). That set does not depend on the
+    // order of offers; the reference's strict `dist < bound` (:152-157) differs only for a
+    // distance equal to the k-th, where it keeps the first offered and this keeps the smaller id.
     __device__ inline bool insert(float nd, uint32_t nid, uint32_t k, uint32_t lane) {
-        if (!__builtin_amdgcn_readfirstlane(nd < bound)) return false;
         const uint64_t nk = key(nd, nid);
+        if (!(nk < kth)) return false;
         if (__ballot(k0 == nk || k1 == nk)) return false;
         const uint32_t pos = __popcll(__ballot(k0 < nk)) + __popcll(__ballot(k1 < nk));
         // slot e takes slot e - 1's key for e > pos; slot 64 (lane 0's second) takes slot 63
@@ -247,15 +253,13 @@ struct WaveList {
     }
 
     // Offer up to 64 keys at once, one per lane (kEmpty where a lane has none); `scratch` is 128
-    // keys of this wave's LDS. Same set semantics as inserting them one by one: only keys below
-    // the bound and not already present enter, and the list keeps the k smallest. With ties at the
-    // k-th distance the batch keeps the smaller ids, where one-by-one insertion keeps whichever
-    // arrived first. Up to `one_by_one` offers go through insert(); more go through a bitonic sort of the
+    // keys of this wave's LDS. Same set as inserting them one by one: the k smallest distinct
+    // keys. Up to `one_by_one` offers go through insert(); more go through a bitonic sort of the
     // batch, a half-cleaner against the list and a bitonic merge of the 128 smallest, then
     // adjacent equal keys (an id offered again) are dropped. Returns whether the list changed.
     __device__ inline bool offer64(uint64_t x, uint32_t k, uint32_t lane, uint64_t* scratch,
                                    uint32_t one_by_one = 4) {
-        if (!((uint32_t)(x >> 32) < __float_as_uint(bound))) x = kEmpty;
+        if (!(x < kth)) x = kEmpty;
         uint64_t m = __ballot(x != kEmpty);
         if (!m) return false;
         if ((uint32_t)__popcll(m) <= one_by_one) {

@@ -330,6 +330,7 @@ __global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigB
         const bool valid = rr < nrows;
         pid[j] = valid ? __builtin_amdgcn_readfirstlane(pm[r0 + rr]) : kNoId;
         lst[j].k0 = lst[j].k1 = WaveList::kEmpty;
+        lst[j].kth = 0;
         lst[j].bound = 0.f;
         if (valid) lst[j].load(a.knn_ids + (size_t)pid[j] * k, a.knn_d + (size_t)pid[j] * k, k, lane);
     }

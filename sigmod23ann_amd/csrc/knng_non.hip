@@ -471,7 +471,8 @@ void k_non_reverse(NonRevArgs a) {
                 } else {
                     ex = dist_ref_lane8(s_brow + cc * ldp, s_arow + i * ldp, d, lane);
                 }
-                const bool keep = e < qn && (lane & 7) == 0 && ex < s_abnd[i];
+                // <=: a distance equal to the bound can still win on its id in the merge
+                const bool keep = e < qn && (lane & 7) == 0 && ex <= s_abnd[i];
                 const uint64_t bal = __ballot(keep);
                 if (bal) {
                     uint32_t base = 0;
