@@ -67,6 +67,10 @@ struct DevBuf {
     ~DevBuf() {
         if (p) (void)hipFree(p);
     }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+    }
     int ensure(size_t count) {
         if (count <= cap && p) return KNNG_OK;
         if (p) {
@@ -171,6 +175,8 @@ struct Builder {
     static constexpr uint32_t kNonSubpasses = 16;
     DevBuf<uint32_t> ovf_flag;
     DevBuf<uint32_t> rcnt, rfill, rsl, ncnt, nbuf_id;
+    DevBuf<uint32_t> snap_prev, rnew, rfill_old;
+    bool have_prev = false;  // snap_prev holds the previous pass's snapshot of the current lists
     DevBuf<uint64_t> roff, scan_tmp;
     DevBuf<float> nbuf_d, norms32;
 
@@ -585,8 +591,11 @@ struct Builder {
     int non_pass() {
         const uint32_t k = kl();
         const size_t nk = (size_t)n * k;
+        if (n > kIdMask) return fail(KNNG_ERR_UNSUPPORTED, "neighbours of neighbours needs n < 2^31");
         KNNG_TRY(snap.ensure(nk));
-        launch_non_snapshot(knn_ids.p, snap.p, n, k, st);
+        // later passes of a build screen only offers with a new hop
+        const bool incremental = have_prev && !(prm.flags & KNNG_FLAG_FULL_NON);
+        launch_non_snapshot(knn_ids.p, snap.p, n, k, incremental ? snap_prev.p : nullptr, st);
         NonArgs da{};
         da.X = X;
         da.ld = ld;
@@ -602,10 +611,13 @@ struct Builder {
         if (!reverse) {
             launch_non_direct(da, nullptr, n, st);
             KNNG_HIP(hipGetLastError());
+            have_prev = false;
             return KNNG_OK;
         }
         KNNG_TRY(rcnt.ensure(n));
         KNNG_TRY(rfill.ensure(n));
+        KNNG_TRY(rnew.ensure(n));
+        KNNG_TRY(rfill_old.ensure(n));
         KNNG_TRY(roff.ensure((size_t)n + 1));
         KNNG_TRY(scan_tmp.ensure(scan_scratch_elems(n)));
         KNNG_TRY(rsl.ensure(nk));
@@ -616,7 +628,8 @@ struct Builder {
         KNNG_TRY(ovf_flag.ensure(n));
         KNNG_TRY(perm.ensure(n));
         if (!perm_is_tree) launch_iota(perm.p, n, st);
-        launch_reverse_lists(snap.p, n, k, rcnt.p, roff.p, scan_tmp.p, rfill.p, rsl.p, st);
+        launch_reverse_lists(snap.p, n, k, rcnt.p, rnew.p, roff.p, scan_tmp.p, rfill.p, rfill_old.p,
+                             rsl.p, st);
         if (verbose) {
             std::vector<uint32_t> rc(n);
             KNNG_HIP(hipMemcpyAsync(rc.data(), rcnt.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
@@ -640,6 +653,7 @@ struct Builder {
         a.snap = snap.p;
         a.roff = roff.p;
         a.rs = rsl.p;
+        a.rnew = rnew.p;
         a.knn_ids = knn_ids.p;
         a.knn_d = knn_d.p;
         a.bound = bound.p;
@@ -691,10 +705,17 @@ struct Builder {
         if (verbose) {
             KNNG_HIP(hipEventSynchronize(ev[3]));
             KNNG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
-            std::fprintf(stderr, "[knng] non pass: %u sub-passes, reverse %.1f ms merge %.1f ms, "
-                                 "%u overflowed points direct %.1f ms\n",
-                         subs, t_rev, t_merge, novf, ms);
+            std::vector<uint32_t> rn(n);
+            KNNG_HIP(hipMemcpy(rn.data(), rnew.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+            uint64_t nnew = 0;
+            for (uint32_t v : rn) nnew += v;
+            std::fprintf(stderr, "[knng] non pass (%s): %u sub-passes, reverse %.1f ms merge %.1f ms, "
+                                 "%u overflowed points direct %.1f ms, new entries %.3f\n",
+                         incremental ? "incremental" : "full", subs, t_rev, t_merge, novf, ms,
+                         entries ? (double)nnew / (double)entries : 0.0);
         }
+        snap.swap(snap_prev);
+        have_prev = true;
         return KNNG_OK;
     }
 
@@ -761,6 +782,7 @@ struct Builder {
         stats.inserts = leafc[2];
         KNNG_HIP(hipMemsetAsync(counters.p, 0, 4 * sizeof(uint64_t), st));
         KNNG_HIP(hipEventRecord(ev[6], st));
+        have_prev = false;
         for (uint32_t p = 0; p < prm.non_passes; ++p) KNNG_TRY(non_pass());
         KNNG_HIP(hipEventRecord(ev[7], st));
         if (verbose && prm.non_passes) {
@@ -1004,6 +1026,7 @@ int knng_non(knng_handle* h, const float* points, uint32_t n, uint32_t d, size_t
     KNNG_TRY(upload_state(b, ids, dists, false));
     KNNG_HIP(hipMemsetAsync(b.counters.p, 0, 4 * sizeof(uint64_t), b.st));
     b.perm_is_tree = false;
+    b.have_prev = false;  // one full pass per call
     KNNG_TRY(b.non_pass());
     return download_state(b, ids, dists);
 }
@@ -1268,9 +1291,9 @@ double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_c
 uint64_t knng_job_footprint(uint32_t n, uint32_t d, const knng_params* params) {
     if (!params) return 0;
     const uint64_t ld = (d + 3) & ~3u, k = std::max(params->k, params->search_k);
-    // points copy, top-k ids/dists, outputs, snapshot + reverse lists, NoN candidate buffers,
+    // points copy, top-k ids/dists, outputs, two snapshots + reverse lists, NoN candidate buffers,
     // per-point scalars (bounds, permutations, flags, offsets, counters, norms)
-    const uint64_t per_point = 4 * ld + 8 * k + 8 * k + 8 * k + 8 * Builder::kNonCap + 64;
+    const uint64_t per_point = 4 * ld + 8 * k + 8 * k + 12 * k + 8 * Builder::kNonCap + 72;
     return (uint64_t)n * per_point + (64ull << 20);
 }
 

@@ -113,7 +113,8 @@ struct NonRevArgs {
     uint32_t ld, d, k, n;
     const uint32_t* snap;   // n*k snapshot ids, each row sorted ascending
     const uint64_t* roff;   // n+1 offsets of the reverse lists
-    const uint32_t* rs;     // reverse lists: RS[q] = { p : q in snap[p] }
+    const uint32_t* rs;     // reverse lists: RS[q] = { p : q in snap[p] }, new entries first
+    const uint32_t* rnew;   // n: new entries at the head of RS[q] (nullable: all new)
     const uint32_t* order;  // q processing order (tree order of the last round), this launch's part
     uint32_t order_stride;  // this launch takes order[0], order[stride], order[2 stride], ...
     uint32_t nq;            // q in this launch
@@ -131,17 +132,18 @@ struct NonRevArgs {
 void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float* out,
                       hipStream_t st);
 
-// Sorted snapshot (unfilled slots -> id 0).
+// Sorted snapshot (unfilled slots -> id 0); an entry absent from `prev` (nullable: all) carries
+// kNewBit.
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
-                         hipStream_t st);
+                         const uint32_t* prev, hipStream_t st);
 // plist == nullptr: points 0..np-1.
 void launch_non_direct(const NonArgs& a, const uint32_t* plist, uint32_t np, hipStream_t st);
 size_t scan_scratch_elems(uint32_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint64_t* out, uint64_t* scratch,
                            hipStream_t st);
 void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t* rcnt,
-                          uint64_t* roff, uint64_t* scratch, uint32_t* rfill, uint32_t* rs,
-                          hipStream_t st);
+                          uint32_t* rnew, uint64_t* roff, uint64_t* scratch, uint32_t* rfill,
+                          uint32_t* rfill_old, uint32_t* rs, hipStream_t st);
 size_t non_reverse_lds(uint32_t k, uint32_t ld);
 void launch_non_reverse(const NonRevArgs& a, hipStream_t st);
 void launch_non_merge(const NonRevArgs& a, uint32_t* ovf_cnt, uint32_t* ovf_list, uint32_t* ovf_flag,

@@ -15,6 +15,12 @@
 //
 // Direct form: one wave per point walks its 2-hop candidates itself (used for overflow and as the
 // small-input path).
+//
+// Incremental passes: a snapshot entry carries bit 31 when its id was not in the previous pass's
+// snapshot of the same point (every entry of a first pass is new). An offer p <- r through q whose
+// two hops (q in S[p], r in S[q]) are both old was already made in the previous pass and can only
+// be rejected again (bounds only shrink, lists keep the k smallest keys), so a later pass screens
+// new rows of RS[q] against all of S[q] and old rows against S[q]'s new entries only.
 #include "knng_device.hpp"
 #include "knng_kernels.hpp"
 
@@ -23,7 +29,8 @@ namespace knng {
 // ---- snapshot sorted by id -------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __restrict__ ids,
                                                              uint32_t* __restrict__ snap,
-                                                             uint32_t n, uint32_t k) {
+                                                             uint32_t n, uint32_t k,
+                                                             const uint32_t* __restrict__ prev) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (p >= n) return;
@@ -46,31 +53,50 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
             r1 += (b < v1 || (b == v1 && j + 64 < e1));
         }
     }
+    // new: not in the previous snapshot of this point (all new without one)
+    uint32_t f0 = kNewBit, f1 = kNewBit;
+    if (prev) {
+        const uint32_t* pp = prev + (size_t)p * k;
+        const uint32_t q0 = lane < k ? pp[lane] & kIdMask : 0xFFFFFFFFu;
+        const uint32_t q1 = lane + 64 < k ? pp[lane + 64] & kIdMask : 0xFFFFFFFFu;
+        bool h0 = false, h1 = false;
+        for (uint32_t j = 0; j < 64; ++j) {
+            const uint32_t a = __shfl(q0, j), b = __shfl(q1, j);
+            h0 |= (a == v0) | (b == v0);
+            h1 |= (a == v1) | (b == v1);
+        }
+        f0 = h0 ? 0u : kNewBit;
+        f1 = h1 ? 0u : kNewBit;
+    }
     uint32_t* dst = snap + (size_t)p * k;
-    if (lane < k) dst[r0] = v0;
-    if (lane + 64 < k) dst[r1] = v1;
+    if (lane < k) dst[r0] = v0 | f0;
+    if (lane + 64 < k) dst[r1] = v1 | f1;
 }
 
 // ---- reverse lists: RS[q] = { p : q in S[p] } ----------------------------------------------------
+// RS[q] holds the p with a new entry q first (rnew[q] of them), then the rest.
 __global__ void k_rs_count(const uint32_t* __restrict__ snap, uint32_t n, uint32_t k,
-                           uint32_t* __restrict__ rcnt) {
+                           uint32_t* __restrict__ rcnt, uint32_t* __restrict__ rnew) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)n * k) return;
     const uint32_t slot = (uint32_t)(i % k);
-    const uint32_t q = snap[i];
-    if (slot > 0 && snap[i - 1] == q) return;  // repeated id 0 of an unfilled list
+    const uint32_t v = snap[i], q = v & kIdMask;
+    if (slot > 0 && (snap[i - 1] & kIdMask) == q) return;  // repeated id 0 of an unfilled list
     atomicAdd(&rcnt[q], 1u);
+    if (v & kNewBit) atomicAdd(&rnew[q], 1u);
 }
 
 __global__ void k_rs_fill(const uint32_t* __restrict__ snap, uint32_t n, uint32_t k,
-                          const uint64_t* __restrict__ roff, uint32_t* __restrict__ rfill,
+                          const uint64_t* __restrict__ roff, const uint32_t* __restrict__ rcnt,
+                          uint32_t* __restrict__ rfill, uint32_t* __restrict__ rfill_old,
                           uint32_t* __restrict__ rs) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)n * k) return;
     const uint32_t slot = (uint32_t)(i % k);
-    const uint32_t q = snap[i];
-    if (slot > 0 && snap[i - 1] == q) return;
-    const uint32_t pos = atomicAdd(&rfill[q], 1u);
+    const uint32_t v = snap[i], q = v & kIdMask;
+    if (slot > 0 && (snap[i - 1] & kIdMask) == q) return;
+    const uint32_t pos = (v & kNewBit) ? atomicAdd(&rfill[q], 1u)
+                                       : rcnt[q] - 1u - atomicAdd(&rfill_old[q], 1u);
     rs[roff[q] + pos] = (uint32_t)(i / k);
 }
 
@@ -323,6 +349,7 @@ void k_non_reverse(NonRevArgs a) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint16_t* s_queue = reinterpret_cast<uint16_t*>(lds + L.off_queue) + wave * kNrQueue;
     const uint32_t nrs = (uint32_t)(a1 - a0);
+    const uint32_t nn = a.rnew ? min(a.rnew[q], nrs) : nrs;  // RS(q)'s new rows come first
     // staging map: 16 threads per chunk row
     const uint32_t sr = tid >> 4, sc = tid & 15;
 
@@ -330,14 +357,39 @@ void k_non_reverse(NonRevArgs a) {
     for (uint32_t t = tid; t < kNrHash; t += kNrThreads) s_hkey[t] = kNoId;
     if (tid < kNrBuckets) s_hcnt[tid] = 0;
     if (tid < 2) s_cnt[tid] = 0;
-    for (uint32_t t = tid; t < L.kb16; t += kNrThreads) {
-        const uint32_t r = t < k ? a.snap[(size_t)q * k + t] : kNoId;
-        s_bid[t] = r;
-        s_bnrm[t] = t < k ? a.norms[r] : 0.f;
+    if (wave == 0) {  // S(q) with its new entries first (slots lane, lane + 64; k <= 128)
+        const uint32_t* sq = a.snap + (size_t)q * k;
+        const uint32_t v0 = lane < k ? sq[lane] : 0u, v1 = lane + 64 < k ? sq[lane + 64] : 0u;
+        const bool n0 = lane < k && (v0 & kNewBit), n1 = lane + 64 < k && (v1 & kNewBit);
+        const uint64_t m0 = __ballot(n0), m1 = __ballot(n1);
+        const uint64_t below = (1ull << lane) - 1ull;
+        const uint32_t nnew = __popcll(m0) + __popcll(m1);
+        const uint32_t b0 = __popcll(m0 & below), b1 = __popcll(m0) + __popcll(m1 & below);
+        if (lane < k) {
+            const uint32_t pos = n0 ? b0 : nnew + lane - b0, id = v0 & kIdMask;
+            s_bid[pos] = id;
+            s_bnrm[pos] = a.norms[id];
+        }
+        if (lane + 64 < k) {
+            const uint32_t pos = n1 ? b1 : nnew + lane + 64 - b1, id = v1 & kIdMask;
+            s_bid[pos] = id;
+            s_bnrm[pos] = a.norms[id];
+        }
+        if (lane == 0) s_cnt[2] = nnew;
+    }
+    for (uint32_t t = k + tid; t < L.kb16; t += kNrThreads) {
+        s_bid[t] = kNoId;
+        s_bnrm[t] = 0.f;
     }
     uint32_t g0 = 0;  // first reverse entry of the staged ids group
     for (uint32_t t = tid; t < min(nrs, kNrIds); t += kNrThreads) s_rsid[t] = a.rs[a0 + t];
     __syncthreads();
+    // old rows pair with S(q)'s new entries only; with none of either there is nothing to offer
+    const uint32_t ntile_new = (s_cnt[2] + 15) / 16;
+    const uint32_t ne = ntile_new ? nrs : nn;
+    if (ne == 0) return;
+    // chunks never straddle the new/old boundary
+    auto cend = [&](uint32_t c) { return c < nn ? min(c + kNrChunk, nn) : min(c + kNrChunk, ne); };
     // Id 0 may be the padding of an unfilled list rather than a member, so it is never hashed:
     // offers of id 0 always go through (the list's id check settles them).
     if (tid < k) {
@@ -411,7 +463,7 @@ void k_non_reverse(NonRevArgs a) {
         if (sr < pf_na) {
 #pragma unroll
             for (uint32_t u = 0; u < kNrPfSnap; ++u) {
-                const uint32_t e = pf_snap[u];
+                const uint32_t e = pf_snap[u] & kIdMask;
                 if (e == 0) continue;
                 const uint32_t b = nr_bucket(e);
                 const uint4 k0 = reinterpret_cast<const uint4*>(s_hkey)[2 * b];
@@ -432,7 +484,7 @@ void k_non_reverse(NonRevArgs a) {
             }
         }
     };
-    fetch(0, min(nrs, kNrChunk));
+    fetch(0, cend(0));
     for (uint32_t t = tid; t < kNrChunk * 4; t += kNrThreads) s_mask[t] = 0;
     __syncthreads();
     commit();
@@ -448,12 +500,14 @@ void k_non_reverse(NonRevArgs a) {
     const uint32_t lrow = lane & 15, lk = lane >> 4;
     uint32_t n_scr = 0;
 
-    for (uint32_t c0 = 0; c0 < nrs; c0 += kNrChunk) {
-        const uint32_t na = min(kNrChunk, nrs - c0);
+    for (uint32_t c0 = 0; c0 < ne;) {
+        const uint32_t c1 = cend(c0);
+        const uint32_t na = c1 - c0;
         // the next chunk's loads fly while this chunk is screened
-        const uint32_t n0 = c0 + kNrChunk;
-        const bool next_in_group = n0 < nrs && n0 + kNrChunk <= g0 + kNrIds;
-        if (next_in_group) fetch(n0 - g0, min(kNrChunk, nrs - n0));
+        const uint32_t n0 = c1, n1 = n0 < ne ? cend(n0) : n0;
+        const bool next_in_group = n0 < ne && n1 <= g0 + kNrIds;
+        if (next_in_group) fetch(n0 - g0, n1 - n0);
+        const uint32_t t_top = c0 >= nn ? min(t_hi, ntile_new) : t_hi;
 
         uint32_t qn = 0;
         auto flush = [&]() {
@@ -509,7 +563,7 @@ void k_non_reverse(NonRevArgs a) {
                 thr0[i] = s_abnd[rr[i]] * (1.0f + 1.0f / 65536.0f);
             }
             const float* arow = s_arow + (16 * strip + lrow) * ldp + lk;
-            for (uint32_t t = t_lo; t < t_hi; ++t) {
+            for (uint32_t t = t_lo; t < t_top; ++t) {
                 if (qn + 256 > kNrQueue) flush();
                 const float* brow = s_brow + (16 * t + lrow) * ldp + lk;
                 const float4_t acc = gram_tile_f32<KS>(arow, brow, ksteps);
@@ -536,19 +590,20 @@ void k_non_reverse(NonRevArgs a) {
         }
         if (qn) flush();
         __syncthreads();
-        if (n0 < nrs) {
+        if (n0 < ne) {
             for (uint32_t t = tid; t < kNrChunk * 4; t += kNrThreads) s_mask[t] = 0;
             if (!next_in_group) {  // next ids group, then a direct fetch
                 g0 = n0;
                 for (uint32_t t = tid; t < min(nrs - g0, kNrIds); t += kNrThreads)
                     s_rsid[t] = a.rs[a0 + g0 + t];
                 __syncthreads();
-                fetch(0, min(kNrChunk, nrs - n0));
+                fetch(0, n1 - n0);
             }
             __syncthreads();
             commit();
             __syncthreads();
         }
+        c0 = c1;
     }
     // the chunk loop ends with a barrier: the list is complete
     const uint32_t nsurv = s_cnt[0];
@@ -625,13 +680,13 @@ __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a, const 
     bool changed = false;
     const uint32_t* sp = a.snap + (size_t)p * k;
     for (uint32_t a2 = 0; a2 < k; ++a2) {
-        const uint32_t q = sp[a2];
-        if (a2 > 0 && sp[a2 - 1] == q) continue;  // repeated id 0 of an unfilled list
+        const uint32_t q = sp[a2] & kIdMask;
+        if (a2 > 0 && (sp[a2 - 1] & kIdMask) == q) continue;  // repeated id 0 of an unfilled list
         const uint32_t* sq = a.snap + (size_t)q * k;
         for (uint32_t c0 = 0; c0 < k; c0 += 64) {
             uint64_t x = WaveList::kEmpty;
             if (c0 + lane < k) {
-                const uint32_t cid = sq[c0 + lane];
+                const uint32_t cid = sq[c0 + lane] & kIdMask;
                 if (cid != p) x = WaveList::key(dist_ref(a.X + (size_t)cid * ld, xr, d), cid);
             }
             changed |= lst.offer64(x, k, lane, scr[wave]);
@@ -672,8 +727,8 @@ void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float
 
 // ---- launchers ----------------------------------------------------------------------------------------
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
-                         hipStream_t st) {
-    if (n) k_non_snapshot_sorted<<<(n + 3) / 4, 256, 0, st>>>(ids, snap, n, k);
+                         const uint32_t* prev, hipStream_t st) {
+    if (n) k_non_snapshot_sorted<<<(n + 3) / 4, 256, 0, st>>>(ids, snap, n, k, prev);
 }
 
 void launch_non_direct(const NonArgs& a, const uint32_t* plist, uint32_t np, hipStream_t st) {
@@ -696,16 +751,18 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint64_t* out, uint64
 }
 
 void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t* rcnt,
-                          uint64_t* roff, uint64_t* scratch, uint32_t* rfill, uint32_t* rs,
-                          hipStream_t st) {
+                          uint32_t* rnew, uint64_t* roff, uint64_t* scratch, uint32_t* rfill,
+                          uint32_t* rfill_old, uint32_t* rs, hipStream_t st) {
     const size_t tot = (size_t)n * k;
     if (!tot) return;
     const unsigned g = (unsigned)((tot + 255) / 256);
     (void)hipMemsetAsync(rcnt, 0, (size_t)n * 4, st);
+    (void)hipMemsetAsync(rnew, 0, (size_t)n * 4, st);
     (void)hipMemsetAsync(rfill, 0, (size_t)n * 4, st);
-    k_rs_count<<<g, 256, 0, st>>>(snap, n, k, rcnt);
+    (void)hipMemsetAsync(rfill_old, 0, (size_t)n * 4, st);
+    k_rs_count<<<g, 256, 0, st>>>(snap, n, k, rcnt, rnew);
     launch_exclusive_scan(rcnt, n, roff, scratch, st);
-    k_rs_fill<<<g, 256, 0, st>>>(snap, n, k, roff, rfill, rs);
+    k_rs_fill<<<g, 256, 0, st>>>(snap, n, k, roff, rcnt, rfill, rfill_old, rs);
 }
 
 void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {

@@ -316,3 +316,21 @@ def test_search_width_outputs_first_k_of_wider_lists():
     ids100, ds100 = K.Builder(K.params(k=100, **kw)).build(X, return_dists=True)
     np.testing.assert_array_equal(ids10, ids100[:, :10])
     np.testing.assert_array_equal(ds10, ds100[:, :10])
+
+
+def test_incremental_non_passes_equal_full_passes():
+    # later passes skip offers whose two hops both predate the previous pass: same lists exactly
+    X = lowrank(30000, 100, 37)
+    kw = dict(k=100, leaf_max=256, small_max=3000, iterations=1, rounds=2, non_passes=3, seed=23)
+    ids_i, ds_i = K.Builder(K.params(**kw)).build(X, return_dists=True)
+    ids_f, ds_f = K.Builder(K.params(flags=K.FLAG_FULL_NON, **kw)).build(X, return_dists=True)
+    np.testing.assert_array_equal(ds_i, ds_f)
+    np.testing.assert_array_equal(ids_i, ids_f)
+
+
+def test_build_two_non_passes_matches_oracle():
+    X = mixture(12000, 64, 50, 47)
+    kw = dict(k=40, leaf_max=128, small_max=3000, iterations=1, rounds=2, non_passes=2, seed=41)
+    ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
+    o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(**kw))
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
