@@ -19,6 +19,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libknng.so")
+CLI_PATH = os.path.join(_HERE, "knng_cli")      # src/main.cpp's flow over the C ABI
+GRADE_PATH = os.path.join(_HERE, "knng_grade")  # src/grade.cpp's semantics
 CSRC = os.path.join(_HERE, "csrc")
 
 OK = 0
@@ -448,3 +450,30 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---- contest file formats (src/io.h) ---------------------------------------------------------
+def read_points(path, dims=100) -> np.ndarray:
+    """uint32 N, then N rows of `dims` float32 (ReadBinArray, src/io.h:66-94)."""
+    with open(path, "rb") as f:
+        n = int(np.frombuffer(f.read(4), np.uint32)[0])
+        X = np.frombuffer(f.read(n * dims * 4), np.float32)
+    if X.size != n * dims:
+        raise ValueError(f"{path}: header says {n} points, file holds {X.size // dims}")
+    return X.reshape(n, dims)
+
+
+def write_points(path, X):
+    X = np.ascontiguousarray(X, np.float32)
+    with open(path, "wb") as f:
+        f.write(np.uint32(X.shape[0]).tobytes())
+        f.write(X.tobytes())
+
+
+def save_graph(path, ids):
+    """n rows of k uint32 ids, no header (SaveKNNG, src/io.h:21-33)."""
+    np.ascontiguousarray(ids, np.uint32).tofile(path)
+
+
+def read_graph(path, n, k=100) -> np.ndarray:
+    return np.fromfile(path, np.uint32, count=n * k).reshape(n, k)
