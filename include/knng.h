@@ -55,6 +55,9 @@ typedef struct knng_params {
     uint32_t search_k;     /* list width while building, <= 128 (0 = k); the output keeps each
                               list's first k. Small k search poorly through neighbours of
                               neighbours, whose candidate count is width^2. */
+    uint32_t non_rank_lo;  /* 0: topUpSingle (SolutionKmeans.hpp:480-555), every neighbour expanded;
+                              r > 0: topUpSingleOrdered (:393-477), only neighbours at distance
+                              ranks >= r (the reference uses 50); list width <= 100 */
     uint64_t seed;
 } knng_params;
 

@@ -36,7 +36,7 @@ class Params(C.Structure):
         ("iterations", C.c_uint32),
         ("rounds", C.c_uint32),
         ("non_passes", C.c_uint32),
-        ("pad_", C.c_uint32),
+        ("non_rank_lo", C.c_uint32),
         ("seed", C.c_uint64),
     ]
 
@@ -54,8 +54,10 @@ class Stats(C.Structure):
 
 
 def make_params(k=100, leaf_max=256, leaf_max_odd=0, small_max=3000, iterations=1, rounds=1,
-                non_passes=0, seed=1234) -> Params:
-    return Params(k, leaf_max, leaf_max_odd, small_max, iterations, rounds, non_passes, 0, seed)
+                non_passes=0, seed=1234, non_rank_lo=0, **_ignored) -> Params:
+    """Reference build parameters; GPU-only knobs (flags, precision, search width) are ignored."""
+    return Params(k, leaf_max, leaf_max_odd, small_max, iterations, rounds, non_passes, non_rank_lo,
+                  seed)
 
 
 _P = C.POINTER
@@ -113,7 +115,7 @@ def lib():
                                      _u32p, _u32p, _u32p, C.c_uint32, _u32p, _f32p, _u32p,
                                      _u32p, _f32p, _u32p, C.c_int]
         L.orc_non.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u32p,
-                              _f32p, _u32p, _u32p, _f32p, _u32p, C.c_int]
+                              _f32p, _u32p, _u32p, _f32p, _u32p, C.c_int, C.c_uint32]
         L.orc_build.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, _P(Params), _u32p,
                                 _f32p, _u32p, C.c_int, _P(Stats)]
         L.orc_exact_knn.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, C.c_uint32,
@@ -267,15 +269,16 @@ def leaf_stage(X, k, perm, leaf_start, leaf_size, state=None, threads: int = 0):
     return ids, ds, sz
 
 
-def non(X, k, state, threads: int = 0):
-    """topUpSingle (SolutionKmeans.hpp:480-555) on a state (ids, dists, sizes)."""
+def non(X, k, state, threads: int = 0, rank_lo: int = 0):
+    """topUpSingle (SolutionKmeans.hpp:480-555) on a state (ids, dists, sizes); with rank_lo > 0
+    topUpSingleOrdered (:393-477), expanding each point's neighbours at ranks >= rank_lo."""
     X, n, d = _points(X)
     ii, di, si = _state_args(state, n, k)
     ids = np.empty((n, k), np.uint32)
     ds = np.empty((n, k), np.float32)
     sz = np.empty(n, np.uint32)
     lib().orc_non(_fp(X), n, d, d, k, _up(ii), _fp(di), _up(si), _up(ids), _fp(ds), _up(sz),
-                  threads)
+                  threads, rank_lo)
     return ids, ds, sz
 
 

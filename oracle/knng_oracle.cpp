@@ -222,6 +222,7 @@ struct KnnSetRef {
 struct Params {
     uint32_t k, leaf_max, leaf_max_odd, small_max, iterations, rounds, non_passes;
     uint64_t seed;
+    uint32_t non_rank_lo;
 };
 
 struct Points {
@@ -473,8 +474,16 @@ static void leaf_all_pairs(const Points& P, const uint32_t* perm, const Leaf& lf
 // topUpSingle, SolutionKmeans.hpp:480-555. The snapshot holds all k slots of each set; slots past
 // `size` are still zero in the reference (KnnSets.hpp:29), so they read as id 0. Candidates are
 // offered in ascending id order (the reference's robin_set order is unspecified).
+static void non_pass_ordered(const Points& P, std::vector<KnnSetRef>& knn,
+                             std::vector<float>& bounds, uint32_t k, uint32_t threads,
+                             uint32_t rank_lo);
+
 static void non_pass(const Points& P, std::vector<KnnSetRef>& knn, std::vector<float>& bounds,
-                     uint32_t k, uint32_t threads) {
+                     uint32_t k, uint32_t threads, uint32_t rank_lo = 0) {
+    if (rank_lo) {
+        non_pass_ordered(P, knn, bounds, k, threads, rank_lo);
+        return;
+    }
     const uint32_t n = P.n;
     std::vector<uint32_t> snap((size_t)n * k);
     parallel_for(n, threads, [&](uint32_t id) {
@@ -500,6 +509,43 @@ static void non_pass(const Points& P, std::vector<KnnSetRef>& knn, std::vector<f
             }
         }
     });
+}
+
+// topUpSingleOrdered, SolutionKmeans.hpp:393-477: each set's filled slots sorted by (dist, id)
+// (:398-419); point id1 expands only its neighbours at ranks [rank_lo, size) (the reference uses
+// 50, :437), each contributing the first min(size, 100) of its own sorted neighbours (:439-442);
+// id1 itself is removed (:447), current neighbours are not. Offers in ascending id order.
+static void non_pass_ordered(const Points& P, std::vector<KnnSetRef>& knn,
+                             std::vector<float>& bounds, uint32_t k, uint32_t threads,
+                             uint32_t rank_lo) {
+    const uint32_t n = P.n;
+    std::vector<std::vector<uint32_t>> sorted(n);
+    parallel_for(n, threads, [&](uint32_t id) {
+        std::vector<std::pair<float, uint32_t>> di;
+        for (uint32_t s = 0; s < knn[id].size; ++s) di.emplace_back(knn[id].dists[s], knn[id].ids[s]);
+        std::sort(di.begin(), di.end());
+        for (auto& e : di) sorted[id].push_back(e.second);
+    });
+    const uint32_t chunk = 256;
+    parallel_for((n + chunk - 1) / chunk, threads, [&](uint32_t c) {
+        std::vector<uint32_t> cand;
+        for (uint32_t id1 = c * chunk; id1 < std::min(n, (c + 1) * chunk); ++id1) {
+            cand.clear();
+            const auto& k1 = sorted[id1];
+            for (uint32_t i2 = rank_lo; i2 < k1.size(); ++i2) {
+                const auto& k2 = sorted[k1[i2]];
+                for (uint32_t i3 = 0; i3 < k2.size() && i3 < 100; ++i3) cand.push_back(k2[i3]);
+            }
+            std::sort(cand.begin(), cand.end());
+            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+            for (uint32_t id3 : cand) {
+                if (id3 == id1) continue;
+                float dist = dist_ref(P.row(id3), P.row(id1), P.d);
+                if (dist < bounds[id1]) knn[id1].add_less_than(bounds[id1], id3, dist);
+            }
+        }
+    });
+    (void)k;
 }
 
 static void load_state(std::vector<KnnSetRef>& knn, std::vector<float>& bounds, uint32_t n,
@@ -553,7 +599,7 @@ struct orc_params {
     uint32_t iterations;    // Lloyd iterations per split (reference knnIterations)
     uint32_t rounds;
     uint32_t non_passes;
-    uint32_t pad_;
+    uint32_t non_rank_lo;   // 0: topUpSingle; r > 0: topUpSingleOrdered expanding ranks >= r
     uint64_t seed;
 };
 
@@ -611,7 +657,7 @@ int orc_split_round(const float* X, uint32_t n, uint32_t d, uint32_t ld, const o
                     uint32_t* n_leaves, int threads) {
     Points P{X, n, d, ld};
     Params p{prm->k, prm->leaf_max, prm->leaf_max_odd, prm->small_max, prm->iterations,
-             prm->rounds, prm->non_passes, prm->seed};
+             prm->rounds, prm->non_passes, prm->seed, prm->non_rank_lo};
     std::vector<Leaf> leaves;
     SplitStats st;
     split_round(P, p, round, perm_out, leaves, pick_threads(threads), st);
@@ -669,13 +715,14 @@ int orc_leaf_stage(const float* X, uint32_t n, uint32_t d, uint32_t ld, uint32_t
 
 int orc_non(const float* X, uint32_t n, uint32_t d, uint32_t ld, uint32_t k,
             const uint32_t* ids_in, const float* dists_in, const uint32_t* sizes_in,
-            uint32_t* ids_out, float* dists_out, uint32_t* sizes_out, int threads) {
+            uint32_t* ids_out, float* dists_out, uint32_t* sizes_out, int threads,
+            uint32_t rank_lo) {
     Points P{X, n, d, ld};
     std::vector<KnnSetRef> knn;
     std::vector<float> bounds;
     load_state(knn, bounds, n, k, ids_in, dists_in, sizes_in);
     // The reference's snapshot reads slots in set order; here they come from the caller's order.
-    non_pass(P, knn, bounds, k, pick_threads(threads));
+    non_pass(P, knn, bounds, k, pick_threads(threads), rank_lo);
     store_state(knn, n, k, ids_out, dists_out, sizes_out);
     return 0;
 }
@@ -688,7 +735,7 @@ int orc_build(const float* X, uint32_t n, uint32_t d, uint32_t ld, const orc_par
               orc_stats* stats) {
     Points P{X, n, d, ld};
     Params p{prm->k, prm->leaf_max, prm->leaf_max_odd, prm->small_max, prm->iterations,
-             prm->rounds, prm->non_passes, prm->seed};
+             prm->rounds, prm->non_passes, prm->seed, prm->non_rank_lo};
     const uint32_t T = pick_threads(threads);
     std::vector<KnnSetRef> knn(n);
     for (auto& s : knn) s.init(p.k);
@@ -710,7 +757,7 @@ int orc_build(const float* X, uint32_t n, uint32_t d, uint32_t ld, const orc_par
         local.leaves += leaves.size();
     }
     auto t3 = clk::now();
-    for (uint32_t pass = 0; pass < p.non_passes; ++pass) non_pass(P, knn, bounds, p.k, T);
+    for (uint32_t pass = 0; pass < p.non_passes; ++pass) non_pass(P, knn, bounds, p.k, T, p.non_rank_lo);
     auto t4 = clk::now();
     std::vector<float> dtmp;
     float* dout = out_dists;

@@ -76,6 +76,7 @@ class Params(C.Structure):
         ("rerank", C.c_uint32),
         ("flags", C.c_uint32),
         ("search_k", C.c_uint32),
+        ("non_rank_lo", C.c_uint32),
         ("seed", C.c_uint64),
     ]
 
@@ -201,9 +202,10 @@ def _check(code):
 
 
 def params(k=100, leaf_max=400, leaf_max_odd=0, small_max=3000, iterations=1, rounds=8,
-           non_passes=1, precision=PREC_FP32, rerank=160, seed=12345, flags=0, search_k=0) -> Params:
+           non_passes=1, precision=PREC_FP32, rerank=160, seed=12345, flags=0, search_k=0,
+           non_rank_lo=0) -> Params:
     return Params(k, leaf_max, leaf_max_odd, small_max, iterations, rounds, non_passes, precision,
-                  rerank, flags, search_k, seed)
+                  rerank, flags, search_k, non_rank_lo, seed)
 
 
 def _f(a):

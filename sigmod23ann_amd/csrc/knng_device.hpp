@@ -13,7 +13,8 @@ namespace knng {
 
 constexpr uint32_t kNoId = 0xFFFFFFFFu;
 constexpr uint32_t kNewBit = 0x80000000u;  // NoN snapshot entry not in the previous pass's snapshot
-constexpr uint32_t kIdMask = 0x7FFFFFFFu;
+constexpr uint32_t kExpBit = 0x40000000u;  // NoN snapshot entry whose neighbours are expanded
+constexpr uint32_t kIdMask = 0x3FFFFFFFu;
 constexpr uint32_t kMaxAttempts = 8;     // split retries before the index-halving fallback
 constexpr uint32_t kLargeChunk = 256;    // sample draws per FP64 partial sum (large branch)
 constexpr uint32_t kMaxDim = 256;        // largest supported dimensionality

@@ -125,6 +125,10 @@ struct HostNode {
 bool valid_params(const knng_params& p, std::string& why) {
     if (p.k < 1 || p.k > kMaxK) { why = "k must be in [1, 128]"; return false; }
     if (p.search_k > kMaxK) { why = "search_k must be <= 128"; return false; }
+    if (p.non_rank_lo && (p.non_rank_lo >= std::max(p.k, p.search_k) || std::max(p.k, p.search_k) > 100)) {
+        why = "non_rank_lo must be below the list width, which must be <= 100 in the ordered form";
+        return false;
+    }
     if (p.leaf_max < 1 || p.leaf_max > kMaxLeafAny) { why = "leaf_max must be in [1, 65536]"; return false; }
     if (p.leaf_max_odd > kMaxLeafAny) { why = "leaf_max_odd must be <= 65536"; return false; }
     if (p.small_max > kMaxSmallNode) { why = "small_max must be <= 4096"; return false; }
@@ -591,11 +595,13 @@ struct Builder {
     int non_pass() {
         const uint32_t k = kl();
         const size_t nk = (size_t)n * k;
-        if (n > kIdMask) return fail(KNNG_ERR_UNSUPPORTED, "neighbours of neighbours needs n < 2^31");
+        if (n > kIdMask) return fail(KNNG_ERR_UNSUPPORTED, "neighbours of neighbours needs n < 2^30");
         KNNG_TRY(snap.ensure(nk));
-        // later passes of a build screen only offers with a new hop
-        const bool incremental = have_prev && !(prm.flags & KNNG_FLAG_FULL_NON);
-        launch_non_snapshot(knn_ids.p, snap.p, n, k, incremental ? snap_prev.p : nullptr, st);
+        // later passes of a build screen only offers with a new hop (not in the ordered form: an
+        // old entry can cross the rank threshold and be expanded for the first time)
+        const bool incremental = have_prev && !(prm.flags & KNNG_FLAG_FULL_NON) && prm.non_rank_lo == 0;
+        launch_non_snapshot(knn_ids.p, snap.p, n, k, incremental ? snap_prev.p : nullptr,
+                            prm.non_rank_lo, st);
         NonArgs da{};
         da.X = X;
         da.ld = ld;

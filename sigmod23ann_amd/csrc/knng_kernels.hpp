@@ -132,10 +132,11 @@ struct NonRevArgs {
 void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float* out,
                       hipStream_t st);
 
-// Sorted snapshot (unfilled slots -> id 0); an entry absent from `prev` (nullable: all) carries
-// kNewBit.
+// Sorted snapshot; an entry absent from `prev` (nullable: all) carries kNewBit, an expanded one
+// kExpBit. rank_lo = 0: unfilled slots -> id 0, every entry expanded; rank_lo > 0: unfilled
+// slots -> the point's own id, filled entries at ranks >= rank_lo expanded.
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
-                         const uint32_t* prev, hipStream_t st);
+                         const uint32_t* prev, uint32_t rank_lo, hipStream_t st);
 // plist == nullptr: points 0..np-1.
 void launch_non_direct(const NonArgs& a, const uint32_t* plist, uint32_t np, hipStream_t st);
 size_t scan_scratch_elems(uint32_t n);

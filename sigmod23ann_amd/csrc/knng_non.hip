@@ -4,6 +4,11 @@
 // read as id 0, as the reference's zero-initialised current_ids do (KnnSets.hpp:29). Point p is
 // offered every id in the snapshot lists of its snapshot neighbours except itself.
 //
+// Ordered form (topUpSingleOrdered, :393-477): p expands only its neighbours at distance ranks
+// >= rank_lo (the reference uses 50), each into its filled neighbours. The snapshot keeps all of a
+// list (it is also the r side and the membership set) and marks the expanded entries (kExpBit);
+// an unfilled slot reads as the point's own id, which every membership test rejects.
+//
 // Reverse form (the fast path). The offers (p <- r) with q in S[p] and r in S[q] are grouped by q:
 // one workgroup per q screens all pairs (p in RS[q], r in S[q]) with an FP32 MFMA Gram, drops
 // pairs with r already in S[p] (binary search in p's id-sorted snapshot) and pairs that
@@ -30,15 +35,20 @@ namespace knng {
 __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __restrict__ ids,
                                                              uint32_t* __restrict__ snap,
                                                              uint32_t n, uint32_t k,
-                                                             const uint32_t* __restrict__ prev) {
+                                                             const uint32_t* __restrict__ prev,
+                                                             uint32_t rank_lo) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (p >= n) return;
     const uint32_t* src = ids + (size_t)p * k;
     uint32_t v0 = lane < k ? src[lane] : 0xFFFFFFFFu;
     uint32_t v1 = lane + 64 < k ? src[lane + 64] : 0xFFFFFFFFu;
-    if (lane < k && v0 == kNoId) v0 = 0;
-    if (lane + 64 < k && v1 == kNoId) v1 = 0;
+    const bool un0 = lane < k && v0 == kNoId, un1 = lane + 64 < k && v1 == kNoId;
+    if (un0) v0 = rank_lo ? p : 0u;
+    if (un1) v1 = rank_lo ? p : 0u;
+    // slots are in (dist, id) order, so a slot is its rank
+    const uint32_t x0 = (!rank_lo || (!un0 && lane >= rank_lo)) ? kExpBit : 0u;
+    const uint32_t x1 = (!rank_lo || (!un1 && lane + 64 >= rank_lo)) ? kExpBit : 0u;
     // rank = number of (value, slot) pairs before mine
     uint32_t r0 = 0, r1 = 0;
     const uint32_t e0 = lane, e1 = lane + 64;
@@ -69,8 +79,8 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
         f1 = h1 ? 0u : kNewBit;
     }
     uint32_t* dst = snap + (size_t)p * k;
-    if (lane < k) dst[r0] = v0 | f0;
-    if (lane + 64 < k) dst[r1] = v1 | f1;
+    if (lane < k) dst[r0] = v0 | f0 | x0;
+    if (lane + 64 < k) dst[r1] = v1 | f1 | x1;
 }
 
 // ---- reverse lists: RS[q] = { p : q in S[p] } ----------------------------------------------------
@@ -81,6 +91,7 @@ __global__ void k_rs_count(const uint32_t* __restrict__ snap, uint32_t n, uint32
     if (i >= (size_t)n * k) return;
     const uint32_t slot = (uint32_t)(i % k);
     const uint32_t v = snap[i], q = v & kIdMask;
+    if (!(v & kExpBit)) return;
     if (slot > 0 && (snap[i - 1] & kIdMask) == q) return;  // repeated id 0 of an unfilled list
     atomicAdd(&rcnt[q], 1u);
     if (v & kNewBit) atomicAdd(&rnew[q], 1u);
@@ -94,6 +105,7 @@ __global__ void k_rs_fill(const uint32_t* __restrict__ snap, uint32_t n, uint32_
     if (i >= (size_t)n * k) return;
     const uint32_t slot = (uint32_t)(i % k);
     const uint32_t v = snap[i], q = v & kIdMask;
+    if (!(v & kExpBit)) return;
     if (slot > 0 && (snap[i - 1] & kIdMask) == q) return;
     const uint32_t pos = (v & kNewBit) ? atomicAdd(&rfill[q], 1u)
                                        : rcnt[q] - 1u - atomicAdd(&rfill_old[q], 1u);
@@ -681,6 +693,7 @@ __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a, const 
     const uint32_t* sp = a.snap + (size_t)p * k;
     for (uint32_t a2 = 0; a2 < k; ++a2) {
         const uint32_t q = sp[a2] & kIdMask;
+        if (!(sp[a2] & kExpBit)) continue;
         if (a2 > 0 && (sp[a2 - 1] & kIdMask) == q) continue;  // repeated id 0 of an unfilled list
         const uint32_t* sq = a.snap + (size_t)q * k;
         for (uint32_t c0 = 0; c0 < k; c0 += 64) {
@@ -727,8 +740,8 @@ void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float
 
 // ---- launchers ----------------------------------------------------------------------------------------
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
-                         const uint32_t* prev, hipStream_t st) {
-    if (n) k_non_snapshot_sorted<<<(n + 3) / 4, 256, 0, st>>>(ids, snap, n, k, prev);
+                         const uint32_t* prev, uint32_t rank_lo, hipStream_t st) {
+    if (n) k_non_snapshot_sorted<<<(n + 3) / 4, 256, 0, st>>>(ids, snap, n, k, prev, rank_lo);
 }
 
 void launch_non_direct(const NonArgs& a, const uint32_t* plist, uint32_t np, hipStream_t st) {

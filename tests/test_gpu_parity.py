@@ -201,6 +201,30 @@ def test_non_matches_oracle(flags):
     assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
 
 
+@pytest.mark.parametrize("flags,leaf", [(0, 100), (K.FLAG_DIRECT_NON, 100), (0, 15), (K.FLAG_DIRECT_NON, 15)])
+def test_non_ordered_matches_oracle(flags, leaf):
+    # topUpSingleOrdered: only neighbours at ranks >= 10 of 20 expanded. Leaf 100 fills every list;
+    # leaf 15 leaves them all short (filled entries only are expanded, padding never)
+    X = mixture(6000, 100, 30, 53)
+    k = 20
+    kw = dict(k=k, leaf_max=leaf, small_max=3000, iterations=1, seed=6)
+    o_prm = oracle.make_params(**kw)
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    st = oracle.leaf_stage(X, k, perm, ls, lz)
+    ids, ds = K.Builder(K.params(flags=flags, non_rank_lo=10, **kw)).non(X, (st[0], st[1]))
+    o_ids, o_ds, _ = oracle.non(X, k, st, rank_lo=10)
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+
+
+def test_build_ordered_non_matches_oracle():
+    X = lowrank(15000, 100, 57)
+    kw = dict(k=100, leaf_max=256, small_max=3000, iterations=1, rounds=2, non_passes=2,
+              non_rank_lo=50, seed=59)
+    ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
+    o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(**kw))
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+
+
 def test_non_k100_lowrank():
     # k = 100 lists on close unit-norm points: many 2-hop candidates per point, buffer overflow path
     X = lowrank(8000, 100, 33)

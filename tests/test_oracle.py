@@ -362,6 +362,35 @@ def test_non_brute_force():
     _assert_state_equals(*new, expect)
 
 
+@pytest.mark.parametrize("rank_lo", [5, 8])
+def test_non_ordered_brute_force(rank_lo):
+    # topUpSingleOrdered: only neighbours ranked >= rank_lo by (dist, id) are expanded, each into
+    # its filled neighbours (first 100); self removed; result = k smallest of old + candidates
+    X = _mixture(600, 16, 6, 19)
+    k = 10
+    p = oracle.make_params(k=k, leaf_max=30, small_max=200, seed=5)
+    perm, ls, lz = oracle.split_round(X, p, 0)
+    st = oracle.leaf_stage(X, k, perm, ls, lz)
+    ids, ds, sz = st
+    new = oracle.non(X, k, st, rank_lo=rank_lo)
+    order = [[int(ids[q, s]) for s in sorted(range(sz[q]), key=lambda s: (ds[q, s], ids[q, s]))]
+             for q in range(X.shape[0])]
+    expect = []
+    for q in range(X.shape[0]):
+        cur = {int(ids[q, s]): ds[q, s] for s in range(sz[q])}
+        cands = set()
+        for nb in order[q][rank_lo:]:
+            cands.update(order[nb][:100])
+        cands.discard(q)
+        for c in cands:
+            if c not in cur:
+                cur[c] = oracle.dist(X[c], X[q])
+        expect.append(sorted(cur.items(), key=lambda t: (t[1], t[0]))[:k])
+    _assert_state_equals(*new, expect)
+    # expanding from rank 0 is a different pass from topUpSingle only through unfilled slots
+    assert (sz == k).any()
+
+
 # ----------------------------------------------------------------------------------------------
 # Exact FP64 oracle and the whole build
 # ----------------------------------------------------------------------------------------------
