@@ -35,8 +35,9 @@ extern "C" {
 
 #define KNNG_FLAG_DIRECT_NON 1u /* neighbours-of-neighbours by the per-point direct kernel */
 #define KNNG_FLAG_BIG_LEAF 2u   /* every leaf through the big-bucket kernel (the path of leaves > 512) */
-#define KNNG_FLAG_FULL_NON 4u   /* every neighbours-of-neighbours pass screens all offers (later passes
-                                   otherwise skip offers whose two hops both predate the last pass) */
+#define KNNG_FLAG_INCREMENTAL_NON 4u /* later neighbours-of-neighbours passes of a build skip offers
+                                        whose two hops both predate the previous pass (same lists;
+                                        pays when few entries change between passes) */
 
 /* Build parameters. The reference hard-codes k = dims = 100 (src/Constants.hpp:23-24),
  * maxGroupSize 200/400 alternating (src/SolutionKmeans.hpp:582-587), knnIterations = 1 (:591),

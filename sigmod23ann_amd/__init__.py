@@ -36,7 +36,7 @@ PREC_BF16 = 2
 
 FLAG_DIRECT_NON = 1
 FLAG_BIG_LEAF = 2
-FLAG_FULL_NON = 4
+FLAG_INCREMENTAL_NON = 4
 
 GEN_LOWRANK = 0
 GEN_MIXTURE = 1

@@ -176,11 +176,13 @@ struct Builder {
 
     // neighbours-of-neighbours scratch
     static constexpr uint32_t kNonCap = 192;  // candidate slots per point before the direct path
-    static constexpr uint32_t kNonSubpasses = 16;
+    static constexpr uint32_t kNonSubpasses = 16;      // first pass of a build
+    static constexpr uint32_t kNonSubpassesLater = 4;  // later passes: lists are close, buffers small
     DevBuf<uint32_t> ovf_flag;
     DevBuf<uint32_t> rcnt, rfill, rsl, ncnt, nbuf_id;
     DevBuf<uint32_t> snap_prev, rnew, rfill_old;
     bool have_prev = false;  // snap_prev holds the previous pass's snapshot of the current lists
+    uint32_t pass_no = 0;    // neighbours-of-neighbours passes so far in this build
     DevBuf<uint64_t> roff, scan_tmp;
     DevBuf<float> nbuf_d, norms32;
 
@@ -599,7 +601,7 @@ struct Builder {
         KNNG_TRY(snap.ensure(nk));
         // later passes of a build screen only offers with a new hop (not in the ordered form: an
         // old entry can cross the rank threshold and be expanded for the first time)
-        const bool incremental = have_prev && !(prm.flags & KNNG_FLAG_FULL_NON) && prm.non_rank_lo == 0;
+        const bool incremental = have_prev && (prm.flags & KNNG_FLAG_INCREMENTAL_NON) && prm.non_rank_lo == 0;
         launch_non_snapshot(knn_ids.p, snap.p, n, k, incremental ? snap_prev.p : nullptr,
                             prm.non_rank_lo, st);
         NonArgs da{};
@@ -634,8 +636,8 @@ struct Builder {
         KNNG_TRY(ovf_flag.ensure(n));
         KNNG_TRY(perm.ensure(n));
         if (!perm_is_tree) launch_iota(perm.p, n, st);
-        launch_reverse_lists(snap.p, n, k, rcnt.p, rnew.p, roff.p, scan_tmp.p, rfill.p, rfill_old.p,
-                             rsl.p, st);
+        launch_reverse_lists(snap.p, n, k, rcnt.p, incremental ? rnew.p : nullptr, roff.p, scan_tmp.p,
+                             rfill.p, rfill_old.p, rsl.p, st);
         if (verbose) {
             std::vector<uint32_t> rc(n);
             KNNG_HIP(hipMemcpyAsync(rc.data(), rcnt.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
@@ -659,7 +661,7 @@ struct Builder {
         a.snap = snap.p;
         a.roff = roff.p;
         a.rs = rsl.p;
-        a.rnew = rnew.p;
+        a.rnew = incremental ? rnew.p : nullptr;
         a.knn_ids = knn_ids.p;
         a.knn_d = knn_d.p;
         a.bound = bound.p;
@@ -675,7 +677,9 @@ struct Builder {
         // Sub-passes over interleaved parts of the tree order (every subs-th q, so a point's q are
         // spread over all of them), each merged before the next: the bounds tighten as the pass
         // goes, which keeps the candidate buffers small.
-        const uint32_t subs = std::max(1u, std::min(kNonSubpasses, n / 1024 + 1));
+        // a sub-pass merges every point's list once, so fewer of them save list traffic; the first
+        // pass needs many to keep the candidate buffers within kNonCap
+        const uint32_t subs = std::max(1u, std::min(pass_no ? kNonSubpassesLater : kNonSubpasses, n / 1024 + 1));
         float t_rev = 0.f, t_merge = 0.f, ms = 0.f;
         for (uint32_t sp = 0; sp < subs; ++sp) {
             a.order = perm.p + sp;
@@ -711,17 +715,23 @@ struct Builder {
         if (verbose) {
             KNNG_HIP(hipEventSynchronize(ev[3]));
             KNNG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
-            std::vector<uint32_t> rn(n);
-            KNNG_HIP(hipMemcpy(rn.data(), rnew.p, (size_t)n * 4, hipMemcpyDeviceToHost));
-            uint64_t nnew = 0;
-            for (uint32_t v : rn) nnew += v;
+            uint64_t nnew = entries;
+            if (incremental) {
+                std::vector<uint32_t> rn(n);
+                KNNG_HIP(hipMemcpy(rn.data(), rnew.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+                nnew = 0;
+                for (uint32_t v : rn) nnew += v;
+            }
             std::fprintf(stderr, "[knng] non pass (%s): %u sub-passes, reverse %.1f ms merge %.1f ms, "
                                  "%u overflowed points direct %.1f ms, new entries %.3f\n",
                          incremental ? "incremental" : "full", subs, t_rev, t_merge, novf, ms,
                          entries ? (double)nnew / (double)entries : 0.0);
         }
-        snap.swap(snap_prev);
-        have_prev = true;
+        if (prm.flags & KNNG_FLAG_INCREMENTAL_NON) {  // keep this snapshot for the next pass
+            snap.swap(snap_prev);
+            have_prev = true;
+        }
+        ++pass_no;
         return KNNG_OK;
     }
 
@@ -789,6 +799,7 @@ struct Builder {
         KNNG_HIP(hipMemsetAsync(counters.p, 0, 4 * sizeof(uint64_t), st));
         KNNG_HIP(hipEventRecord(ev[6], st));
         have_prev = false;
+        pass_no = 0;
         for (uint32_t p = 0; p < prm.non_passes; ++p) KNNG_TRY(non_pass());
         KNNG_HIP(hipEventRecord(ev[7], st));
         if (verbose && prm.non_passes) {
@@ -1033,6 +1044,7 @@ int knng_non(knng_handle* h, const float* points, uint32_t n, uint32_t d, size_t
     KNNG_HIP(hipMemsetAsync(b.counters.p, 0, 4 * sizeof(uint64_t), b.st));
     b.perm_is_tree = false;
     b.have_prev = false;  // one full pass per call
+    b.pass_no = 0;
     KNNG_TRY(b.non_pass());
     return download_state(b, ids, dists);
 }

@@ -94,7 +94,7 @@ __global__ void k_rs_count(const uint32_t* __restrict__ snap, uint32_t n, uint32
     if (!(v & kExpBit)) return;
     if (slot > 0 && (snap[i - 1] & kIdMask) == q) return;  // repeated id 0 of an unfilled list
     atomicAdd(&rcnt[q], 1u);
-    if (v & kNewBit) atomicAdd(&rnew[q], 1u);
+    if (rnew && (v & kNewBit)) atomicAdd(&rnew[q], 1u);
 }
 
 __global__ void k_rs_fill(const uint32_t* __restrict__ snap, uint32_t n, uint32_t k,
@@ -107,8 +107,8 @@ __global__ void k_rs_fill(const uint32_t* __restrict__ snap, uint32_t n, uint32_
     const uint32_t v = snap[i], q = v & kIdMask;
     if (!(v & kExpBit)) return;
     if (slot > 0 && (snap[i - 1] & kIdMask) == q) return;
-    const uint32_t pos = (v & kNewBit) ? atomicAdd(&rfill[q], 1u)
-                                       : rcnt[q] - 1u - atomicAdd(&rfill_old[q], 1u);
+    const uint32_t pos = (!rfill_old || (v & kNewBit)) ? atomicAdd(&rfill[q], 1u)
+                                                       : rcnt[q] - 1u - atomicAdd(&rfill_old[q], 1u);
     rs[roff[q] + pos] = (uint32_t)(i / k);
 }
 
@@ -769,13 +769,16 @@ void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t
     const size_t tot = (size_t)n * k;
     if (!tot) return;
     const unsigned g = (unsigned)((tot + 255) / 256);
+    // rnew == nullptr: no new/old split (every entry filled from the front)
     (void)hipMemsetAsync(rcnt, 0, (size_t)n * 4, st);
-    (void)hipMemsetAsync(rnew, 0, (size_t)n * 4, st);
     (void)hipMemsetAsync(rfill, 0, (size_t)n * 4, st);
-    (void)hipMemsetAsync(rfill_old, 0, (size_t)n * 4, st);
+    if (rnew) {
+        (void)hipMemsetAsync(rnew, 0, (size_t)n * 4, st);
+        (void)hipMemsetAsync(rfill_old, 0, (size_t)n * 4, st);
+    }
     k_rs_count<<<g, 256, 0, st>>>(snap, n, k, rcnt, rnew);
     launch_exclusive_scan(rcnt, n, roff, scratch, st);
-    k_rs_fill<<<g, 256, 0, st>>>(snap, n, k, roff, rcnt, rfill, rfill_old, rs);
+    k_rs_fill<<<g, 256, 0, st>>>(snap, n, k, roff, rcnt, rfill, rnew ? rfill_old : nullptr, rs);
 }
 
 void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {

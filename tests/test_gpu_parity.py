@@ -346,8 +346,8 @@ def test_incremental_non_passes_equal_full_passes():
     # later passes skip offers whose two hops both predate the previous pass: same lists exactly
     X = lowrank(30000, 100, 37)
     kw = dict(k=100, leaf_max=256, small_max=3000, iterations=1, rounds=2, non_passes=3, seed=23)
-    ids_i, ds_i = K.Builder(K.params(**kw)).build(X, return_dists=True)
-    ids_f, ds_f = K.Builder(K.params(flags=K.FLAG_FULL_NON, **kw)).build(X, return_dists=True)
+    ids_i, ds_i = K.Builder(K.params(flags=K.FLAG_INCREMENTAL_NON, **kw)).build(X, return_dists=True)
+    ids_f, ds_f = K.Builder(K.params(**kw)).build(X, return_dists=True)
     np.testing.assert_array_equal(ds_i, ds_f)
     np.testing.assert_array_equal(ids_i, ids_f)
 
