@@ -296,14 +296,18 @@ struct Builder {
         return (round % 2 == 1 && prm.leaf_max_odd) ? prm.leaf_max_odd : prm.leaf_max;
     }
 
-    // Make the points usable by the kernels: 16-byte aligned rows with a stride of 4k floats.
+    // Make the points usable by the kernels: 16-byte aligned rows with a stride of 4k floats whose
+    // first ceil(d/4) float4 hold nothing beyond dim d-1 but zeros. A device buffer is used in place
+    // when it qualifies (d a multiple of 4: the caller's padding is never loaded); otherwise the
+    // rows are copied into zero-padded rows of (d + 3) & ~3 floats.
     int bind_points(const float* pts, bool on_device, uint32_t nn, uint32_t dd, size_t stride) {
         if (dd < 1 || dd > kMaxDim) return fail(KNNG_ERR_UNSUPPORTED, "d must be in [1, 256]");
         if (stride < dd) return fail(KNNG_ERR_INVALID, "row_stride < d");
         if (nn && !pts) return fail(KNNG_ERR_INVALID, "points is null");
         n = nn;
         d = dd;
-        const bool usable = on_device && stride % 4 == 0 && (reinterpret_cast<uintptr_t>(pts) & 15) == 0;
+        const bool usable = on_device && dd % 4 == 0 && stride % 4 == 0 &&
+                            (reinterpret_cast<uintptr_t>(pts) & 15) == 0;
         if (usable) {
             X = pts;
             ld = (uint32_t)stride;

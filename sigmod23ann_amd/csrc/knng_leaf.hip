@@ -157,7 +157,8 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
                 const float nsum = rn[i] + cn;
                 const float approx = nsum - 2.0f * g;
                 const float thr = exact ? thr_base[i] + margin_c * nsum : thr_base[i];
-                const bool ok = c < b && rr[i] < b && c != rr[i] && approx < thr;
+                // <=: a zero distance (both rows at the centre) may meet a zero bound and win on its id
+                const bool ok = c < b && rr[i] < b && c != rr[i] && approx <= thr;
                 bits[i] = __ballot(ok);
             }
             if (lane < 16) {
@@ -457,7 +458,7 @@ __global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigB
                     const float nsum = rn[i] + cn;
                     const float approx = nsum - 2.0f * g;
                     const float thr = exact ? thr_base[i] + margin_c * nsum : thr_base[i];
-                    const bool ok = c < nc && rr[i] < nrows && r0 + rr[i] != c0 + c && approx < thr;
+                    const bool ok = c < nc && rr[i] < nrows && r0 + rr[i] != c0 + c && approx <= thr;
                     bits[i] = __ballot(ok);
                 }
                 if (lane < 16) {

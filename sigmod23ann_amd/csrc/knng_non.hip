@@ -187,8 +187,10 @@ __global__ void k_scan_total(const uint32_t* __restrict__ in, uint32_t n, uint64
 }
 
 // ---- reverse screening kernel --------------------------------------------------------------------
-// One workgroup per q. FP32 rows of S[q] and 32-row chunks of RS[q] sit in LDS; an FP32 MFMA Gram
-// (16x16x4) screens every pair with a rigorous margin; pairs with r already in S[p] are masked by
+// One workgroup per q. FP32 rows of S[q] and 32-row chunks of RS[q] sit in LDS, each loaded as its
+// first ceil(d/4) float4 (the dims beyond, up to that bound, are zero in every buffer the builder
+// binds) and zero-filled to the LDS stride; an FP32 MFMA Gram (16x16x4) over ceil(d/4) steps
+// screens every pair with a rigorous margin; pairs with r already in S[p] are masked by
 // a per-chunk membership bitmap (S(q)'s ids sit in a bucketed hash table: one lookup is two
 // 16-byte LDS reads, with no probe loop for the lanes of a wave to diverge on); the survivors queue up per wave and are re-ranked exactly, eight
 // lanes per survivor (lane j owns the reference's accumulator j, then the lanes are summed in
@@ -341,7 +343,7 @@ void k_non_reverse(NonRevArgs a) {
     const uint64_t a0 = a.roff[q], a1 = a.roff[q + 1];
     if (a0 == a1) return;
     const NrLds L = nr_layout(k, ld);
-    const uint32_t ldp = L.ldp, ld4 = ld / 4, ldp4 = ldp / 4;
+    const uint32_t ldp = L.ldp, d4 = (d + 3) / 4, ldp4 = ldp / 4;
     uint32_t* s_bid = reinterpret_cast<uint32_t*>(lds + L.off_bid);
     float* s_bnrm = reinterpret_cast<float*>(lds + L.off_bnrm);
     float* s_brow = reinterpret_cast<float*>(lds + L.off_brow);
@@ -422,7 +424,7 @@ void k_non_reverse(NonRevArgs a) {
         for (uint32_t u = 0; u < kNrBRows; ++u) {
             const uint32_t t = tid + u * kNrThreads;
             const uint32_t r = t / ldp4, q4 = t - r * ldp4;
-            v[u] = (t < tot && r < k && q4 < ld4)
+            v[u] = (t < tot && r < k && q4 < d4)
                        ? reinterpret_cast<const float4*>(a.X + (size_t)s_bid[r] * ld)[q4]
                        : make_float4(0.f, 0.f, 0.f, 0.f);
         }
@@ -449,8 +451,8 @@ void k_non_reverse(NonRevArgs a) {
 #pragma unroll
         for (uint32_t u = 0; u < kNrPfRows; ++u) {
             const uint32_t q4 = sc + 16 * u;
-            pf_row[u] = (sr < na && q4 < ld4) ? reinterpret_cast<const float4*>(xr)[q4]
-                                              : make_float4(0.f, 0.f, 0.f, 0.f);
+            pf_row[u] = (sr < na && q4 < d4) ? reinterpret_cast<const float4*>(xr)[q4]
+                                             : make_float4(0.f, 0.f, 0.f, 0.f);
         }
         const uint32_t* sp = a.snap + (size_t)(sr < na ? pf_id : 0) * k;
 #pragma unroll
@@ -502,9 +504,9 @@ void k_non_reverse(NonRevArgs a) {
     commit();
     __syncthreads();
 
-    // |approx - exact| <= (K + 8) 2^-23 (n_p + n_r) for the FP32 Gram over K = ld terms
-    const float margin_c = (float)(ld + 8) * (1.0f / 8388608.0f);
-    const uint32_t ksteps = ld / 4;
+    // |approx - exact| <= (K + 8) 2^-23 (n_p + n_r) for the FP32 Gram over K = 4 ceil(d/4) terms
+    const float margin_c = (float)(4 * d4 + 8) * (1.0f / 8388608.0f);
+    const uint32_t ksteps = d4;
     const uint32_t ntile = L.kb16 / 16;
     const uint32_t strip = wave & 1, th = wave >> 1;
     const uint32_t t_per = (ntile + 3) / 4;
@@ -588,7 +590,8 @@ void k_non_reverse(NonRevArgs a) {
                 for (int i = 0; i < 4; ++i) {
                     const float nsum = rn[i] + cn;
                     const float approx = nsum - 2.0f * acc[i];
-                    bool ok = rr[i] < na && c < k && rid != pid[i] && approx < thr0[i] + margin_c * nsum;
+                    // <=: a distance equal to a zero bound can still win on its id
+                    bool ok = rr[i] < na && c < k && rid != pid[i] && approx <= thr0[i] + margin_c * nsum;
                     if (ok) ok = !((s_mask[rr[i] * 4 + (c >> 5)] >> (c & 31)) & 1u);
                     const uint64_t bal = __ballot(ok);
                     if (ok) {
@@ -790,7 +793,7 @@ void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         kernel<<<grid, kNrThreads, lds, st>>>(a);
     };
-    switch (a.ld / 4) {
+    switch ((a.d + 3) / 4) {  // the Gram's k-steps
         case 16: go(&k_non_reverse<16>); break;
         case 25: go(&k_non_reverse<25>); break;
         case 32: go(&k_non_reverse<32>); break;

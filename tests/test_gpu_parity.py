@@ -280,6 +280,28 @@ def test_build_reference_stride_and_padding():
         assert np.all(ids[r, o_sz[r]:] == 1)
 
 
+@pytest.mark.parametrize("d,stride", [(100, 112), (98, 100)])
+def test_build_device_rows_ignore_padding(d, stride):
+    # device rows used in place: dims d..stride-1 hold NaN and large values, which no kernel may
+    # read into a distance, a screen or a split (the host path zero-pads its copy); with d = 98 the
+    # padding shares the last 16-byte word of each row with dims 96 and 97
+    n, k = 6000, 50
+    X = mixture(n, d, 30, 8)
+    Xs = np.empty((n, stride), np.float32)
+    Xs[:, :d] = X
+    Xs[:, d:] = np.float32(3e4)
+    Xs[::3, d:] = np.nan
+    kw = dict(k=k, leaf_max=256, small_max=2000, iterations=1, rounds=3, non_passes=2, seed=4)
+    ids_h, ds_h = K.Builder(K.params(**kw)).build(X, return_dists=True)
+    dX = K.DeviceArray(Xs.nbytes)
+    dX.upload(Xs)
+    dI = K.DeviceArray(n * k * 4)
+    dD = K.DeviceArray(n * k * 4)
+    K.Builder(K.params(**kw)).build_device(dX.ptr, n, d, stride, dI.ptr, dD.ptr)
+    np.testing.assert_array_equal(dI.download((n, k), np.uint32), ids_h)
+    np.testing.assert_array_equal(dD.download((n, k), np.float32), ds_h)
+
+
 # ---------------------------------------------------------------------------------------------
 # Precision forms, exact FP64 tool, generators
 # ---------------------------------------------------------------------------------------------
