@@ -257,21 +257,35 @@ size_t non_reverse_lds(uint32_t k, uint32_t ld) { return nr_layout(k, ld).total;
 
 __device__ inline uint32_t nr_bucket(uint32_t id) { return (id * 0x9E3779B1u) >> 26; }
 
+// The 8 partial sums of a lane group (lanes 8g .. 8g+7) added in lane order into 0.0f, valid in
+// the group's first lane. A group never crosses a 16-lane DPP row, so lane 8g reads lane 8g + j
+// with row_shl:j: register moves, no LDS round trip.
+__device__ inline float lane8_sum_in_order(float acc) {
+    const int v = __float_as_int(acc);
+    float s = 0.0f + acc;
+    s = s + __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x101, 0xF, 0xF, false));
+    s = s + __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x102, 0xF, 0xF, false));
+    s = s + __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x103, 0xF, 0xF, false));
+    s = s + __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x104, 0xF, 0xF, false));
+    s = s + __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x105, 0xF, 0xF, false));
+    s = s + __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x106, 0xF, 0xF, false));
+    s = s + __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x107, 0xF, 0xF, false));
+    return s;
+}
+
 // Reference-order distance of two rows, computed by the 8 lanes of a lane group: lane j of the
 // group accumulates dims j, j+8, ... below floor(d/8)*8 with fused multiply-add; the 8 partial sums
-// are added in lane order into 0.0f; the tail dims are added with fused multiply-add. Every lane
-// of the group returns the same value, equal to dist_ref(a, b, d).
+// are added in lane order into 0.0f; the tail dims are added with fused multiply-add. The group's
+// first lane (lane % 8 == 0) returns dist_ref(a, b, d); the other lanes return partial values.
 __device__ inline float dist_ref_lane8(const float* __restrict__ a, const float* __restrict__ b,
                                        uint32_t d, uint32_t lane) {
-    const uint32_t main = d & ~7u, l8 = lane & 7u, base = lane & ~7u;
+    const uint32_t main = d & ~7u, l8 = lane & 7u;
     float acc = 0.f;
     for (uint32_t i = l8; i < main; i += 8) {
         const float t = a[i] - b[i];
         acc = __builtin_fmaf(t, t, acc);
     }
-    float s = 0.0f;
-#pragma unroll
-    for (uint32_t j = 0; j < 8; ++j) s = s + __shfl(acc, base + j);
+    float s = lane8_sum_in_order(acc);
     for (uint32_t i = main; i < d; ++i) {
         const float t = a[i] - b[i];
         s = __builtin_fmaf(t, t, s);
@@ -284,16 +298,14 @@ template <uint32_t D>
 __device__ inline float dist_ref_lane8_fixed(const float* __restrict__ a, const float* __restrict__ b,
                                              uint32_t lane) {
     constexpr uint32_t kMain = D & ~7u;
-    const uint32_t l8 = lane & 7u, base = lane & ~7u;
+    const uint32_t l8 = lane & 7u;
     float acc = 0.f;
 #pragma unroll
     for (uint32_t i = 0; i < kMain; i += 8) {
         const float t = a[i + l8] - b[i + l8];
         acc = __builtin_fmaf(t, t, acc);
     }
-    float s = 0.0f;
-#pragma unroll
-    for (uint32_t j = 0; j < 8; ++j) s = s + __shfl(acc, base + j);
+    float s = lane8_sum_in_order(acc);
 #pragma unroll
     for (uint32_t i = kMain; i < D; ++i) {
         const float t = a[i] - b[i];
