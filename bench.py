@@ -1,7 +1,8 @@
 """Benchmark: KNN-graph build throughput on one MI355X.
 
 Workload (default): 10,000,000 x 100 float32 points, low-rank embedding-like data (24 latent dims,
-sigma_i = 1/sqrt(1+i), noise 0.05, unit norm), k = 100, leaf <= 400, generated on the device so the
+sigma_i = 1/sqrt(1+i), noise 0.05, unit norm), k = 100, leaf <= 400, 6 split rounds, 2
+neighbours-of-neighbours passes, generated on the device so the
 input is resident in HBM before timing. One step = one whole KNN-graph build (all split rounds,
 leaf stages, the neighbours-of-neighbours pass and the finalize) through the C ABI
 (knng_build_device). value = points / second of the build.
@@ -46,7 +47,9 @@ def parse():
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--d", type=int, default=100)
     ap.add_argument("--k", type=int, default=100)
-    ap.add_argument("--rounds", type=int, default=8)
+    # the smallest round count whose recall clears 0.94 with margin at 10M (2 NoN passes):
+    # 5 rounds give 0.9432 / 0.9403 on two seeds, 6 give 0.9537 / 0.9512
+    ap.add_argument("--rounds", type=int, default=6)
     ap.add_argument("--leaf", type=int, default=400)
     ap.add_argument("--iterations", type=int, default=1)
     ap.add_argument("--non", type=int, default=2)

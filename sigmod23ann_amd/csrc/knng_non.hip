@@ -314,6 +314,36 @@ __device__ inline float dist_ref_lane8_fixed(const float* __restrict__ a, const 
     return s;
 }
 
+// dist_ref (the reference's order: eight accumulators over dims j, j+8, ... below floor(d/8)*8,
+// summed in order into 0.0f, then the tail with fused multiply-add) for a compile-time width D,
+// fully unrolled so that every LDS load of the two rows is issued at once. Rows are 16-byte
+// aligned.
+template <uint32_t D>
+__device__ inline float dist_ref_fixed(const float* __restrict__ a, const float* __restrict__ b) {
+    constexpr uint32_t kMain = D & ~7u;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (uint32_t i = 0; i < kMain; i += 4) {
+        const float4 x = *reinterpret_cast<const float4*>(a + i);
+        const float4 y = *reinterpret_cast<const float4*>(b + i);
+        const uint32_t j = i & 4u;
+        float t;
+        t = x.x - y.x; acc[j + 0] = __builtin_fmaf(t, t, acc[j + 0]);
+        t = x.y - y.y; acc[j + 1] = __builtin_fmaf(t, t, acc[j + 1]);
+        t = x.z - y.z; acc[j + 2] = __builtin_fmaf(t, t, acc[j + 2]);
+        t = x.w - y.w; acc[j + 3] = __builtin_fmaf(t, t, acc[j + 3]);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) s = s + acc[j];
+#pragma unroll
+    for (uint32_t i = kMain; i < D; ++i) {
+        const float t = a[i] - b[i];
+        s = __builtin_fmaf(t, t, s);
+    }
+    return s;
+}
+
 // FP32 Gram of a 16x16 tile over K = 4 KS dims, four independent accumulation chains (the
 // screen's error bound holds for any summation order). KS is a compile-time constant for the
 // common widths; KS = 0 is the generic runtime loop.
@@ -538,21 +568,34 @@ void k_non_reverse(NonRevArgs a) {
         uint32_t qn = 0;
         auto flush = [&]() {
             __builtin_amdgcn_wave_barrier();
-            for (uint32_t e0 = 0; e0 < qn; e0 += 8) {
-                const uint32_t e = e0 + (lane >> 3);
-                const uint32_t ent = e < qn ? s_queue[e] : 0u;
+            // up to 8 survivors: eight lanes each (one short pass); more: one lane each, every LDS
+            // load of a distance issued at once. Both give dist_ref's value.
+            const bool wide = qn <= 8;
+            for (uint32_t e0 = 0; e0 < qn; e0 += wide ? 8 : 64) {
+                const uint32_t e = e0 + (wide ? lane >> 3 : lane);
+                const bool act = e < qn;
+                const uint32_t ent = act ? s_queue[e] : 0u;
                 const uint32_t i = ent >> 7, cc = ent & 127;
-                float ex;
-                if constexpr (KS > 0) {
-                    if (d == 4 * KS)
-                        ex = dist_ref_lane8_fixed<4 * KS>(s_brow + cc * ldp, s_arow + i * ldp, lane);
-                    else
-                        ex = dist_ref_lane8(s_brow + cc * ldp, s_arow + i * ldp, d, lane);
-                } else {
-                    ex = dist_ref_lane8(s_brow + cc * ldp, s_arow + i * ldp, d, lane);
+                const float* rb = s_brow + cc * ldp;
+                const float* ra = s_arow + i * ldp;
+                float ex = 0.f;
+                if (wide) {
+                    if constexpr (KS > 0) {
+                        if (d == 4 * KS) ex = dist_ref_lane8_fixed<4 * KS>(rb, ra, lane);
+                        else ex = dist_ref_lane8(rb, ra, d, lane);
+                    } else {
+                        ex = dist_ref_lane8(rb, ra, d, lane);
+                    }
+                } else if (act) {
+                    if constexpr (KS > 0) {
+                        if (d == 4 * KS) ex = dist_ref_fixed<4 * KS>(rb, ra);
+                        else ex = dist_ref(rb, ra, d);
+                    } else {
+                        ex = dist_ref(rb, ra, d);
+                    }
                 }
                 // <=: a distance equal to the bound can still win on its id in the merge
-                const bool keep = e < qn && (lane & 7) == 0 && ex <= s_abnd[i];
+                const bool keep = act && (!wide || (lane & 7) == 0) && ex <= s_abnd[i];
                 const uint64_t bal = __ballot(keep);
                 if (bal) {
                     uint32_t base = 0;
