@@ -137,6 +137,9 @@ __device__ inline float lane_bcast(float v, uint32_t src) {
 __device__ inline uint32_t lane_bcast(uint32_t v, uint32_t src) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)src);
 }
+__device__ inline uint64_t lane_bcast64(uint64_t v, uint32_t src) {
+    return ((uint64_t)lane_bcast((uint32_t)(v >> 32), src) << 32) | lane_bcast((uint32_t)v, src);
+}
 // Lane - 1 of v (lane 0 receives `fill`): DPP wave_shr:1, a register move with no LDS round trip.
 __device__ inline uint32_t wave_shr1(uint32_t v, uint32_t fill) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138, 0xF, 0xF, false);

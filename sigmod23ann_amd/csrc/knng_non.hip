@@ -27,6 +27,8 @@
 // be rejected again (bounds only shrink, lists keep the k smallest keys), so a later pass screens
 // new rows of RS[q] against all of S[q] and old rows against S[q]'s new entries only.
 #include "knng_device.hpp"
+#include <algorithm>
+
 #include "knng_kernels.hpp"
 
 namespace knng {
@@ -203,15 +205,19 @@ constexpr uint32_t kNrChunk = 32;     // reverse-neighbour rows per chunk
 constexpr uint32_t kNrMaxK4 = 32;     // ld / 4 for ld <= 128 (larger ld takes the direct kernel)
 constexpr uint32_t kNrIds = 512;      // reverse-neighbour ids staged at a time
 constexpr uint32_t kNrQueue = 320;    // per-wave survivor queue; flushed before it could overflow
-constexpr uint32_t kNrPfRows = (kNrChunk * (kNrMaxK4 + 1) + kNrThreads - 1) / kNrThreads;  // float4 per thread (padded stride <= 132 floats)
+constexpr uint32_t kNrPfRows = kNrChunk * kNrMaxK4 / kNrThreads;  // float4 per thread: dims < ld <= 128
 constexpr uint32_t kNrPfSnap = kNrChunk * 128 / kNrThreads;        // snapshot ids per thread
 constexpr uint32_t kNrSurv = 512;     // accepted offers held in LDS per workgroup before appending
+typedef __attribute__((address_space(3))) uint32_t kLdsU32;
+constexpr uint32_t kNrOversub = 4;    // persistent workgroups per resident slot
+static_assert(kNrIds == kNrThreads, "a q's first reverse ids are staged one per thread");
 
 __host__ __device__ inline uint32_t nr_row_stride(uint32_t ld) { return ld % 8 == 0 ? ld + 4 : ld; }
 
 struct NrLds {
     uint32_t kb16, ldp;
-    uint32_t off_bid, off_bnrm, off_brow, off_hkey, off_hval, off_hcnt, off_rsid;
+    uint32_t off_bid, off_bnrm, off_brow, off_hkey, off_hval, off_hcnt, off_rsid, off_rsid2, off_nsq, off_nbn,
+        off_meta;
     uint32_t off_aid, off_abnd, off_anrm, off_arow, off_mask, off_queue;
     uint32_t off_spid, off_srid, off_sd, off_cnt, total;
 };
@@ -237,6 +243,10 @@ __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
     L.off_hval = o; o += kNrHash;
     L.off_hcnt = o; o += kNrBuckets * 4;
     L.off_rsid = o; o += kNrIds * 4;
+    L.off_rsid2 = o; o += kNrIds * 4;
+    L.off_nsq = o;  o += 128 * 4;
+    L.off_nbn = o;  o += 128 * 4;
+    L.off_meta = o; o += 32;
     L.off_aid = o;  o += kNrChunk * 4;
     L.off_abnd = o; o += kNrChunk * 4;
     L.off_anrm = o; o += kNrChunk * 4;
@@ -377,22 +387,30 @@ __global__ __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu(4, 4
 void k_non_reverse(NonRevArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t k = a.k, d = a.d, ld = a.ld;
-    // XCD-aware order: the blocks an XCD runs take one contiguous stretch of this launch's q list
-    const uint32_t per = (a.nq + 7) / 8;
-    const uint32_t idx = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (idx >= a.nq) return;
-    const uint32_t q = a.order[(size_t)idx * a.order_stride];
-    const uint64_t a0 = a.roff[q], a1 = a.roff[q + 1];
-    if (a0 == a1) return;
+    // Persistent, XCD-aware schedule: the workgroups of one XCD (blockIdx % 8) share one contiguous
+    // stretch of this launch's q list, each taking every nslot-th q of it. A q's metadata (its
+    // position, reverse-list offsets, snapshot slots and first reverse ids) is loaded while the
+    // previous q is processed, so only the row gathers start cold.
+    const uint32_t per = (a.nq + 7) / 8, xcd = blockIdx.x & 7, nslot = gridDim.x >> 3;
+    const uint32_t i_end = min(a.nq, (xcd + 1) * per);
+    uint32_t idx = xcd * per + (blockIdx.x >> 3);
+    if (idx >= i_end) return;
     const NrLds L = nr_layout(k, ld);
     const uint32_t ldp = L.ldp, d4 = (d + 3) / 4, ldp4 = ldp / 4;
     uint32_t* s_bid = reinterpret_cast<uint32_t*>(lds + L.off_bid);
     float* s_bnrm = reinterpret_cast<float*>(lds + L.off_bnrm);
     float* s_brow = reinterpret_cast<float*>(lds + L.off_brow);
+    __attribute__((address_space(3))) float* s_brow_lds =
+        (__attribute__((address_space(3))) float*)(s_brow);
     uint32_t* s_hkey = reinterpret_cast<uint32_t*>(lds + L.off_hkey);
     uint8_t* s_hval = lds + L.off_hval;
     uint32_t* s_hcnt = reinterpret_cast<uint32_t*>(lds + L.off_hcnt);
-    uint32_t* s_rsid = reinterpret_cast<uint32_t*>(lds + L.off_rsid);
+    // reverse ids of the current q and of the next one (the two swap every q)
+    uint32_t* s_rsbuf0 = reinterpret_cast<uint32_t*>(lds + L.off_rsid);
+    uint32_t* s_rsbuf1 = reinterpret_cast<uint32_t*>(lds + L.off_rsid2);
+    uint32_t* s_nsq = reinterpret_cast<uint32_t*>(lds + L.off_nsq);  // next S(q): snapshot slots
+    float* s_nbn = reinterpret_cast<float*>(lds + L.off_nbn);        // ... and their norms
+    uint64_t* s_meta = reinterpret_cast<uint64_t*>(lds + L.off_meta);  // next q: q, roff[q], roff[q+1]
     uint32_t* s_aid = reinterpret_cast<uint32_t*>(lds + L.off_aid);
     float* s_abnd = reinterpret_cast<float*>(lds + L.off_abnd);
     float* s_anrm = reinterpret_cast<float*>(lds + L.off_anrm);
@@ -402,89 +420,52 @@ void k_non_reverse(NonRevArgs a) {
     uint32_t* s_srid = reinterpret_cast<uint32_t*>(lds + L.off_srid);
     float* s_sd = reinterpret_cast<float*>(lds + L.off_sd);
     uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds + L.off_cnt);  // [0] accepted, [1] screened
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t tid0 = threadIdx.x, lane0 = tid0 & 63, wave0 = tid0 >> 6;
+
+
+    // The current q's metadata: the first q's is loaded here; every later q's arrives in LDS
+    // (s_nsq, s_nbn, the other reverse-id buffer) while the previous q is processed, through
+    // global_load_lds (lane i of a wave writes at the wave's base + 4 i), with no registers held.
+    uint32_t q = a.order[(size_t)idx * a.order_stride];
+    uint64_t a0 = a.roff[q], a1 = a.roff[q + 1];
+    for (uint32_t t = tid0; t < k; t += kNrThreads) {
+        const uint32_t v = a.snap[(size_t)q * k + t];
+        s_nsq[t] = v;
+        s_nbn[t] = a.norms[v & kIdMask];
+    }
+    for (uint32_t t = tid0; t < min(a1 - a0, (uint64_t)kNrIds); t += kNrThreads) s_rsbuf0[t] = a.rs[a0 + t];
+    __syncthreads();
+    uint32_t par = 0;
+
+    for (;;) {
+    // Per-thread indices are re-derived each q from an opaque copy of the thread id: hoisted out
+    // of this loop they would stay live across it and spill, and a spill reload waits for every
+    // load in flight, prefetches included.
+    uint32_t tid = tid0;
+    asm volatile("" : "+v"(tid));
+    const uint32_t lane = tid & 63, wave = tid >> 6;
     uint16_t* s_queue = reinterpret_cast<uint16_t*>(lds + L.off_queue) + wave * kNrQueue;
-    const uint32_t nrs = (uint32_t)(a1 - a0);
-    const uint32_t nn = a.rnew ? min(a.rnew[q], nrs) : nrs;  // RS(q)'s new rows come first
     // staging map: 16 threads per chunk row
     const uint32_t sr = tid >> 4, sc = tid & 15;
-
-    // ---- S(q): ids, norms, hash table id -> slot, rows ---------------------------------------
-    for (uint32_t t = tid; t < kNrHash; t += kNrThreads) s_hkey[t] = kNoId;
-    if (tid < kNrBuckets) s_hcnt[tid] = 0;
-    if (tid < 2) s_cnt[tid] = 0;
-    if (wave == 0) {  // S(q) with its new entries first (slots lane, lane + 64; k <= 128)
-        const uint32_t* sq = a.snap + (size_t)q * k;
-        const uint32_t v0 = lane < k ? sq[lane] : 0u, v1 = lane + 64 < k ? sq[lane + 64] : 0u;
-        const bool n0 = lane < k && (v0 & kNewBit), n1 = lane + 64 < k && (v1 & kNewBit);
-        const uint64_t m0 = __ballot(n0), m1 = __ballot(n1);
-        const uint64_t below = (1ull << lane) - 1ull;
-        const uint32_t nnew = __popcll(m0) + __popcll(m1);
-        const uint32_t b0 = __popcll(m0 & below), b1 = __popcll(m0) + __popcll(m1 & below);
-        if (lane < k) {
-            const uint32_t pos = n0 ? b0 : nnew + lane - b0, id = v0 & kIdMask;
-            s_bid[pos] = id;
-            s_bnrm[pos] = a.norms[id];
-        }
-        if (lane + 64 < k) {
-            const uint32_t pos = n1 ? b1 : nnew + lane + 64 - b1, id = v1 & kIdMask;
-            s_bid[pos] = id;
-            s_bnrm[pos] = a.norms[id];
-        }
-        if (lane == 0) s_cnt[2] = nnew;
-    }
-    for (uint32_t t = k + tid; t < L.kb16; t += kNrThreads) {
-        s_bid[t] = kNoId;
-        s_bnrm[t] = 0.f;
-    }
-    uint32_t g0 = 0;  // first reverse entry of the staged ids group
-    for (uint32_t t = tid; t < min(nrs, kNrIds); t += kNrThreads) s_rsid[t] = a.rs[a0 + t];
-    __syncthreads();
-    // old rows pair with S(q)'s new entries only; with none of either there is nothing to offer
-    const uint32_t ntile_new = (s_cnt[2] + 15) / 16;
-    const uint32_t ne = ntile_new ? nrs : nn;
-    if (ne == 0) return;
-    // chunks never straddle the new/old boundary
-    auto cend = [&](uint32_t c) { return c < nn ? min(c + kNrChunk, nn) : min(c + kNrChunk, ne); };
-    // Id 0 may be the padding of an unfilled list rather than a member, so it is never hashed:
-    // offers of id 0 always go through (the list's id check settles them).
-    if (tid < k) {
-        const uint32_t id = s_bid[tid];
-        if (id != 0) {
-            const uint32_t b = nr_bucket(id);
-            const uint32_t slot = atomicAdd(&s_hcnt[b], 1u);
-            if (slot < kNrSlots) {
-                s_hkey[b * kNrSlots + slot] = id;
-                s_hval[b * kNrSlots + slot] = (uint8_t)tid;
-            }
-        }
-    }
-    {  // every S(q) row load in flight at once
-        const uint32_t tot = L.kb16 * ldp4;
-        float4 v[kNrBRows];
-#pragma unroll
-        for (uint32_t u = 0; u < kNrBRows; ++u) {
-            const uint32_t t = tid + u * kNrThreads;
-            const uint32_t r = t / ldp4, q4 = t - r * ldp4;
-            v[u] = (t < tot && r < k && q4 < d4)
-                       ? reinterpret_cast<const float4*>(a.X + (size_t)s_bid[r] * ld)[q4]
-                       : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kNrBRows; ++u) {
-            const uint32_t t = tid + u * kNrThreads;
-            if (t < tot) reinterpret_cast<float4*>(s_brow)[t] = v[u];
-        }
-    }
-
-    // ---- chunk staging: global -> registers (fetch), registers -> LDS + membership (commit) --
+    uint32_t* s_rsid = par ? s_rsbuf1 : s_rsbuf0;  // RS(q)'s first kNrIds entries, staged
+    uint32_t* s_nrs = par ? s_rsbuf0 : s_rsbuf1;   // the next q's
+    const uint32_t nidx = idx + nslot;
+    const bool has_next = nidx < i_end;
+    // next q, step 1: its position (vector loads by lane 0 of each wave: a scalar load would
+    // share its wait counter with the LDS operations below)
+    uint32_t nq_l = 0;
+    if (has_next && lane == 0) nq_l = a.order[(size_t)nidx * a.order_stride];
+    const uint32_t nrs = (uint32_t)(a1 - a0);
+    const uint32_t nn = a.rnew ? min(a.rnew[q], nrs) : nrs;  // RS(q)'s new rows come first
+    // ---- chunk staging: global -> registers (fetch) ------------------------------------------
     float4 pf_row[kNrPfRows];
     uint32_t pf_snap[kNrPfSnap];
     uint32_t pf_id = kNoId, pf_na = 0;
     float pf_bnd = 0.f, pf_nrm = 0.f;
-    auto fetch = [&](uint32_t c_off, uint32_t na) {
+    // id: this thread's chunk row (row sr), valid for sr < na
+    auto fetch_id = [&](uint32_t id, uint32_t na) {
         pf_na = na;
-        pf_id = sr < na ? s_rsid[c_off + sr] : kNoId;
+        pf_id = sr < na ? id : kNoId;
         if (sc == 0 && sr < na) {
             pf_bnd = a.bound[pf_id];
             pf_nrm = a.norms[pf_id];
@@ -503,6 +484,113 @@ void k_non_reverse(NonRevArgs a) {
             pf_snap[u] = (sr < na && j < k) ? sp[j] : 0u;
         }
     };
+    auto fetch = [&](uint32_t c_off, uint32_t na) { fetch_id(sr < na ? s_rsid[c_off + sr] : 0u, na); };
+
+    // ---- S(q): ids, norms, hash table id -> slot, rows ---------------------------------------
+    for (uint32_t t = tid; t < kNrHash; t += kNrThreads) s_hkey[t] = kNoId;
+    if (tid < kNrBuckets) s_hcnt[tid] = 0;
+    if (tid < 2) s_cnt[tid] = 0;
+    if (wave == 0) {  // S(q) with its new entries first (slots lane, lane + 64; k <= 128)
+        const uint32_t v0 = lane < k ? s_nsq[lane] : 0u, v1 = lane + 64 < k ? s_nsq[lane + 64] : 0u;
+        const float bn0 = lane < k ? s_nbn[lane] : 0.f, bn1 = lane + 64 < k ? s_nbn[lane + 64] : 0.f;
+        const bool n0 = lane < k && (v0 & kNewBit), n1 = lane + 64 < k && (v1 & kNewBit);
+        const uint64_t m0 = __ballot(n0), m1 = __ballot(n1);
+        const uint64_t below = (1ull << lane) - 1ull;
+        const uint32_t nnew = __popcll(m0) + __popcll(m1);
+        const uint32_t b0 = __popcll(m0 & below), b1 = __popcll(m0) + __popcll(m1 & below);
+        if (lane < k) {
+            const uint32_t pos = n0 ? b0 : nnew + lane - b0, id = v0 & kIdMask;
+            s_bid[pos] = id;
+            s_bnrm[pos] = bn0;
+        }
+        if (lane + 64 < k) {
+            const uint32_t pos = n1 ? b1 : nnew + lane + 64 - b1, id = v1 & kIdMask;
+            s_bid[pos] = id;
+            s_bnrm[pos] = bn1;
+        }
+        if (lane == 0) s_cnt[2] = nnew;
+    }
+    for (uint32_t t = k + tid; t < L.kb16; t += kNrThreads) {
+        s_bid[t] = kNoId;
+        s_bnrm[t] = 0.f;
+    }
+    uint32_t g0 = 0;  // first reverse entry of the staged ids group
+    __syncthreads();
+    // next q, step 2 (wave 0 has read s_nsq and s_nbn): reverse-list offsets (lane 0 of each wave)
+    // and its snapshot slots (wave 0, into s_nsq)
+    uint32_t nq = 0;
+    uint64_t nr0 = 0, nr1 = 0;
+    if (has_next) {
+        nq = lane_bcast(nq_l, 0);
+        if (lane == 0) {
+            nr0 = a.roff[nq];
+            nr1 = a.roff[nq + 1];
+        }
+        if (wave == 0) {
+            const uint32_t* sq = a.snap + (size_t)nq * k;
+            if (lane < k) __builtin_amdgcn_global_load_lds(sq + lane, (kLdsU32*)s_nsq, 4, 0, 0);
+            if (lane + 64 < k) __builtin_amdgcn_global_load_lds(sq + 64 + lane, (kLdsU32*)(s_nsq + 64), 4, 0, 0);
+        }
+    }
+    // next q, step 3 (after two barriers): its first reverse ids (into s_nrs) and its S(q)'s norms
+    auto prefetch_rs = [&]() {
+        if (!has_next) return;
+        __builtin_amdgcn_s_waitcnt(0);  // the offsets, and s_nsq for wave 0
+        const uint64_t na0 = lane_bcast64(nr0, 0), na1 = lane_bcast64(nr1, 0);
+        if (tid == 0) {  // through LDS rather than registers live across the chunk loop
+            s_meta[0] = nq;
+            s_meta[1] = na0;
+            s_meta[2] = na1;
+        }
+        if (tid < min(na1 - na0, (uint64_t)kNrIds))
+            __builtin_amdgcn_global_load_lds(a.rs + na0 + tid, (kLdsU32*)(s_nrs + wave * 64), 4, 0, 0);
+        if (wave == 0) {
+            if (lane < k)
+                __builtin_amdgcn_global_load_lds(a.norms + (s_nsq[lane] & kIdMask), (kLdsU32*)s_nbn, 4, 0, 0);
+            if (lane + 64 < k)
+                __builtin_amdgcn_global_load_lds(a.norms + (s_nsq[lane + 64] & kIdMask), (kLdsU32*)(s_nbn + 64), 4, 0, 0);
+        }
+    };
+    // old rows pair with S(q)'s new entries only; with none of either there is nothing to offer
+    const uint32_t ntile_new = (s_cnt[2] + 15) / 16;
+    const uint32_t ne = ntile_new ? nrs : nn;
+    if (ne > 0) {
+    // chunks never straddle the new/old boundary
+    auto cend = [&](uint32_t c) { return c < nn ? min(c + kNrChunk, nn) : min(c + kNrChunk, ne); };
+    // Id 0 may be the padding of an unfilled list rather than a member, so it is never hashed:
+    // offers of id 0 always go through (the list's id check settles them).
+    if (tid < k) {
+        const uint32_t id = s_bid[tid];
+        if (id != 0) {
+            const uint32_t b = nr_bucket(id);
+            const uint32_t slot = atomicAdd(&s_hcnt[b], 1u);
+            if (slot < kNrSlots) {
+                s_hkey[b * kNrSlots + slot] = id;
+                s_hval[b * kNrSlots + slot] = (uint8_t)tid;
+            }
+        }
+    }
+    {  // S(q)'s rows go from global memory straight into LDS (global_load_lds_dwordx4: lane i of a
+       // wave writes 16 bytes at the wave's base + 16 i), every load in flight at once and no
+       // registers held; slots past a row's dims or past k are zeroed with ordinary stores. The
+       // data is waited for (vmcnt) before the barrier that precedes its first read.
+        const uint32_t tot = L.kb16 * ldp4;
+#pragma unroll
+        for (uint32_t u = 0; u < kNrBRows; ++u) {
+            const uint32_t t = tid + u * kNrThreads;
+            const uint32_t r = t / ldp4, q4 = t - r * ldp4;
+            if (t < tot && r < k && q4 < d4) {
+                __builtin_amdgcn_global_load_lds(
+                    a.X + (size_t)s_bid[r] * ld + 4 * q4,
+                    (__attribute__((address_space(3))) void*)(s_brow_lds + 4 * (u * kNrThreads + wave * 64)),
+                    16, 0, 0);
+            } else if (t < tot) {
+                reinterpret_cast<float4*>(s_brow)[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+    }
+
+    // ---- chunk commit: registers -> LDS + membership ---------------------------------------
     // needs the mask words zeroed (by the caller, one barrier earlier)
     auto commit = [&]() {
         if (sc == 0) {
@@ -544,7 +632,9 @@ void k_non_reverse(NonRevArgs a) {
     for (uint32_t t = tid; t < kNrChunk * 4; t += kNrThreads) s_mask[t] = 0;
     __syncthreads();
     commit();
+    __builtin_amdgcn_s_waitcnt(0);  // S(q)'s rows have landed in LDS
     __syncthreads();
+    prefetch_rs();
 
     // |approx - exact| <= (K + 8) 2^-23 (n_p + n_r) for the FP32 Gram over K = 4 ceil(d/4) terms
     const float margin_c = (float)(4 * d4 + 8) * (1.0f / 8388608.0f);
@@ -692,6 +782,18 @@ void k_non_reverse(NonRevArgs a) {
             atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)s_cnt[1]);
             atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[1]), (unsigned long long)nsurv);
         }
+    }
+    } else {
+        prefetch_rs();
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // the next q's metadata has landed in LDS
+    __syncthreads();                // ... and the next q reuses the LDS
+    if (!has_next) break;
+    idx = nidx;
+    q = (uint32_t)s_meta[0];
+    a0 = s_meta[1];
+    a1 = s_meta[2];
+    par ^= 1u;
     }
 }
 
@@ -842,10 +944,16 @@ void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t
 void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {
     if (!a.nq) return;
     const size_t lds = non_reverse_lds(a.k, a.ld);
-    const uint32_t grid = 8 * ((a.nq + 7) / 8);
     auto go = [&](auto kernel) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        // persistent grid: kNrOversub workgroups per resident slot, a multiple of the 8 XCDs
+        int dev = 0, cus = 0, per_cu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kNrThreads, lds);
+        const uint32_t slots = std::max(1u, (uint32_t)(per_cu * cus) / 8u) * kNrOversub;
+        const uint32_t grid = 8 * std::min((a.nq + 7) / 8, slots);
         kernel<<<grid, kNrThreads, lds, st>>>(a);
     };
     switch ((a.d + 3) / 4) {  // the Gram's k-steps
