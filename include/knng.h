@@ -38,6 +38,9 @@ extern "C" {
 #define KNNG_FLAG_INCREMENTAL_NON 4u /* later neighbours-of-neighbours passes of a build skip offers
                                         whose two hops both predate the previous pass (same lists;
                                         pays when few entries change between passes) */
+#define KNNG_FLAG_SMALL_NON_CAP 8u   /* tests: 16 neighbours-of-neighbours candidate slots per point
+                                        instead of 512, so the overflow hand-off to the per-point
+                                        direct kernel runs */
 
 /* Build parameters. The reference hard-codes k = dims = 100 (src/Constants.hpp:23-24),
  * maxGroupSize 200/400 alternating (src/SolutionKmeans.hpp:582-587), knnIterations = 1 (:591),

@@ -37,6 +37,7 @@ PREC_BF16 = 2
 FLAG_DIRECT_NON = 1
 FLAG_BIG_LEAF = 2
 FLAG_INCREMENTAL_NON = 4
+FLAG_SMALL_NON_CAP = 8  # tests: tiny candidate buffers, so the overflow path runs
 
 GEN_LOWRANK = 0
 GEN_MIXTURE = 1

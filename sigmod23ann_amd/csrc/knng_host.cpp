@@ -175,9 +175,13 @@ struct Builder {
     DevBuf<uint32_t> cand_id, qcount, ovf;
 
     // neighbours-of-neighbours scratch
-    static constexpr uint32_t kNonCap = 192;  // candidate slots per point before the direct path
-    static constexpr uint32_t kNonSubpasses = 16;      // first pass of a build
-    static constexpr uint32_t kNonSubpassesLater = 4;  // later passes: lists are close, buffers small
+    // candidate slots per point before the direct path (41 GB at 10M points); with 6 sub-passes
+    // in the first pass about 1e-5 of the points overflow
+    static constexpr uint32_t kNonCap = 512;
+    static constexpr uint32_t kNonCapSmall = 16;       // KNNG_FLAG_SMALL_NON_CAP (tests)
+    static constexpr uint32_t kNonSubpasses = 6;       // first pass of a build
+    static constexpr uint32_t kNonSubpassesLater = 2;  // later passes: lists are close, buffers small
+    uint32_t non_cap() const { return (prm.flags & KNNG_FLAG_SMALL_NON_CAP) ? kNonCapSmall : kNonCap; }
     DevBuf<uint32_t> ovf_flag;
     DevBuf<uint32_t> rcnt, rfill, rsl, ncnt, nbuf_id;
     DevBuf<uint32_t> snap_prev, rnew, rfill_old;
@@ -634,8 +638,8 @@ struct Builder {
         KNNG_TRY(scan_tmp.ensure(scan_scratch_elems(n)));
         KNNG_TRY(rsl.ensure(nk));
         KNNG_TRY(ncnt.ensure(n));
-        KNNG_TRY(nbuf_d.ensure((size_t)n * kNonCap));
-        KNNG_TRY(nbuf_id.ensure((size_t)n * kNonCap));
+        KNNG_TRY(nbuf_d.ensure((size_t)n * non_cap()));
+        KNNG_TRY(nbuf_id.ensure((size_t)n * non_cap()));
         KNNG_TRY(ovf.ensure((size_t)n + 1));
         KNNG_TRY(ovf_flag.ensure(n));
         KNNG_TRY(perm.ensure(n));
@@ -672,7 +676,7 @@ struct Builder {
         a.ncnt = ncnt.p;
         a.nbuf_d = nbuf_d.p;
         a.nbuf_id = nbuf_id.p;
-        a.cap = kNonCap;
+        a.cap = non_cap();
         a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
         a.counters = verbose ? counters.p : nullptr;  // same-address atomics: diagnostics only
         KNNG_TRY(norms32.ensure(n));
@@ -682,7 +686,8 @@ struct Builder {
         // spread over all of them), each merged before the next: the bounds tighten as the pass
         // goes, which keeps the candidate buffers small.
         // a sub-pass merges every point's list once, so fewer of them save list traffic; the first
-        // pass needs many to keep the candidate buffers within kNonCap
+        // pass needs more, to keep the candidate buffers within their capacity (at 10M points:
+        // 6 sub-passes and 512 slots take 100 ms less than 16 and 192, with the same screen time)
         const uint32_t subs = std::max(1u, std::min(pass_no ? kNonSubpassesLater : kNonSubpasses, n / 1024 + 1));
         float t_rev = 0.f, t_merge = 0.f, ms = 0.f;
         for (uint32_t sp = 0; sp < subs; ++sp) {

@@ -201,6 +201,22 @@ def test_non_matches_oracle(flags):
     assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
 
 
+def test_non_overflow_path_matches_oracle():
+    # 16 candidate slots per point: most buffers overflow and those points are redone by the
+    # direct kernel; the lists must still equal the oracle's
+    X = mixture(6000, 100, 30, 37)
+    k = 20
+    kw = dict(k=k, leaf_max=100, small_max=3000, iterations=1, seed=4)
+    o_prm = oracle.make_params(**kw)
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    st = oracle.leaf_stage(X, k, perm, ls, lz)
+    b = K.Builder(K.params(flags=K.FLAG_SMALL_NON_CAP, **kw))
+    ids, ds = b.non(X, (st[0], st[1]))
+    assert b.stats()["non_overflow"] > 0
+    o_ids, o_ds, _ = oracle.non(X, k, st)
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+
+
 @pytest.mark.parametrize("flags,leaf", [(0, 100), (K.FLAG_DIRECT_NON, 100), (0, 15), (K.FLAG_DIRECT_NON, 15)])
 def test_non_ordered_matches_oracle(flags, leaf):
     # topUpSingleOrdered: only neighbours at ranks >= 10 of 20 expanded. Leaf 100 fills every list;
