@@ -141,6 +141,9 @@ bool valid_params(const knng_params& p, std::string& why) {
 struct Builder {
     knng_params prm{};
     hipStream_t st = nullptr;
+    // A round's split runs on its own stream, so that it overlaps the previous round's leaf stage
+    // (the split reads only the points; the host drives it level by level).
+    hipStream_t sst = nullptr;
     hipEvent_t ev[8] = {};
     knng_stats stats{};
 
@@ -150,7 +153,8 @@ struct Builder {
     uint32_t n = 0, d = 0, ld = 0;
 
     // split state
-    DevBuf<uint32_t> perm, perm_tmp, tile_lo, tile_base, pcnt;
+    DevBuf<uint32_t> perm, perm_next, perm_tmp, tile_lo, tile_base, pcnt;  // perm_next: the other round's
+    uint32_t* split_perm = nullptr;  // the permutation the running split writes
     DevBuf<uint8_t> flags;
     DevBuf<NodeDesc> nodes;
     DevBuf<ChunkDesc> chunks;
@@ -160,14 +164,16 @@ struct Builder {
     PinBuf<NodeDesc> h_nodes;
     PinBuf<ChunkDesc> h_chunks;
     PinBuf<TileDesc> h_tiles;
-    PinBuf<LeafDesc> h_leaves;
-    PinBuf<BigBlock> h_big;
+    // leaf descriptors, one set per round parity; up_ev[s]: set s's upload has been read
+    PinBuf<LeafDesc> h_leaves[2];
+    PinBuf<BigBlock> h_big[2];
+    hipEvent_t up_ev[2] = {};
 
     // top-k state
     DevBuf<uint32_t> knn_ids, snap;
     DevBuf<float> knn_d, bound;
-    DevBuf<LeafDesc> leaves;
-    DevBuf<BigBlock> bigblocks;
+    DevBuf<LeafDesc> leaves[2];
+    DevBuf<BigBlock> bigblocks[2];
     DevBuf<uint64_t> counters;
 
     // exact k-NN scratch
@@ -204,7 +210,10 @@ struct Builder {
             if (e) (void)hipEventDestroy(e);
         for (auto& e : stage_ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : up_ev)
+            if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
+        if (sst) (void)hipStreamDestroy(sst);
     }
 
     bool verbose = false;
@@ -215,8 +224,10 @@ struct Builder {
         if (!valid_params(p, why)) return fail(KNNG_ERR_INVALID, why);
         prm = p;
         KNNG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        KNNG_HIP(hipStreamCreateWithFlags(&sst, hipStreamNonBlocking));
         for (auto& e : ev) KNNG_HIP(hipEventCreate(&e));
         for (auto& e : stage_ev) KNNG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto& e : up_ev) KNNG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         return KNNG_OK;
     }
 
@@ -335,7 +346,7 @@ struct Builder {
         a.X = X;
         a.ld = ld;
         a.d = d;
-        a.perm = perm.p;
+        a.perm = split_perm;
         a.perm_tmp = perm_tmp.p;
         a.nodes = nodes.p;
         a.nnodes = nn;
@@ -358,7 +369,7 @@ struct Builder {
     }
 
     // Upload the tables of `batch` nodes (already in h_nodes), run the level, read back.
-    int run_level(uint32_t nb, uint32_t nc, uint32_t nt, bool lloyd) {
+    int run_level(uint32_t nb, uint32_t nc, uint32_t nt, bool lloyd, hipStream_t s) {
         KNNG_TRY(nodes.ensure(nb));
         KNNG_TRY(chunks.ensure(nc));
         KNNG_TRY(tiles.ensure(nt));
@@ -370,18 +381,18 @@ struct Builder {
         KNNG_TRY(pcnt.ensure((size_t)std::max(nc, 1u) * 2));
         KNNG_TRY(tile_lo.ensure(nt));
         KNNG_TRY(tile_base.ensure((size_t)nt + 1));
-        KNNG_HIP(hipMemcpyAsync(nodes.p, h_nodes.p, nb * sizeof(NodeDesc), hipMemcpyHostToDevice, st));
-        if (nc) KNNG_HIP(hipMemcpyAsync(chunks.p, h_chunks.p, nc * sizeof(ChunkDesc), hipMemcpyHostToDevice, st));
-        KNNG_HIP(hipMemcpyAsync(tiles.p, h_tiles.p, nt * sizeof(TileDesc), hipMemcpyHostToDevice, st));
+        KNNG_HIP(hipMemcpyAsync(nodes.p, h_nodes.p, nb * sizeof(NodeDesc), hipMemcpyHostToDevice, s));
+        if (nc) KNNG_HIP(hipMemcpyAsync(chunks.p, h_chunks.p, nc * sizeof(ChunkDesc), hipMemcpyHostToDevice, s));
+        KNNG_HIP(hipMemcpyAsync(tiles.p, h_tiles.p, nt * sizeof(TileDesc), hipMemcpyHostToDevice, s));
         SplitLevelArgs a = level_args(nb, nc, nt, lloyd);
-        launch_split_prepare(a, st);
-        KNNG_HIP(hipEventRecord(ev[2], st));
-        launch_split_side(a, st);
-        KNNG_HIP(hipEventRecord(ev[3], st));
-        launch_split_partition(a, st);
+        launch_split_prepare(a, s);
+        KNNG_HIP(hipEventRecord(ev[2], s));
+        launch_split_side(a, s);
+        KNNG_HIP(hipEventRecord(ev[3], s));
+        launch_split_partition(a, s);
         KNNG_HIP(hipGetLastError());
-        KNNG_HIP(hipMemcpyAsync(h_nodes.p, nodes.p, nb * sizeof(NodeDesc), hipMemcpyDeviceToHost, st));
-        KNNG_HIP(hipStreamSynchronize(st));
+        KNNG_HIP(hipMemcpyAsync(h_nodes.p, nodes.p, nb * sizeof(NodeDesc), hipMemcpyDeviceToHost, s));
+        KNNG_HIP(hipStreamSynchronize(s));
         float ms = 0.f;
         KNNG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
         stats.side_ms += ms;
@@ -446,12 +457,15 @@ struct Builder {
         return KNNG_OK;
     }
 
-    // One round of splitKmeansBinaryProcess from the identity permutation.
-    int split_round(uint32_t round, std::vector<LeafDesc>& out_leaves) {
-        KNNG_TRY(perm.ensure(n));
+    // One round of splitKmeansBinaryProcess from the identity permutation, into `dst`, on stream
+    // `s` (the host waits for each level on it).
+    int split_round(uint32_t round, std::vector<LeafDesc>& out_leaves, hipStream_t s,
+                    DevBuf<uint32_t>& dst) {
+        KNNG_TRY(dst.ensure(n));
         KNNG_TRY(perm_tmp.ensure(n));
         KNNG_TRY(flags.ensure(n));
-        launch_iota(perm.p, n, st);
+        split_perm = dst.p;
+        launch_iota(dst.p, n, s);
         perm_is_tree = true;
         out_leaves.clear();
         const uint32_t leaf_max = leaf_max_for(round);
@@ -468,7 +482,7 @@ struct Builder {
             while (!pending.empty()) {
                 uint32_t nc = 0, nt = 0;
                 KNNG_TRY(plan(level, pending, nc, nt));
-                KNNG_TRY(run_level((uint32_t)pending.size(), nc, nt, true));
+                KNNG_TRY(run_level((uint32_t)pending.size(), nc, nt, true, s));
                 again.clear();
                 for (uint32_t b = 0; b < pending.size(); ++b) {
                     HostNode& hn = level[pending[b]];
@@ -516,8 +530,16 @@ struct Builder {
     // Leaves are binned by size: each bin of the LDS kernel gets LDS sized for its own largest
     // leaf (small leaves then fit several workgroups per CU), and leaves beyond it (or all of them
     // with KNNG_FLAG_BIG_LEAF) go to the big-bucket kernel as 64-row blocks.
-    int leaf_stage(const std::vector<LeafDesc>& lv) {
-        if (lv.empty()) return KNNG_OK;
+    // Leaf stage of one round on `st`, reading the round's permutation `pm` and its descriptors
+    // through set `slot`. With sync, the host waits and the stage's time is added to the stats;
+    // without, the caller times it from the events e0/e1 recorded around its kernels.
+    int leaf_stage(const std::vector<LeafDesc>& lv, const uint32_t* pm, uint32_t slot, bool sync,
+                   hipEvent_t e0, hipEvent_t e1) {
+        if (lv.empty()) {
+            KNNG_HIP(hipEventRecord(e0, st));
+            KNNG_HIP(hipEventRecord(e1, st));
+            return KNNG_OK;
+        }
         if (d > 128) return fail(KNNG_ERR_UNSUPPORTED, "leaf stage supports d <= 128");
         static constexpr uint32_t kEdges[4] = {64, 128, 256, kMaxLeaf};
         std::vector<LeafDesc> bin[4];
@@ -541,26 +563,32 @@ struct Builder {
             return fail(KNNG_ERR_UNSUPPORTED, "big-bucket leaf kernel does not fit LDS at this d");
         size_t nl = 0;
         for (auto& v : bin) nl += v.size();
-        KNNG_TRY(h_leaves.ensure(std::max<size_t>(nl, 1)));
-        KNNG_TRY(leaves.ensure(std::max<size_t>(nl, 1)));
+        // the pinned set's previous upload (two rounds ago) must have been read before it is refilled
+        KNNG_HIP(hipEventSynchronize(up_ev[slot]));
+        KNNG_TRY(h_leaves[slot].ensure(std::max<size_t>(nl, 1)));
+        KNNG_TRY(leaves[slot].ensure(std::max<size_t>(nl, 1)));
         size_t off = 0;
         for (auto& v : bin) {
-            if (!v.empty()) std::memcpy(h_leaves.p + off, v.data(), v.size() * sizeof(LeafDesc));
+            if (!v.empty()) std::memcpy(h_leaves[slot].p + off, v.data(), v.size() * sizeof(LeafDesc));
             off += v.size();
         }
-        if (nl) KNNG_HIP(hipMemcpyAsync(leaves.p, h_leaves.p, nl * sizeof(LeafDesc), hipMemcpyHostToDevice, st));
+        if (nl)
+            KNNG_HIP(hipMemcpyAsync(leaves[slot].p, h_leaves[slot].p, nl * sizeof(LeafDesc),
+                                    hipMemcpyHostToDevice, st));
         if (!big.empty()) {
-            KNNG_TRY(h_big.ensure(big.size()));
-            std::memcpy(h_big.p, big.data(), big.size() * sizeof(BigBlock));
-            KNNG_TRY(bigblocks.ensure(big.size()));
-            KNNG_HIP(hipMemcpyAsync(bigblocks.p, h_big.p, big.size() * sizeof(BigBlock), hipMemcpyHostToDevice, st));
+            KNNG_TRY(h_big[slot].ensure(big.size()));
+            std::memcpy(h_big[slot].p, big.data(), big.size() * sizeof(BigBlock));
+            KNNG_TRY(bigblocks[slot].ensure(big.size()));
+            KNNG_HIP(hipMemcpyAsync(bigblocks[slot].p, h_big[slot].p, big.size() * sizeof(BigBlock),
+                                    hipMemcpyHostToDevice, st));
         }
+        KNNG_HIP(hipEventRecord(up_ev[slot], st));
         LeafArgs a{};
         a.X = X;
         a.ld = ld;
         a.d = d;
         a.k = kl();
-        a.perm = perm.p;
+        a.perm = pm;
         a.knn_ids = knn_ids.p;
         a.knn_d = knn_d.p;
         a.bound = bound.p;
@@ -568,13 +596,13 @@ struct Builder {
         a.rerank_cap = prm.rerank;
         a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
         a.counters = counters.p;
-        KNNG_HIP(hipEventRecord(ev[4], st));
+        KNNG_HIP(hipEventRecord(e0, st));
         off = 0;
         for (auto& v : bin) {
             if (!v.empty()) {
                 uint32_t mx = 0;
                 for (auto& l : v) mx = std::max(mx, l.size);
-                a.leaves = leaves.p + off;
+                a.leaves = leaves[slot].p + off;
                 a.nleaves = (uint32_t)v.size();
                 a.max_leaf = mx;
                 launch_leaf(a, st);
@@ -585,15 +613,17 @@ struct Builder {
         if (!big.empty()) {
             a.leaves = nullptr;
             a.nleaves = 0;
-            launch_leaf_big(a, bigblocks.p, (uint32_t)big.size(), st);
+            launch_leaf_big(a, bigblocks[slot].p, (uint32_t)big.size(), st);
             stats.leaf_launches++;
         }
         KNNG_HIP(hipGetLastError());
-        KNNG_HIP(hipEventRecord(ev[5], st));
-        KNNG_HIP(hipEventSynchronize(ev[5]));
-        float ms = 0.f;
-        KNNG_HIP(hipEventElapsedTime(&ms, ev[4], ev[5]));
-        stats.leaf_ms += ms;
+        KNNG_HIP(hipEventRecord(e1, st));
+        if (sync) {
+            KNNG_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            KNNG_HIP(hipEventElapsedTime(&ms, e0, e1));
+            stats.leaf_ms += ms;
+        }
         stats.leaves += lv.size();
         stats.leaf_pairs += pairs;
         return KNNG_OK;
@@ -783,24 +813,56 @@ struct Builder {
         KNNG_HIP(hipMemsetAsync(counters.p, 0, 4 * sizeof(uint64_t), st));
         KNNG_HIP(hipEventRecord(ev[0], st));
         launch_init_knn(knn_ids.p, knn_d.p, bound.p, n, kl(), st);
-        std::vector<LeafDesc> lv;
-        for (uint32_t r = 0; r < prm.rounds; ++r) {
-            KNNG_HIP(hipEventRecord(ev[6], st));
-            KNNG_TRY(split_round(r, lv));
-            KNNG_HIP(hipEventRecord(ev[7], st));
-            KNNG_HIP(hipEventSynchronize(ev[7]));
-            float ms = 0.f;
-            KNNG_HIP(hipEventElapsedTime(&ms, ev[6], ev[7]));
-            stats.split_ms += ms;
-            KNNG_TRY(leaf_stage(lv));
+        // Rounds are pipelined over two streams: split r+1 (sst, host-driven level by level) runs
+        // while leaf stage r (st) does. Leaf r waits for split r; split r+1 writes the permutation
+        // buffer that leaf r-1 read, so it waits for leaf r-1. Events per round: split start/end
+        // (sst), leaf start/end (st).
+        const uint32_t R = prm.rounds;
+        std::vector<hipEvent_t> rev(4 * (size_t)R, nullptr);
+        struct EvGuard {
+            std::vector<hipEvent_t>& v;
+            ~EvGuard() {
+                for (auto e : v)
+                    if (e) (void)hipEventDestroy(e);
+            }
+        } guard{rev};
+        for (auto& e : rev) KNNG_HIP(hipEventCreate(&e));
+        DevBuf<uint32_t>* pb[2] = {&perm, &perm_next};
+        std::vector<LeafDesc> lv[2];
+        KNNG_HIP(hipEventRecord(ev[2], st));  // the points are bound (uploads on st) before any split
+        KNNG_HIP(hipStreamWaitEvent(sst, ev[2], 0));
+        auto split = [&](uint32_t r) -> int {
+            if (r >= 2) KNNG_HIP(hipStreamWaitEvent(sst, rev[4 * (r - 2) + 3], 0));
+            KNNG_HIP(hipEventRecord(rev[4 * r], sst));
+            KNNG_TRY(split_round(r, lv[r & 1], sst, *pb[r & 1]));
+            KNNG_HIP(hipEventRecord(rev[4 * r + 1], sst));
+            return KNNG_OK;
+        };
+        if (R) KNNG_TRY(split(0));
+        for (uint32_t r = 0; r < R; ++r) {
+            KNNG_HIP(hipStreamWaitEvent(st, rev[4 * r + 1], 0));
+            KNNG_TRY(leaf_stage(lv[r & 1], pb[r & 1]->p, r & 1, false, rev[4 * r + 2], rev[4 * r + 3]));
+            if (r + 1 < R) KNNG_TRY(split(r + 1));
             if (verbose) {
                 uint64_t c[4];
+                KNNG_HIP(hipStreamSynchronize(st));
                 KNNG_HIP(hipMemcpy(c, counters.p, sizeof(c), hipMemcpyDeviceToHost));
-                std::fprintf(stderr, "[knng] round %u: split %.2f ms leaves %zu survivors %llu inserts %llu (cumulative leaf %.1f ms)\n",
-                             r, ms, lv.size(), (unsigned long long)c[0], (unsigned long long)c[2],
-                             stats.leaf_ms);
+                std::fprintf(stderr, "[knng] round %u: leaves %zu survivors %llu inserts %llu\n", r,
+                             lv[r & 1].size(), (unsigned long long)c[0], (unsigned long long)c[2]);
             }
         }
+        KNNG_HIP(hipStreamSynchronize(st));
+        KNNG_HIP(hipStreamSynchronize(sst));
+        for (uint32_t r = 0; r < R; ++r) {
+            float ms = 0.f;
+            KNNG_HIP(hipEventElapsedTime(&ms, rev[4 * r], rev[4 * r + 1]));
+            stats.split_ms += ms;
+            KNNG_HIP(hipEventElapsedTime(&ms, rev[4 * r + 2], rev[4 * r + 3]));
+            stats.leaf_ms += ms;
+        }
+        // neighbours of neighbours take the last round's tree order from perm
+        if (R && ((R - 1) & 1)) perm.swap(perm_next);
+        perm_is_tree = R > 0;
         uint64_t leafc[4];
         KNNG_HIP(hipMemcpy(leafc, counters.p, sizeof(leafc), hipMemcpyDeviceToHost));
         stats.survivors = leafc[0];
@@ -958,7 +1020,7 @@ int knng_split_round(knng_handle* h, const float* points, uint32_t n, uint32_t d
     Builder& b = h->b;
     KNNG_TRY(b.bind_points(points, false, n, d, row_stride));
     std::vector<knng::LeafDesc> lv;
-    KNNG_TRY(b.split_round(round, lv));
+    KNNG_TRY(b.split_round(round, lv, b.st, b.perm));
     if (n) KNNG_HIP(hipMemcpyAsync(perm_out, b.perm.p, (size_t)n * 4, hipMemcpyDeviceToHost, b.st));
     KNNG_HIP(hipStreamSynchronize(b.st));
     for (size_t i = 0; i < lv.size(); ++i) {
@@ -1014,7 +1076,8 @@ int knng_partition_injected(knng_handle* h, const float* points, uint32_t n, uin
         std::memcpy(&cf[(size_t)i * b.ld], coefs + (size_t)i * d, d * sizeof(float));
     KNNG_HIP(hipMemcpyAsync(b.coef.p, cf.data(), cf.size() * 4, hipMemcpyHostToDevice, b.st));
     KNNG_HIP(hipMemcpyAsync(b.off.p, offsets, n_nodes * 4, hipMemcpyHostToDevice, b.st));
-    KNNG_TRY(b.run_level(n_nodes, 0, nt, false));
+    b.split_perm = b.perm.p;
+    KNNG_TRY(b.run_level(n_nodes, 0, nt, false, b.st));
     for (uint32_t i = 0; i < n_nodes; ++i) lo_out[i] = b.h_nodes.p[i].lo;
     KNNG_HIP(hipMemcpy(perm, b.perm.p, (size_t)n * 4, hipMemcpyDeviceToHost));
     return KNNG_OK;
@@ -1039,7 +1102,7 @@ int knng_leaf_stage(knng_handle* h, const float* points, uint32_t n, uint32_t d,
     KNNG_HIP(hipMemcpyAsync(b.perm.p, perm, (size_t)n * 4, hipMemcpyHostToDevice, b.st));
     KNNG_TRY(upload_state(b, ids, dists, start_empty != 0));
     KNNG_HIP(hipMemsetAsync(b.counters.p, 0, 4 * sizeof(uint64_t), b.st));
-    KNNG_TRY(b.leaf_stage(lv));
+    KNNG_TRY(b.leaf_stage(lv, b.perm.p, 0, true, b.ev[4], b.ev[5]));
     return download_state(b, ids, dists);
 }
 
