@@ -54,6 +54,10 @@ def parse():
     ap.add_argument("--iterations", type=int, default=1)
     ap.add_argument("--non", type=int, default=2)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"])
+    ap.add_argument("--splitter", default="kmeans", choices=["kmeans", "kd"],
+                    help="kmeans: SolutionKmeans' 2-means split; kd: SolutionRandomKD's "
+                         "random-rotation split (the reference runs it with 5 rounds, leaves < 400, "
+                         "no NoN: --splitter kd --rounds 5 --leaf 399 --non 0)")
     ap.add_argument("--seed", type=int, default=2023)
     ap.add_argument("--recall-queries", type=int, default=100_000)
     ap.add_argument("--cpu-sample", type=int, default=250_000)
@@ -280,8 +284,9 @@ def main():
         copy_calibration(a.copy_calibration_mib)
 
     prec = {"fp32": K.PREC_FP32, "fp16": K.PREC_FP16, "bf16": K.PREC_BF16}[a.precision]
+    flags = K.FLAG_KD_SPLIT if a.splitter == "kd" else 0
     prm = K.params(k=a.k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
-                   rounds=a.rounds, non_passes=a.non, precision=prec, seed=a.seed)
+                   rounds=a.rounds, non_passes=a.non, precision=prec, seed=a.seed, flags=flags)
     b = K.Builder(prm)
     n, d, k = a.n, a.d, a.k
     ld = (d + 3) & ~3
@@ -319,16 +324,29 @@ def main():
         ids = dIds.download((n, k), np.uint32)
         got = np.ascontiguousarray(ids[q])
         dq = K.DeviceArray(nq * 4)
-        dq.upload(q)
         dl = K.DeviceArray(nq * k * 4)
-        dl.upload(got)
         dub = K.DeviceArray(nq * 8)
-        builder.exact_upper_from_lists(dX.ptr, n, d, ld, dq.ptr, nq, dl.ptr, k, dub.ptr)
         dti = K.DeviceArray(nq * k * 4)
         dtd = K.DeviceArray(nq * k * 8)
-        builder.exact_knn_device(dX.ptr, n, d, ld, dq.ptr, nq, k, dub.ptr, 4 * k + 256, dti.ptr,
-                                 dtd.ptr)
-        truth = dti.download((nq, k), np.uint32)
+        truth = np.empty((nq, k), np.uint32)
+        # batches of queries: a weak graph gives loose bounds, and the exact scan's re-scan of
+        # queries with many candidates is memory-capped per call (halve a batch that overflows)
+        q0, batch = 0, 100_000
+        while q0 < nq:
+            m = min(batch, nq - q0)
+            dq.upload(q[q0:q0 + m])
+            dl.upload(got[q0:q0 + m])
+            builder.exact_upper_from_lists(dX.ptr, n, d, ld, dq.ptr, m, dl.ptr, k, dub.ptr)
+            try:
+                builder.exact_knn_device(dX.ptr, n, d, ld, dq.ptr, m, k, dub.ptr, 4 * k + 256,
+                                         dti.ptr, dtd.ptr)
+            except K.KnngError as e:
+                if e.code != -5 or batch <= 64:
+                    raise
+                batch //= 2
+                continue
+            truth[q0:q0 + m] = dti.download((m, k), np.uint32)
+            q0 += m
         rec = K.recall(got, truth)
         log(f"[bench] recall@{k} on {nq} sampled points vs FP64: {rec:.4f} "
             f"({time.perf_counter() - t0:.1f} s)")
@@ -433,7 +451,8 @@ def main():
         Xh = dX.download((n, ld), np.float32)[:ns, :d].copy()
         threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
         op = oracle.make_params(k=k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
-                                rounds=a.rounds, non_passes=a.non, seed=a.seed)
+                                rounds=a.rounds, non_passes=a.non, seed=a.seed,
+                                splitter=1 if a.splitter == "kd" else 0)
         t0 = time.perf_counter()
         oracle.build_graph(Xh, op, threads=threads)
         ct = time.perf_counter() - t0
@@ -450,7 +469,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"low-rank {n}x{d} (24 latent, unit norm), k={k}, leaf<={a.leaf}",
                    "n": n, "d": d, "k": k, "rounds": a.rounds, "iterations": a.iterations,
-                   "non_passes": a.non, "precision": a.precision,
+                   "non_passes": a.non, "precision": a.precision, "splitter": a.splitter,
                    "recall_at_k": None if rec is None else round(rec, 4),
                    "recall_queries": nq, "parallelism": "single GPU"},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),

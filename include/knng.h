@@ -38,6 +38,8 @@ extern "C" {
 #define KNNG_FLAG_INCREMENTAL_NON 4u /* later neighbours-of-neighbours passes of a build skip offers
                                         whose two hops both predate the previous pass (same lists;
                                         pays when few entries change between passes) */
+#define KNNG_FLAG_KD_SPLIT 16u      /* split rounds with the random-rotation KD splitter
+                                        (SolutionRandomKD) instead of 2-means */
 #define KNNG_FLAG_SMALL_NON_CAP 8u   /* tests: 16 neighbours-of-neighbours candidate slots per point
                                         instead of 512, so the overflow hand-off to the per-point
                                         direct kernel runs */

@@ -37,6 +37,8 @@ class Params(C.Structure):
         ("rounds", C.c_uint32),
         ("non_passes", C.c_uint32),
         ("non_rank_lo", C.c_uint32),
+        ("splitter", C.c_uint32),
+        ("pad_", C.c_uint32),
         ("seed", C.c_uint64),
     ]
 
@@ -54,10 +56,11 @@ class Stats(C.Structure):
 
 
 def make_params(k=100, leaf_max=256, leaf_max_odd=0, small_max=3000, iterations=1, rounds=1,
-                non_passes=0, seed=1234, non_rank_lo=0, **_ignored) -> Params:
-    """Reference build parameters; GPU-only knobs (flags, precision, search width) are ignored."""
+                non_passes=0, seed=1234, non_rank_lo=0, splitter=0, **_ignored) -> Params:
+    """Reference build parameters; GPU-only knobs (flags, precision, search width) are ignored.
+    splitter: 0 = 2-means (SolutionKmeans), 1 = random-rotation KD (SolutionRandomKD)."""
     return Params(k, leaf_max, leaf_max_odd, small_max, iterations, rounds, non_passes, non_rank_lo,
-                  seed)
+                  splitter, 0, seed)
 
 
 _P = C.POINTER
@@ -118,6 +121,11 @@ def lib():
                               _f32p, _u32p, _u32p, _f32p, _u32p, C.c_int, C.c_uint32]
         L.orc_build.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, _P(Params), _u32p,
                                 _f32p, _u32p, C.c_int, _P(Stats)]
+        L.orc_kd_directions.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
+        L.orc_kd_normal.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_kd_normal.restype = C.c_double
+        L.orc_kd_pick_dim.argtypes = [_f64p, C.c_uint32, C.c_uint64]
+        L.orc_kd_pick_dim.restype = C.c_uint32
         L.orc_exact_knn.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, C.c_uint32,
                                     C.c_uint32, _u32p, _f64p, C.c_int]
         _lib = L
@@ -229,6 +237,22 @@ def split_round(X, params: Params, round_idx: int, threads: int = 0):
     lib().orc_split_round(_fp(X), n, d, d, C.byref(params), round_idx, _up(perm), _up(ls),
                           _up(lz), C.byref(nl), threads)
     return perm, ls[: nl.value].copy(), lz[: nl.value].copy()
+
+
+def kd_directions(seed: int, round_idx: int, m: int, d: int) -> np.ndarray:
+    """The KD splitter's projection directions for a round (rotateRandom, SolutionRandomKD.hpp:258-264)."""
+    out = np.empty((m, d), np.float32)
+    lib().orc_kd_directions(seed, round_idx, m, d, _fp(out))
+    return out
+
+
+def kd_normal(key: int, i: int) -> float:
+    return lib().orc_kd_normal(key, i)
+
+
+def kd_pick_dim(var, node_key: int) -> int:
+    v = np.ascontiguousarray(var, np.float64)
+    return lib().orc_kd_pick_dim(v.ctypes.data_as(_f64p), v.size, node_key)
 
 
 def partition_injected(X, perm, node_start, node_size, coefs, offsets):

@@ -23,6 +23,9 @@
 //    combinable concatenation order at :262-304 is scheduling-dependent, stable is one instance).
 //  * Degenerate splits retry with a new attempt key (the `goto` at :177-179, :203-205, :251-253,
 //    :291-293); after 8 failed attempts the node is split at its middle position.
+//  * Random-rotation KD splitter (SolutionRandomKD.hpp): normal draws by Box-Muller on the counter
+//    stream; node FP64 sums sequential within tiles of 1024 positions, tiles added in order; a split
+//    that leaves a side empty (infinite recursion in the reference) cuts the node at its middle.
 
 #include <immintrin.h>
 
@@ -223,6 +226,7 @@ struct Params {
     uint32_t k, leaf_max, leaf_max_odd, small_max, iterations, rounds, non_passes;
     uint64_t seed;
     uint32_t non_rank_lo;
+    uint32_t splitter;  // 0: 2-means (SolutionKmeans), 1: random-rotation KD (SolutionRandomKD)
 };
 
 struct Points {
@@ -401,6 +405,113 @@ static uint32_t split_node(const Points& P, const Params& prm, uint32_t* perm, u
     return n / 2;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Random-rotation KD splitter, SolutionRandomKD.hpp:258-428.
+// ---------------------------------------------------------------------------------------------
+constexpr uint32_t kKdDims = 30;    // newDims, SolutionRandomKD.hpp:387
+constexpr uint32_t kKdTile = 1024;  // FP64 sums: sequential within tiles, tiles added in order
+
+// N(0, 1) by Box-Muller on two counter draws (in place of std::normal_distribution over rd,
+// LinearAlgebra.hpp:104).
+static double kd_normal(uint64_t key, uint64_t i) {
+    const uint64_t a = mix64(key + 0x9E3779B97F4A7C15ull * (2 * i + 1));
+    const uint64_t b = mix64(key + 0x9E3779B97F4A7C15ull * (2 * i + 2));
+    const double u1 = ((double)(a >> 11) + 1.0) * 0x1p-53;
+    const double u2 = (double)(b >> 11) * 0x1p-53;
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+// rotateRandom's directions (SolutionRandomKD.hpp:258-264): randUniformUnitVec
+// (LinearAlgebra.hpp:103-116; norm's float sum of squares, :79-85; normalizeInPlace's division by
+// its double root, :87-92), then gramSchmidt (:211-222) as written: normalize (:94-101) returns its
+// argument, so u_0 = v_0 and u_i = v_i - sum_{j<i} dot(u_j, v_i) * u_j (project, :204-206; sub and
+// scalarMult, :146-161), with no renormalization.
+static std::vector<float> kd_directions(uint64_t seed, uint32_t round, uint32_t m, uint32_t d) {
+    const uint64_t key = mix64(round_key(seed, round) ^ 0x6A09E667F3BCC909ull);
+    std::vector<float> v((size_t)m * d), u((size_t)m * d);
+    for (uint32_t c = 0; c < m; ++c) {
+        float sum_sq = 0.0f;
+        for (uint32_t j = 0; j < d; ++j) {
+            const float x = (float)kd_normal(key, (uint64_t)c * d + j);
+            v[(size_t)c * d + j] = x;
+            sum_sq += x * x;
+        }
+        const double norm = std::sqrt((double)sum_sq);
+        for (uint32_t j = 0; j < d; ++j) v[(size_t)c * d + j] = (float)(v[(size_t)c * d + j] / norm);
+    }
+    for (uint32_t i = 0; i < m; ++i) {
+        for (uint32_t e = 0; e < d; ++e) u[(size_t)i * d + e] = v[(size_t)i * d + e];
+        for (uint32_t j = 0; j < i; ++j) {
+            const float c = dot_ref(&u[(size_t)j * d], &v[(size_t)i * d], d);
+            for (uint32_t e = 0; e < d; ++e) u[(size_t)i * d + e] -= c * u[(size_t)j * d + e];
+        }
+    }
+    return u;
+}
+
+// pickIndexFull's draw (SolutionRandomKD.hpp:223-248) given the node's variances.
+static uint32_t kd_pick_dim(const std::vector<double>& var, uint64_t node_key) {
+    const uint32_t m = (uint32_t)var.size();
+    float total_var = 0;
+    for (uint32_t j = 0; j < m; ++j) total_var += var[j];
+    std::vector<std::pair<double, uint32_t>> by_var;
+    for (uint32_t j = 0; j < m; ++j) by_var.emplace_back(var[j], j);
+    std::sort(by_var.begin(), by_var.end(), std::greater<>{});
+    const float u01 = (float)(mix64(node_key + 0x9E3779B97F4A7C15ull) >> 40) * 0x1p-24f;
+    const float sample_var = u01 * total_var;
+    float var_so_far = 0;
+    for (auto& [vj, j] : by_var) {
+        var_so_far += vj;
+        if (var_so_far >= sample_var) return j;
+    }
+    return by_var.back().second;  // the reference's __builtin_unreachable
+}
+
+// One node (split, SolutionRandomKD.hpp:282-373): FP64 means and sample variances of the
+// projections over the range (getMean / getVariance, :104-151), a dimension by pickIndexFull, the
+// mean as split value; x < mean goes first (:338-339), stable. Returns the first group's size.
+static uint32_t kd_split_node(const std::vector<float>& rot, uint32_t m, uint32_t* perm,
+                              uint32_t start, uint32_t n, uint64_t key, SplitStats& st) {
+    std::vector<double> mean(m), var(m);
+    for (uint32_t c = 0; c < m; ++c) {
+        double total = 0.0;
+        for (uint32_t t0 = 0; t0 < n; t0 += kKdTile) {
+            double s = 0.0;
+            for (uint32_t e = t0; e < std::min(n, t0 + kKdTile); ++e)
+                s += (double)rot[(size_t)perm[start + e] * m + c];
+            total += s;
+        }
+        mean[c] = total / (double)n;
+    }
+    for (uint32_t c = 0; c < m; ++c) {
+        double total = 0.0;
+        for (uint32_t t0 = 0; t0 < n; t0 += kKdTile) {
+            double s = 0.0;
+            for (uint32_t e = t0; e < std::min(n, t0 + kKdTile); ++e) {
+                const double diff = (double)rot[(size_t)perm[start + e] * m + c] - mean[c];
+                s += diff * diff;
+            }
+            total += s;
+        }
+        var[c] = total / (double)(n - 1);
+    }
+    const uint32_t dim = kd_pick_dim(var, key);
+    const double split_value = mean[dim];
+    std::vector<uint32_t> g1, g2;
+    for (uint32_t e = 0; e < n; ++e) {
+        const uint32_t id = perm[start + e];
+        ((double)rot[(size_t)id * m + dim] < split_value ? g1 : g2).push_back(id);
+    }
+    if (g1.empty() || g2.empty()) {  // the reference would recurse on the same range forever
+        st.attempts_failed++;
+        st.fallbacks++;
+        return n / 2;
+    }
+    std::memcpy(perm + start, g1.data(), g1.size() * sizeof(uint32_t));
+    std::memcpy(perm + start + g1.size(), g2.data(), g2.size() * sizeof(uint32_t));
+    return (uint32_t)g1.size();
+}
+
 struct Node {
     uint32_t start, n;
     uint64_t key;
@@ -423,6 +534,36 @@ static void split_round(const Points& P, const Params& prm, uint32_t round, uint
         return;
     }
     level.push_back({0, n, round_key(prm.seed, round)});
+    if (prm.splitter == 1) {  // SolutionRandomKD: project (rotateRandom, :258-274), then split
+        const uint32_t m = std::min(kKdDims, P.d);
+        const std::vector<float> u = kd_directions(prm.seed, round, m, P.d);
+        std::vector<float> rot((size_t)n * m);
+        parallel_for(n, threads, [&](uint32_t r) {
+            for (uint32_t c = 0; c < m; ++c) rot[(size_t)r * m + c] = dot_ref(&u[(size_t)c * P.d], P.row(r), P.d);
+        });
+        while (!level.empty()) {
+            std::vector<uint32_t> nlo(level.size());
+            std::vector<SplitStats> sts(level.size());
+            parallel_for((uint32_t)level.size(), threads, [&](uint32_t i) {
+                nlo[i] = kd_split_node(rot, m, perm, level[i].start, level[i].n, level[i].key, sts[i]);
+            });
+            std::vector<Node> next;
+            for (uint32_t i = 0; i < level.size(); ++i) {
+                st.attempts_failed += sts[i].attempts_failed;
+                st.fallbacks += sts[i].fallbacks;
+                const Node& nd = level[i];
+                Node ch[2] = {{nd.start, nlo[i], child_key(nd.key, 0)},
+                              {nd.start + nlo[i], nd.n - nlo[i], child_key(nd.key, 1)}};
+                for (auto& c : ch) {
+                    if (c.n <= leaf_max) leaves.push_back({c.start, c.n});
+                    else next.push_back(c);
+                }
+            }
+            level.swap(next);
+        }
+        std::sort(leaves.begin(), leaves.end(), [](const Leaf& a, const Leaf& b) { return a.start < b.start; });
+        return;
+    }
     while (!level.empty()) {
         std::vector<uint32_t> nlo(level.size());
         std::vector<SplitStats> sts(level.size());
@@ -600,6 +741,8 @@ struct orc_params {
     uint32_t rounds;
     uint32_t non_passes;
     uint32_t non_rank_lo;   // 0: topUpSingle; r > 0: topUpSingleOrdered expanding ranks >= r
+    uint32_t splitter;      // 0: 2-means (SolutionKmeans); 1: random-rotation KD (SolutionRandomKD)
+    uint32_t pad_;
     uint64_t seed;
 };
 
@@ -609,6 +752,15 @@ struct orc_stats {
 };
 
 float orc_dist(const float* a, const float* b, uint32_t d) { return dist_ref(a, b, d); }
+// the KD splitter's directions for (seed, round): m rows of d floats
+void orc_kd_directions(uint64_t seed, uint32_t round, uint32_t m, uint32_t d, float* out) {
+    const std::vector<float> u = kd_directions(seed, round, m, d);
+    std::memcpy(out, u.data(), u.size() * sizeof(float));
+}
+double orc_kd_normal(uint64_t key, uint64_t i) { return kd_normal(key, i); }
+uint32_t orc_kd_pick_dim(const double* var, uint32_t m, uint64_t node_key) {
+    return kd_pick_dim(std::vector<double>(var, var + m), node_key);
+}
 float orc_dot(const float* a, const float* b, uint32_t d) { return dot_ref(a, b, d); }
 uint32_t orc_sample_count(uint32_t n) { return sample_count(n); }
 uint32_t orc_start_count(uint32_t n) { return start_count(n); }
@@ -657,7 +809,7 @@ int orc_split_round(const float* X, uint32_t n, uint32_t d, uint32_t ld, const o
                     uint32_t* n_leaves, int threads) {
     Points P{X, n, d, ld};
     Params p{prm->k, prm->leaf_max, prm->leaf_max_odd, prm->small_max, prm->iterations,
-             prm->rounds, prm->non_passes, prm->seed, prm->non_rank_lo};
+             prm->rounds, prm->non_passes, prm->seed, prm->non_rank_lo, prm->splitter};
     std::vector<Leaf> leaves;
     SplitStats st;
     split_round(P, p, round, perm_out, leaves, pick_threads(threads), st);
@@ -735,7 +887,7 @@ int orc_build(const float* X, uint32_t n, uint32_t d, uint32_t ld, const orc_par
               orc_stats* stats) {
     Points P{X, n, d, ld};
     Params p{prm->k, prm->leaf_max, prm->leaf_max_odd, prm->small_max, prm->iterations,
-             prm->rounds, prm->non_passes, prm->seed, prm->non_rank_lo};
+             prm->rounds, prm->non_passes, prm->seed, prm->non_rank_lo, prm->splitter};
     const uint32_t T = pick_threads(threads);
     std::vector<KnnSetRef> knn(n);
     for (auto& s : knn) s.init(p.k);

@@ -38,6 +38,7 @@ FLAG_DIRECT_NON = 1
 FLAG_BIG_LEAF = 2
 FLAG_INCREMENTAL_NON = 4
 FLAG_SMALL_NON_CAP = 8  # tests: tiny candidate buffers, so the overflow path runs
+FLAG_KD_SPLIT = 16  # random-rotation KD splitter (SolutionRandomKD) instead of 2-means
 
 GEN_LOWRANK = 0
 GEN_MIXTURE = 1

@@ -20,6 +20,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -138,6 +139,86 @@ bool valid_params(const knng_params& p, std::string& why) {
 
 }  // namespace
 
+// ---- random-rotation KD splitter: host parts (SolutionRandomKD.hpp:258-428) ----------------
+// The reference-order dot (LinearAlgebra.hpp:179-200) on the host: 8 lane accumulators with fused
+// multiply-add over dims below floor(d/8)*8, lanes added in order into 0.0f, then the tail fused.
+static float host_dot_ref(const float* a, const float* b, uint32_t d) {
+    const uint32_t main = d & ~7u;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (uint32_t i = 0; i < main; i += 8)
+        for (uint32_t j = 0; j < 8; ++j) acc[j] = std::fmaf(a[i + j], b[i + j], acc[j]);
+    float s = 0.0f;
+    for (uint32_t j = 0; j < 8; ++j) s = s + acc[j];
+    for (uint32_t i = main; i < d; ++i) s = std::fmaf(a[i], b[i], s);
+    return s;
+}
+
+// One N(0, 1) draw: Box-Muller on two counter-based draws of the stream `key` (in place of
+// std::normal_distribution over the thread-local mt19937, LinearAlgebra.hpp:104).
+static double kd_normal(uint64_t key, uint64_t i) {
+    const uint64_t h1 = knng::mix64(key + 0x9E3779B97F4A7C15ull * (2 * i + 1));
+    const uint64_t h2 = knng::mix64(key + 0x9E3779B97F4A7C15ull * (2 * i + 2));
+    const double u1 = ((double)(h1 >> 11) + 1.0) * (1.0 / 9007199254740992.0);  // (0, 1]
+    const double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);          // [0, 1)
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+// rotateRandom's m directions for a round (SolutionRandomKD.hpp:258-264), as the reference
+// computes them: randUniformUnitVec (LinearAlgebra.hpp:103-116: float draws, divided by the
+// double square root of their float sum of squares, :79-92), then gramSchmidt (:211-222) whose
+// normalize (:94-101) returns its argument unchanged: u_0 = v_0 and u_i = v_i - sum_j
+// dot(u_j, v_i) u_j, not renormalized. Rows of d floats.
+static std::vector<float> kd_directions(uint64_t seed, uint32_t round, uint32_t m, uint32_t d) {
+    const uint64_t key = knng::mix64(knng::round_key(seed, round) ^ 0x6A09E667F3BCC909ull);
+    std::vector<float> v((size_t)m * d), u((size_t)m * d);
+    for (uint32_t c = 0; c < m; ++c) {
+        float* vc = &v[(size_t)c * d];
+        float ss = 0.0f;
+        for (uint32_t j = 0; j < d; ++j) {
+            vc[j] = (float)kd_normal(key, (uint64_t)c * d + j);
+            const float sq = vc[j] * vc[j];
+            ss = ss + sq;
+        }
+        const double nrm = std::sqrt((double)ss);
+        for (uint32_t j = 0; j < d; ++j) vc[j] = (float)((double)vc[j] / nrm);
+    }
+    for (uint32_t i = 0; i < m; ++i) {
+        float* ui = &u[(size_t)i * d];
+        const float* vi = &v[(size_t)i * d];
+        std::memcpy(ui, vi, d * sizeof(float));
+        for (uint32_t j = 0; j < i; ++j) {
+            const float* uj = &u[(size_t)j * d];
+            const float c = host_dot_ref(uj, vi, d);
+            for (uint32_t e = 0; e < d; ++e) {
+                const float p = c * uj[e];
+                ui[e] = ui[e] - p;
+            }
+        }
+    }
+    return u;
+}
+
+// pickIndexFull's choice (SolutionRandomKD.hpp:223-248): the dimensions ordered by variance,
+// descending (ties: the larger index first, std::greater on (variance, index)); a float draw in
+// [0, totalVar) with totalVar the float sum of the variances in index order; the first dimension
+// whose float running sum reaches the draw (the last one if rounding leaves the sum short, where
+// the reference reaches __builtin_unreachable).
+static uint32_t kd_pick(const double* var, uint32_t m, uint64_t node_key) {
+    float total = 0.0f;
+    for (uint32_t j = 0; j < m; ++j) total = (float)((double)total + var[j]);
+    std::vector<std::pair<double, uint32_t>> order(m);
+    for (uint32_t j = 0; j < m; ++j) order[j] = {var[j], j};
+    std::sort(order.begin(), order.end(), std::greater<std::pair<double, uint32_t>>());
+    const float u = (float)(knng::mix64(node_key + 0x9E3779B97F4A7C15ull) >> 40) * (1.0f / 16777216.0f);
+    const float sample = u * total;
+    float so_far = 0.0f;
+    for (auto& [vj, j] : order) {
+        so_far = (float)((double)so_far + vj);
+        if (so_far >= sample) return j;
+    }
+    return order.back().second;
+}
+
 struct Builder {
     knng_params prm{};
     hipStream_t st = nullptr;
@@ -164,6 +245,13 @@ struct Builder {
     PinBuf<NodeDesc> h_nodes;
     PinBuf<ChunkDesc> h_chunks;
     PinBuf<TileDesc> h_tiles;
+    // KD splitter: projected rows, directions, per-node statistics and choices
+    DevBuf<float> kd_rot, kd_dirs;
+    DevBuf<double> kd_part, kd_mean, kd_var, kd_split;
+    DevBuf<uint32_t> kd_dim;
+    PinBuf<double> h_kd_stat;   // per node: kKdStride means then kKdStride variances
+    PinBuf<double> h_kd_split;
+    PinBuf<uint32_t> h_kd_dim;
     // leaf descriptors, one set per round parity; up_ev[s]: set s's upload has been read
     PinBuf<LeafDesc> h_leaves[2];
     PinBuf<BigBlock> h_big[2];
@@ -467,6 +555,7 @@ struct Builder {
         split_perm = dst.p;
         launch_iota(dst.p, n, s);
         perm_is_tree = true;
+        if (prm.flags & KNNG_FLAG_KD_SPLIT) return kd_split_round(round, out_leaves, s, dst);
         out_leaves.clear();
         const uint32_t leaf_max = leaf_max_for(round);
         if (n <= leaf_max) {
@@ -507,6 +596,110 @@ struct Builder {
                 const HostNode& hn = level[i];
                 HostNode ch[2] = {{hn.start, lo[i], child_key(hn.key, 0), 0},
                                   {hn.start + lo[i], hn.size - lo[i], child_key(hn.key, 1), 0}};
+                for (auto& c : ch) {
+                    if (c.size <= leaf_max) out_leaves.push_back({c.start, c.size});
+                    else next.push_back(c);
+                }
+            }
+            level.swap(next);
+        }
+        std::sort(out_leaves.begin(), out_leaves.end(),
+                  [](const LeafDesc& a, const LeafDesc& b) { return a.start < b.start; });
+        return KNNG_OK;
+    }
+
+    // One round of the random-rotation KD splitter (SolutionRandomKD.hpp:258-428) into `dst` (already
+    // the identity): project onto m = min(30, d) directions, then split level by level: each node
+    // above leaf_max takes the FP64 means and sample variances of its projections, a dimension
+    // drawn by variance, and the mean as split value (x < mean first, stable). A node whose split
+    // leaves a side empty (the reference recurses forever) is cut at its middle position.
+    int kd_split_round(uint32_t round, std::vector<LeafDesc>& out_leaves, hipStream_t s,
+                       DevBuf<uint32_t>& dst) {
+        const uint32_t leaf_max = leaf_max_for(round);
+        out_leaves.clear();
+        if (n <= leaf_max) {
+            if (n) out_leaves.push_back({0, n});
+            return KNNG_OK;
+        }
+        const uint32_t m = std::min(kKdDims, d);
+        if (kd_rotate_lds(ld) > 64 * 1024) return fail(KNNG_ERR_UNSUPPORTED, "KD splitter supports d <= 256");
+        const std::vector<float> u = kd_directions(prm.seed, round, m, d);
+        std::vector<float> R((size_t)m * ld, 0.f);
+        for (uint32_t c = 0; c < m; ++c) std::memcpy(&R[(size_t)c * ld], &u[(size_t)c * d], d * sizeof(float));
+        KNNG_TRY(kd_dirs.ensure(R.size()));
+        KNNG_TRY(kd_rot.ensure((size_t)n * kKdStride));
+        KNNG_HIP(hipMemcpyAsync(kd_dirs.p, R.data(), R.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        launch_kd_rotate(X, ld, d, kd_dirs.p, m, n, kd_rot.p, s);
+        KNNG_HIP(hipStreamSynchronize(s));  // R (pageable) is released when this returns
+        std::vector<HostNode> level{{0, n, round_key(prm.seed, round), 0}}, next;
+        while (!level.empty()) {
+            const uint32_t nb = (uint32_t)level.size();
+            uint32_t nt = 0;
+            KNNG_TRY(h_nodes.ensure(nb));
+            for (uint32_t b = 0; b < nb; ++b) {
+                NodeDesc& nd = h_nodes.p[b];
+                nd = NodeDesc{};
+                nd.start = level[b].start;
+                nd.size = level[b].size;
+                nd.tile0 = nt;
+                nd.ntiles = (nd.size + kTileSize - 1) / kTileSize;
+                nd.status = kNodeActive;
+                nt += nd.ntiles;
+            }
+            KNNG_TRY(h_tiles.ensure(nt));
+            for (uint32_t b = 0; b < nb; ++b) {
+                const NodeDesc& nd = h_nodes.p[b];
+                for (uint32_t t = 0; t < nd.ntiles; ++t)
+                    h_tiles.p[nd.tile0 + t] = TileDesc{b, t * kTileSize, std::min(kTileSize, nd.size - t * kTileSize)};
+            }
+            KNNG_TRY(nodes.ensure(nb));
+            KNNG_TRY(tiles.ensure(nt));
+            KNNG_TRY(tile_lo.ensure(nt));
+            KNNG_TRY(tile_base.ensure((size_t)nt + 1));
+            KNNG_TRY(kd_part.ensure((size_t)nt * kKdStride));
+            KNNG_TRY(kd_mean.ensure((size_t)nb * kKdStride));
+            KNNG_TRY(kd_var.ensure((size_t)nb * kKdStride));
+            KNNG_TRY(kd_split.ensure(nb));
+            KNNG_TRY(kd_dim.ensure(nb));
+            KNNG_TRY(h_kd_stat.ensure((size_t)nb * 2 * kKdStride));
+            KNNG_TRY(h_kd_split.ensure(nb));
+            KNNG_TRY(h_kd_dim.ensure(nb));
+            KNNG_HIP(hipMemcpyAsync(nodes.p, h_nodes.p, nb * sizeof(NodeDesc), hipMemcpyHostToDevice, s));
+            KNNG_HIP(hipMemcpyAsync(tiles.p, h_tiles.p, nt * sizeof(TileDesc), hipMemcpyHostToDevice, s));
+            launch_kd_stats(kd_rot.p, dst.p, nodes.p, nb, tiles.p, nt, m, kd_part.p, kd_mean.p, kd_var.p, s);
+            KNNG_HIP(hipMemcpyAsync(h_kd_stat.p, kd_mean.p, (size_t)nb * kKdStride * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+            KNNG_HIP(hipMemcpyAsync(h_kd_stat.p + (size_t)nb * kKdStride, kd_var.p,
+                                    (size_t)nb * kKdStride * sizeof(double), hipMemcpyDeviceToHost, s));
+            KNNG_HIP(hipStreamSynchronize(s));
+            for (uint32_t b = 0; b < nb; ++b) {
+                const double* mean = h_kd_stat.p + (size_t)b * kKdStride;
+                const double* var = h_kd_stat.p + ((size_t)nb + b) * kKdStride;
+                const uint32_t j = kd_pick(var, m, level[b].key);
+                h_kd_dim.p[b] = j;
+                h_kd_split.p[b] = mean[j];
+            }
+            KNNG_HIP(hipMemcpyAsync(kd_dim.p, h_kd_dim.p, nb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+            KNNG_HIP(hipMemcpyAsync(kd_split.p, h_kd_split.p, nb * sizeof(double), hipMemcpyHostToDevice, s));
+            launch_kd_side(kd_rot.p, dst.p, nodes.p, tiles.p, nt, kd_dim.p, kd_split.p, flags.p, tile_lo.p, s);
+            SplitLevelArgs a = level_args(nb, 0, nt, false);
+            launch_split_partition(a, s);
+            KNNG_HIP(hipGetLastError());
+            KNNG_HIP(hipMemcpyAsync(h_nodes.p, nodes.p, nb * sizeof(NodeDesc), hipMemcpyDeviceToHost, s));
+            KNNG_HIP(hipStreamSynchronize(s));
+            next.clear();
+            for (uint32_t b = 0; b < nb; ++b) {
+                const HostNode& hn = level[b];
+                const NodeDesc& nd = h_nodes.p[b];
+                stats.side_rows += hn.size;
+                uint32_t lo = nd.lo;
+                if (nd.status != kNodeActive) {  // a side was empty: the order is unchanged
+                    stats.split_failures++;
+                    stats.split_fallbacks++;
+                    lo = hn.size / 2;
+                }
+                HostNode ch[2] = {{hn.start, lo, child_key(hn.key, 0), 0},
+                                  {hn.start + lo, hn.size - lo, child_key(hn.key, 1), 0}};
                 for (auto& c : ch) {
                     if (c.size <= leaf_max) out_leaves.push_back({c.start, c.size});
                     else next.push_back(c);

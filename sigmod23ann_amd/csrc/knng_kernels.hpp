@@ -56,6 +56,20 @@ struct SplitLevelArgs {
     bool run_lloyd;     // false for injected hyperplanes
 };
 
+// Random-rotation KD splitter (knng_kd.hip): projections onto kKdDims directions, rows of
+// kKdStride floats; per-node FP64 means and sample variances; side flags for the shared partition.
+constexpr uint32_t kKdDims = 30;    // newDims, SolutionRandomKD.hpp:387
+constexpr uint32_t kKdStride = 32;  // floats per projected row (and per node statistics row)
+size_t kd_rotate_lds(uint32_t ld);
+void launch_kd_rotate(const float* X, uint32_t ld, uint32_t d, const float* R, uint32_t m,
+                      uint32_t n, float* rot, hipStream_t st);
+void launch_kd_stats(const float* rot, const uint32_t* perm, const NodeDesc* nodes, uint32_t nnodes,
+                     const TileDesc* tiles, uint32_t ntiles, uint32_t m, double* part,
+                     double* mean, double* var, hipStream_t st);
+void launch_kd_side(const float* rot, const uint32_t* perm, const NodeDesc* nodes,
+                    const TileDesc* tiles, uint32_t ntiles, const uint32_t* kdim,
+                    const double* ksplit, uint8_t* flags, uint32_t* tile_lo, hipStream_t st);
+
 void launch_iota(uint32_t* perm, uint32_t n, hipStream_t st);
 void launch_split_prepare(const SplitLevelArgs& a, hipStream_t st);    // start pair + Lloyd
 void launch_split_side(const SplitLevelArgs& a, hipStream_t st);       // full-range side test

@@ -420,7 +420,7 @@ void k_non_reverse(NonRevArgs a) {
     uint32_t* s_srid = reinterpret_cast<uint32_t*>(lds + L.off_srid);
     float* s_sd = reinterpret_cast<float*>(lds + L.off_sd);
     uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds + L.off_cnt);  // [0] accepted, [1] screened
-    const uint32_t tid0 = threadIdx.x, lane0 = tid0 & 63, wave0 = tid0 >> 6;
+    const uint32_t tid0 = threadIdx.x;
 
 
     // The current q's metadata: the first q's is loaded here; every later q's arrives in LDS

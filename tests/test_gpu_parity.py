@@ -104,6 +104,31 @@ def test_split_round_lowrank_and_alternating_leaf():
         assert lz.max() <= (199 if r == 0 else 399)
 
 
+@pytest.mark.parametrize("n,d,leaf,seed", [(30000, 100, 399, 5), (12000, 37, 150, 6), (20000, 64, 255, 7)])
+def test_kd_split_round_bitexact(n, d, leaf, seed):
+    # random-rotation KD splitter (SolutionRandomKD): same directions, FP64 node statistics,
+    # dimension draws and stable partition as the oracle
+    X = mixture(n, d, 60, seed)
+    kw = dict(k=10, leaf_max=leaf, seed=seed)
+    b = K.Builder(K.params(flags=K.FLAG_KD_SPLIT, **kw))
+    o_prm = oracle.make_params(splitter=1, **kw)
+    for r in (0, 1):
+        perm, ls, lz = b.split_round(X, r)
+        o_perm, o_ls, o_lz = oracle.split_round(X, o_prm, r)
+        np.testing.assert_array_equal(ls, o_ls)
+        np.testing.assert_array_equal(lz, o_lz)
+        np.testing.assert_array_equal(perm, o_perm)
+
+
+def test_kd_split_identical_points():
+    X = np.ones((3000, 16), np.float32)
+    kw = dict(k=10, leaf_max=99, seed=3)
+    perm, ls, lz = K.Builder(K.params(flags=K.FLAG_KD_SPLIT, **kw)).split_round(X, 0)
+    o_perm, o_ls, o_lz = oracle.split_round(X, oracle.make_params(splitter=1, **kw), 0)
+    np.testing.assert_array_equal(perm, o_perm)
+    np.testing.assert_array_equal(lz, o_lz)
+
+
 def test_split_degenerate_points():
     X = np.ones((5000, 16), np.float32)
     X[:10] += np.arange(10)[:, None]          # a few distinct rows, the rest identical
@@ -275,6 +300,15 @@ def test_build_end_to_end_matches_oracle(k, rounds, non):
     kw = dict(k=k, leaf_max=256, small_max=3000, iterations=2, rounds=rounds, non_passes=non, seed=17)
     ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
     o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(**kw))
+    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+
+
+@pytest.mark.parametrize("non", [0, 1])
+def test_kd_build_matches_oracle(non):
+    X = mixture(15000, 100, 60, 43)
+    kw = dict(k=20, leaf_max=399, rounds=3, non_passes=non, seed=9)
+    ids, ds = K.Builder(K.params(flags=K.FLAG_KD_SPLIT, **kw)).build(X, return_dists=True)
+    o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(splitter=1, **kw))
     assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
 
 

@@ -293,6 +293,123 @@ def test_split_degenerate_identical_points():
 
 
 # ----------------------------------------------------------------------------------------------
+# Random-rotation KD splitter (SolutionRandomKD.hpp:258-428)
+# ----------------------------------------------------------------------------------------------
+_M64 = (1 << 64) - 1
+
+
+def _mix64(z):
+    z &= _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def _kd_key(seed, rnd):
+    rk = _mix64(seed + 0x9E3779B97F4A7C15 * (rnd + 1))
+    return _mix64(rk ^ 0x6A09E667F3BCC909)
+
+
+def test_kd_normal_is_box_muller_of_the_counter_stream():
+    key = _kd_key(11, 2)
+    for i in (0, 1, 7, 12345):
+        a = _mix64(key + 0x9E3779B97F4A7C15 * (2 * i + 1))
+        b = _mix64(key + 0x9E3779B97F4A7C15 * (2 * i + 2))
+        u1 = ((a >> 11) + 1.0) * 2.0 ** -53
+        u2 = (b >> 11) * 2.0 ** -53
+        expect = np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2)
+        assert oracle.kd_normal(key, i) == pytest.approx(expect, rel=1e-14, abs=1e-14)
+    z = np.array([oracle.kd_normal(key, i) for i in range(20000)])
+    assert abs(z.mean()) < 0.03 and abs(z.var() - 1.0) < 0.04
+
+
+@pytest.mark.parametrize("m,d", [(30, 100), (8, 37)])
+def test_kd_directions_follow_the_reference_formula(m, d):
+    # randUniformUnitVec then gramSchmidt with the reference's identity `normalize`
+    # (LinearAlgebra.hpp:94-101): u_0 = v_0, u_i = v_i - sum_j (u_j . v_i) u_j, recomputed in FP64
+    seed, rnd = 7, 3
+    key = _kd_key(seed, rnd)
+    v = np.array([[np.float32(oracle.kd_normal(key, c * d + j)) for j in range(d)] for c in range(m)],
+                 np.float64)
+    v = v / np.sqrt((v * v).sum(axis=1, keepdims=True))
+    u = np.zeros_like(v)
+    for i in range(m):
+        u[i] = v[i] - sum(np.dot(u[j], v[i]) * u[j] for j in range(i))
+    got = oracle.kd_directions(seed, rnd, m, d).astype(np.float64)
+    np.testing.assert_allclose(got, u, rtol=0, atol=2e-5)
+    # not renormalised: the later rows are shorter than 1 (what the reference's bug produces)
+    assert np.linalg.norm(got[0]) == pytest.approx(1.0, abs=1e-6)
+    assert np.linalg.norm(got[-1]) < 0.999
+
+
+def _pick_dim_py(var, key):
+    m = len(var)
+    tot = np.float32(0)
+    for j in range(m):
+        tot = np.float32(np.float64(tot) + var[j])
+    order = sorted(((var[j], j) for j in range(m)), reverse=True)
+    u01 = np.float32(_mix64(key + 0x9E3779B97F4A7C15) >> 40) * np.float32(2.0 ** -24)
+    sample = np.float32(u01 * tot)
+    so_far = np.float32(0)
+    for vj, j in order:
+        so_far = np.float32(np.float64(so_far) + vj)
+        if so_far >= sample:
+            return j
+    return order[-1][1]
+
+
+def test_kd_pick_dim_rule():
+    rng = np.random.default_rng(8)
+    hits = np.zeros(4)
+    for t in range(3000):
+        var = rng.uniform(0, 1, 4)
+        if t % 7 == 0:
+            var[1] = var[2]                     # ties: the larger index comes first
+        key = int(rng.integers(0, 2 ** 63))
+        j = oracle.kd_pick_dim(var, key)
+        assert j == _pick_dim_py(var, key)
+        hits[j] += var[j] / var.sum()
+    assert oracle.kd_pick_dim(np.zeros(5), 3) == 4  # all zero: the draw is 0, the first in order
+    # dimensions are drawn in proportion to their variance
+    var = np.array([1.0, 3.0, 0.0, 6.0])
+    cnt = np.bincount([oracle.kd_pick_dim(var, k) for k in range(20000)], minlength=4) / 20000
+    np.testing.assert_allclose(cnt, var / var.sum(), atol=0.015)
+
+
+def test_kd_split_round_root_is_a_mean_threshold():
+    X = _mixture(6000, 100, 20, 12)
+    p = oracle.make_params(leaf_max=399, splitter=1, seed=21)
+    perm, ls, lz = oracle.split_round(X, p, 0)
+    _check_round(X, perm, ls, lz, 399)
+    perm1, _, lz1 = oracle.split_round(X, p, 0, threads=1)
+    np.testing.assert_array_equal(perm, perm1)
+    # the root's first child is exactly {x : rot_j(x) < mean_j} for one projected dimension j
+    u = oracle.kd_directions(21, 0, 30, 100).astype(np.float64)
+    rot = X.astype(np.float64) @ u.T
+    # recover the root's lo size: the leaves that lie in the first child are those whose points all
+    # fall below the threshold; search j and the split position together
+    found = False
+    for j in range(30):
+        thr = rot[:, j].mean()
+        lo = np.sort(np.where(rot[:, j] < thr)[0])
+        nlo = len(lo)
+        if 0 < nlo < 6000 and np.array_equal(np.sort(perm[:nlo]), lo):
+            found = True
+            break
+    assert found
+    perm2, _, _ = oracle.split_round(X, p, 1)
+    assert not np.array_equal(perm, perm2)
+
+
+def test_kd_split_identical_points_fall_back_to_halves():
+    X = np.ones((2000, 16), np.float32)
+    p = oracle.make_params(leaf_max=99, splitter=1, seed=2)
+    perm, ls, lz = oracle.split_round(X, p, 0)
+    _check_round(X, perm, ls, lz, 99)
+    np.testing.assert_array_equal(perm, np.arange(2000))
+
+
+# ----------------------------------------------------------------------------------------------
 # Leaf stage (KnnSets.hpp:131-160) and NoN (SolutionKmeans.hpp:480-555) against brute force
 # ----------------------------------------------------------------------------------------------
 def _brute_sets(X, groups, k, old=None):
