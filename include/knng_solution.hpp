@@ -1,6 +1,8 @@
-// Drop-in for the reference's solution struct (src/SolutionKmeans.hpp:46, :558): a caller written
-// against `SolutionKmeans::constructResult(points, numPoints, result)` (src/main.cpp:59) compiles
-// unchanged against `SolutionKmeansMI355X::constructResult` and gets the graph built on the GPU.
+// Drop-ins for the reference's solution structs (src/SolutionKmeans.hpp:46, :558;
+// src/SolutionRandomKD.hpp:377): a caller written against
+// `SolutionKmeans::constructResult(points, numPoints, result)` (src/main.cpp:59) compiles unchanged
+// against `SolutionKmeansMI355X::constructResult` (or `SolutionRandomKDMI355X::constructResult`
+// for the other line, :58) and gets the graph built on the GPU.
 //
 // Contract kept from the reference:
 //  * points: numPoints rows of 112 floats (src/io.h:77), the first 100 used; read only, not freed.
@@ -18,23 +20,13 @@
 
 #include "knng.h"
 
-struct SolutionKmeansMI355X {
-    // Build parameters; defaults follow the reference (k = dims = 100, knnIterations = 1, the
-    // 3000-point small branch) with leaf <= 400 and the rounds / NoN passes that reach
-    // recall@100 >= 0.94 on 10M embedding-like points.
-    static knng_params& params() {
-        static knng_params p = [] {
-            knng_params q;
-            knng_default_params(&q);
-            return q;
-        }();
-        return p;
-    }
-
+// Shared body of the drop-in solutions: build with P::params() and copy the ids out.
+template <class P>
+struct KnngSolutionMI355X {
     template <size_t Stride>
     static void constructResult(float points[][Stride], uint32_t numPoints,
                                 std::vector<std::vector<uint32_t>>& result, uint32_t dims = 100) {
-        const knng_params& p = params();
+        const knng_params& p = P::params();
         knng_handle* h = nullptr;
         check(knng_create(&p, &h));
         std::vector<uint32_t> ids((size_t)numPoints * p.k);
@@ -49,5 +41,38 @@ struct SolutionKmeansMI355X {
    private:
     static void check(int rc) {
         if (rc != KNNG_OK) throw std::runtime_error(std::string("knng: ") + knng_last_error());
+    }
+};
+
+struct SolutionKmeansMI355X : KnngSolutionMI355X<SolutionKmeansMI355X> {
+    // Build parameters; defaults follow the reference (k = dims = 100, knnIterations = 1, the
+    // 3000-point small branch) with leaf <= 400 and the rounds / NoN passes that reach
+    // recall@100 >= 0.94 on 10M embedding-like points.
+    static knng_params& params() {
+        static knng_params p = [] {
+            knng_params q;
+            knng_default_params(&q);
+            return q;
+        }();
+        return p;
+    }
+};
+
+// Drop-in for SolutionRandomKD::constructResult (src/SolutionRandomKD.hpp:377, the other line of
+// src/main.cpp:58-59): the random-rotation KD splitter, with the reference's 5 rounds, leaves of
+// fewer than 400 points and no neighbours-of-neighbours pass (:385-417).
+struct SolutionRandomKDMI355X : KnngSolutionMI355X<SolutionRandomKDMI355X> {
+    static knng_params& params() {
+        static knng_params p = [] {
+            knng_params q;
+            knng_default_params(&q);
+            q.flags |= KNNG_FLAG_KD_SPLIT;
+            q.rounds = 5;
+            q.leaf_max = 399;
+            q.leaf_max_odd = 0;
+            q.non_passes = 0;
+            return q;
+        }();
+        return p;
     }
 };

