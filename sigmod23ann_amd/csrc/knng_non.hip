@@ -211,6 +211,8 @@ constexpr uint32_t kNrSurv = 512;     // accepted offers held in LDS per workgro
 typedef __attribute__((address_space(3))) uint32_t kLdsU32;
 constexpr uint32_t kNrOversub = 4;    // persistent workgroups per resident slot
 static_assert(kNrIds == kNrThreads, "a q's first reverse ids are staged one per thread");
+static_assert(kNrChunk == 32 && (kNrThreads / 64) % 2 == 0,
+              "a chunk is one or two 16-row strips, and a wave keeps its strip from job to job");
 
 __host__ __device__ inline uint32_t nr_row_stride(uint32_t ld) { return ld % 8 == 0 ? ld + 4 : ld; }
 
@@ -640,9 +642,6 @@ void k_non_reverse(NonRevArgs a) {
     const float margin_c = (float)(4 * d4 + 8) * (1.0f / 8388608.0f);
     const uint32_t ksteps = d4;
     const uint32_t ntile = L.kb16 / 16;
-    const uint32_t strip = wave & 1, th = wave >> 1;
-    const uint32_t t_per = (ntile + 3) / 4;
-    const uint32_t t_lo = th * t_per, t_hi = min(ntile, t_lo + t_per);
     const uint32_t lrow = lane & 15, lk = lane >> 4;
     uint32_t n_scr = 0;
 
@@ -653,7 +652,8 @@ void k_non_reverse(NonRevArgs a) {
         const uint32_t n0 = c1, n1 = n0 < ne ? cend(n0) : n0;
         const bool next_in_group = n0 < ne && n1 <= g0 + kNrIds;
         if (next_in_group) fetch(n0 - g0, n1 - n0);
-        const uint32_t t_top = c0 >= nn ? min(t_hi, ntile_new) : t_hi;
+        // old rows pair with S(q)'s new columns only (the first ntile_new tiles)
+        const uint32_t tt = c0 >= nn ? ntile_new : ntile;
 
         uint32_t qn = 0;
         auto flush = [&]() {
@@ -711,7 +711,13 @@ void k_non_reverse(NonRevArgs a) {
             __builtin_amdgcn_wave_barrier();
             qn = 0;
         };
-        if (16 * strip < na) {
+        // Jobs (16-row strip, 16-column tile) are dealt to the waves in turn, tile-major: with two
+        // strips a wave keeps one strip (the wave count is even), and a chunk of one strip still
+        // spreads its tiles over every wave.
+        const uint32_t nstrip = (na + 15) / 16;
+        const uint32_t wv = __builtin_amdgcn_readfirstlane(wave);
+        if (wv < nstrip * tt) {
+            const uint32_t strip = nstrip == 2 ? (wv & 1u) : 0u;
             float rn[4], thr0[4];
             uint32_t rr[4], pid[4];
 #pragma unroll
@@ -722,7 +728,8 @@ void k_non_reverse(NonRevArgs a) {
                 thr0[i] = s_abnd[rr[i]] * (1.0f + 1.0f / 65536.0f);
             }
             const float* arow = s_arow + (16 * strip + lrow) * ldp + lk;
-            for (uint32_t t = t_lo; t < t_top; ++t) {
+            for (uint32_t j = wv; j < nstrip * tt; j += kNrThreads / 64) {
+                const uint32_t t = nstrip == 2 ? j >> 1 : j;
                 if (qn + 256 > kNrQueue) flush();
                 const float* brow = s_brow + (16 * t + lrow) * ldp + lk;
                 const float4_t acc = gram_tile_f32<KS>(arow, brow, ksteps);
