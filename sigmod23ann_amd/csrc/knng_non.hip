@@ -425,49 +425,21 @@ void k_non_reverse(NonRevArgs a) {
     const uint32_t tid0 = threadIdx.x;
 
 
-    // The current q's metadata: the first q's is loaded here; every later q's arrives in LDS
-    // (s_nsq, s_nbn, the other reverse-id buffer) while the previous q is processed, through
-    // global_load_lds (lane i of a wave writes at the wave's base + 4 i), with no registers held.
-    uint32_t q = a.order[(size_t)idx * a.order_stride];
-    uint64_t a0 = a.roff[q], a1 = a.roff[q + 1];
-    for (uint32_t t = tid0; t < k; t += kNrThreads) {
-        const uint32_t v = a.snap[(size_t)q * k + t];
-        s_nsq[t] = v;
-        s_nbn[t] = a.norms[v & kIdMask];
-    }
-    for (uint32_t t = tid0; t < min(a1 - a0, (uint64_t)kNrIds); t += kNrThreads) s_rsbuf0[t] = a.rs[a0 + t];
-    __syncthreads();
-    uint32_t par = 0;
-
-    for (;;) {
-    // Per-thread indices are re-derived each q from an opaque copy of the thread id: hoisted out
-    // of this loop they would stay live across it and spill, and a spill reload waits for every
-    // load in flight, prefetches included.
-    uint32_t tid = tid0;
-    asm volatile("" : "+v"(tid));
-    const uint32_t lane = tid & 63, wave = tid >> 6;
-    uint16_t* s_queue = reinterpret_cast<uint16_t*>(lds + L.off_queue) + wave * kNrQueue;
-    // staging map: 16 threads per chunk row
-    const uint32_t sr = tid >> 4, sc = tid & 15;
-    uint32_t* s_rsid = par ? s_rsbuf1 : s_rsbuf0;  // RS(q)'s first kNrIds entries, staged
-    uint32_t* s_nrs = par ? s_rsbuf0 : s_rsbuf1;   // the next q's
-    const uint32_t nidx = idx + nslot;
-    const bool has_next = nidx < i_end;
-    // next q, step 1: its position (vector loads by lane 0 of each wave: a scalar load would
-    // share its wait counter with the LDS operations below)
-    uint32_t nq_l = 0;
-    if (has_next && lane == 0) nq_l = a.order[(size_t)nidx * a.order_stride];
-    const uint32_t nrs = (uint32_t)(a1 - a0);
-    const uint32_t nn = a.rnew ? min(a.rnew[q], nrs) : nrs;  // RS(q)'s new rows come first
-    // ---- chunk staging: global -> registers (fetch) ------------------------------------------
+    // Pipeline. While q is processed, the next q's position (two ahead), reverse-list offsets,
+    // snapshot slots, their norms and first reverse ids arrive in LDS (global_load_lds: lane i of
+    // a wave writes at the wave's base + 4 i or 16 i, no registers held). In q's epilogue the next
+    // q's S is ordered and hashed, its rows are put in flight to LDS and, in full passes, its first
+    // chunk is fetched into registers, so those loads overlap q's candidate appends.
+    const bool early_fetch = a.rnew == nullptr;  // the first chunk's extent needs no new-entry count
     float4 pf_row[kNrPfRows];
     uint32_t pf_snap[kNrPfSnap];
     uint32_t pf_id = kNoId, pf_na = 0;
     float pf_bnd = 0.f, pf_nrm = 0.f;
-    // id: this thread's chunk row (row sr), valid for sr < na
-    auto fetch_id = [&](uint32_t id, uint32_t na) {
+    // ids[sr] for chunk rows sr < na (16 threads per row): bound, norm, row, snapshot -> registers
+    auto fetch_rows = [&](uint32_t tid, const uint32_t* ids, uint32_t na) {
+        const uint32_t sr = tid >> 4, sc = tid & 15;
         pf_na = na;
-        pf_id = sr < na ? id : kNoId;
+        pf_id = sr < na ? ids[sr] : kNoId;
         if (sc == 0 && sr < na) {
             pf_bnd = a.bound[pf_id];
             pf_nrm = a.norms[pf_id];
@@ -486,40 +458,113 @@ void k_non_reverse(NonRevArgs a) {
             pf_snap[u] = (sr < na && j < k) ? sp[j] : 0u;
         }
     };
-    auto fetch = [&](uint32_t c_off, uint32_t na) { fetch_id(sr < na ? s_rsid[c_off + sr] : 0u, na); };
+    // S from its snapshot slots and norms (s_nsq, s_nbn): ids and norms with the new entries first
+    // (wave 0; slots lane, lane + 64; k <= 128), s_cnt[2] = its new-entry count; the hash cleared
+    auto prep_order = [&](uint32_t tid) {
+        const uint32_t lane = tid & 63, wave = tid >> 6;
+        for (uint32_t t = tid; t < kNrHash; t += kNrThreads) s_hkey[t] = kNoId;
+        if (tid < kNrBuckets) s_hcnt[tid] = 0;
+        if (wave == 0) {
+            const uint32_t v0 = lane < k ? s_nsq[lane] : 0u, v1 = lane + 64 < k ? s_nsq[lane + 64] : 0u;
+            const float bn0 = lane < k ? s_nbn[lane] : 0.f, bn1 = lane + 64 < k ? s_nbn[lane + 64] : 0.f;
+            const bool n0 = lane < k && (v0 & kNewBit), n1 = lane + 64 < k && (v1 & kNewBit);
+            const uint64_t m0 = __ballot(n0), m1 = __ballot(n1);
+            const uint64_t below = (1ull << lane) - 1ull;
+            const uint32_t nnew = __popcll(m0) + __popcll(m1);
+            const uint32_t b0 = __popcll(m0 & below), b1 = __popcll(m0) + __popcll(m1 & below);
+            if (lane < k) {
+                const uint32_t pos = n0 ? b0 : nnew + lane - b0, id = v0 & kIdMask;
+                s_bid[pos] = id;
+                s_bnrm[pos] = bn0;
+            }
+            if (lane + 64 < k) {
+                const uint32_t pos = n1 ? b1 : nnew + lane + 64 - b1, id = v1 & kIdMask;
+                s_bid[pos] = id;
+                s_bnrm[pos] = bn1;
+            }
+            if (lane == 0) s_cnt[2] = nnew;
+        }
+        for (uint32_t t = k + tid; t < L.kb16; t += kNrThreads) {
+            s_bid[t] = kNoId;
+            s_bnrm[t] = 0.f;
+        }
+    };
+    // after prep_order and a barrier: S's ids hashed, S's rows in flight to LDS
+    auto prep_rows = [&](uint32_t tid) {
+        const uint32_t wave = tid >> 6;
+        // Id 0 may be the padding of an unfilled list rather than a member, so it is never hashed:
+        // offers of id 0 always go through (the list's id check settles them).
+        if (tid < k) {
+            const uint32_t id = s_bid[tid];
+            if (id != 0) {
+                const uint32_t b = nr_bucket(id);
+                const uint32_t slot = atomicAdd(&s_hcnt[b], 1u);
+                if (slot < kNrSlots) {
+                    s_hkey[b * kNrSlots + slot] = id;
+                    s_hval[b * kNrSlots + slot] = (uint8_t)tid;
+                }
+            }
+        }
+        // S's rows go from global memory straight into LDS (global_load_lds_dwordx4: lane i of a
+        // wave writes 16 bytes at the wave's base + 16 i), every load in flight at once and no
+        // registers held. The data is waited for (vmcnt) before the barrier that precedes its
+        // first read. Slots past a row's first ceil(d/4) float4 and rows past k are left as they
+        // are: the Gram reads dims below 4 ceil(d/4) only, a Gram column depends on its own row
+        // only and columns past k are never offered, and the exact distances read dims below d of
+        // rows below k.
+        const uint32_t tot = L.kb16 * ldp4;
+#pragma unroll
+        for (uint32_t u = 0; u < kNrBRows; ++u) {
+            const uint32_t t = tid + u * kNrThreads;
+            const uint32_t r = t / ldp4, q4 = t - r * ldp4;
+            if (t < tot && r < k && q4 < d4) {
+                __builtin_amdgcn_global_load_lds(
+                    a.X + (size_t)s_bid[r] * ld + 4 * q4,
+                    (__attribute__((address_space(3))) void*)(s_brow_lds + 4 * (u * kNrThreads + wave * 64)),
+                    16, 0, 0);
+            }
+        }
+    };
 
-    // ---- S(q): ids, norms, hash table id -> slot, rows ---------------------------------------
-    for (uint32_t t = tid; t < kNrHash; t += kNrThreads) s_hkey[t] = kNoId;
-    if (tid < kNrBuckets) s_hcnt[tid] = 0;
-    if (tid < 2) s_cnt[tid] = 0;
-    if (wave == 0) {  // S(q) with its new entries first (slots lane, lane + 64; k <= 128)
-        const uint32_t v0 = lane < k ? s_nsq[lane] : 0u, v1 = lane + 64 < k ? s_nsq[lane + 64] : 0u;
-        const float bn0 = lane < k ? s_nbn[lane] : 0.f, bn1 = lane + 64 < k ? s_nbn[lane + 64] : 0.f;
-        const bool n0 = lane < k && (v0 & kNewBit), n1 = lane + 64 < k && (v1 & kNewBit);
-        const uint64_t m0 = __ballot(n0), m1 = __ballot(n1);
-        const uint64_t below = (1ull << lane) - 1ull;
-        const uint32_t nnew = __popcll(m0) + __popcll(m1);
-        const uint32_t b0 = __popcll(m0 & below), b1 = __popcll(m0) + __popcll(m1 & below);
-        if (lane < k) {
-            const uint32_t pos = n0 ? b0 : nnew + lane - b0, id = v0 & kIdMask;
-            s_bid[pos] = id;
-            s_bnrm[pos] = bn0;
-        }
-        if (lane + 64 < k) {
-            const uint32_t pos = n1 ? b1 : nnew + lane + 64 - b1, id = v1 & kIdMask;
-            s_bid[pos] = id;
-            s_bnrm[pos] = bn1;
-        }
-        if (lane == 0) s_cnt[2] = nnew;
+    // the first q: its metadata straight into LDS, then prepared like every later one
+    uint32_t q = a.order[(size_t)idx * a.order_stride];
+    uint64_t a0 = a.roff[q], a1 = a.roff[q + 1];
+    for (uint32_t t = tid0; t < k; t += kNrThreads) {
+        const uint32_t v = a.snap[(size_t)q * k + t];
+        s_nsq[t] = v;
+        s_nbn[t] = a.norms[v & kIdMask];
     }
-    for (uint32_t t = k + tid; t < L.kb16; t += kNrThreads) {
-        s_bid[t] = kNoId;
-        s_bnrm[t] = 0.f;
-    }
-    uint32_t g0 = 0;  // first reverse entry of the staged ids group
+    for (uint32_t t = tid0; t < min(a1 - a0, (uint64_t)kNrIds); t += kNrThreads) s_rsbuf0[t] = a.rs[a0 + t];
+    uint32_t nq_l = 0;  // the next q's position (lane 0 of each wave)
+    if ((tid0 & 63) == 0 && idx + nslot < i_end) nq_l = a.order[(size_t)(idx + nslot) * a.order_stride];
     __syncthreads();
-    // next q, step 2 (wave 0 has read s_nsq and s_nbn): reverse-list offsets (lane 0 of each wave)
-    // and its snapshot slots (wave 0, into s_nsq)
+    prep_order(tid0);
+    __syncthreads();
+    if (a1 > a0) {
+        prep_rows(tid0);
+        if (early_fetch) fetch_rows(tid0, s_rsbuf0, (uint32_t)min(a1 - a0, (uint64_t)kNrChunk));
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    uint32_t par = 0;
+
+    for (;;) {
+    // Per-thread indices are re-derived each q from an opaque copy of the thread id: hoisted out
+    // of this loop they would stay live across it and spill, and a spill reload waits for every
+    // load in flight, prefetches included.
+    uint32_t tid = tid0;
+    asm volatile("" : "+v"(tid));
+    const uint32_t lane = tid & 63, wave = tid >> 6;
+    uint16_t* s_queue = reinterpret_cast<uint16_t*>(lds + L.off_queue) + wave * kNrQueue;
+    // commit map: 16 threads per chunk row
+    const uint32_t sr = tid >> 4, sc = tid & 15;
+    uint32_t* s_rsid = par ? s_rsbuf1 : s_rsbuf0;  // RS(q)'s first kNrIds entries, staged
+    uint32_t* s_nrs = par ? s_rsbuf0 : s_rsbuf1;   // the next q's
+    const uint32_t nidx = idx + nslot;
+    const bool has_next = nidx < i_end;
+    const uint32_t nrs = (uint32_t)(a1 - a0);
+    const uint32_t nn = a.rnew ? min(a.rnew[q], nrs) : nrs;  // RS(q)'s new rows come first
+    // next q: reverse-list offsets (lane 0 of each wave) and snapshot slots (wave 0, into s_nsq)
     uint32_t nq = 0;
     uint64_t nr0 = 0, nr1 = 0;
     if (has_next) {
@@ -534,7 +579,9 @@ void k_non_reverse(NonRevArgs a) {
             if (lane + 64 < k) __builtin_amdgcn_global_load_lds(sq + 64 + lane, (kLdsU32*)(s_nsq + 64), 4, 0, 0);
         }
     }
-    // next q, step 3 (after two barriers): its first reverse ids (into s_nrs) and its S(q)'s norms
+    auto fetch = [&](uint32_t c_off, uint32_t na) { fetch_rows(tid, s_rsid + c_off, na); };
+    // next q: its first reverse ids (into s_nrs) and its S's norms, once its offsets and snapshot
+    // slots have landed
     auto prefetch_rs = [&]() {
         if (!has_next) return;
         __builtin_amdgcn_s_waitcnt(0);  // the offsets, and s_nsq for wave 0
@@ -556,41 +603,12 @@ void k_non_reverse(NonRevArgs a) {
     // old rows pair with S(q)'s new entries only; with none of either there is nothing to offer
     const uint32_t ntile_new = (s_cnt[2] + 15) / 16;
     const uint32_t ne = ntile_new ? nrs : nn;
+    if (tid < 2) s_cnt[tid] = 0;
+    uint32_t g0 = 0;  // first reverse entry of the staged ids group
+    uint32_t n_scr = 0, nsurv = 0;
     if (ne > 0) {
     // chunks never straddle the new/old boundary
     auto cend = [&](uint32_t c) { return c < nn ? min(c + kNrChunk, nn) : min(c + kNrChunk, ne); };
-    // Id 0 may be the padding of an unfilled list rather than a member, so it is never hashed:
-    // offers of id 0 always go through (the list's id check settles them).
-    if (tid < k) {
-        const uint32_t id = s_bid[tid];
-        if (id != 0) {
-            const uint32_t b = nr_bucket(id);
-            const uint32_t slot = atomicAdd(&s_hcnt[b], 1u);
-            if (slot < kNrSlots) {
-                s_hkey[b * kNrSlots + slot] = id;
-                s_hval[b * kNrSlots + slot] = (uint8_t)tid;
-            }
-        }
-    }
-    {  // S(q)'s rows go from global memory straight into LDS (global_load_lds_dwordx4: lane i of a
-       // wave writes 16 bytes at the wave's base + 16 i), every load in flight at once and no
-       // registers held; slots past a row's dims or past k are zeroed with ordinary stores. The
-       // data is waited for (vmcnt) before the barrier that precedes its first read.
-        const uint32_t tot = L.kb16 * ldp4;
-#pragma unroll
-        for (uint32_t u = 0; u < kNrBRows; ++u) {
-            const uint32_t t = tid + u * kNrThreads;
-            const uint32_t r = t / ldp4, q4 = t - r * ldp4;
-            if (t < tot && r < k && q4 < d4) {
-                __builtin_amdgcn_global_load_lds(
-                    a.X + (size_t)s_bid[r] * ld + 4 * q4,
-                    (__attribute__((address_space(3))) void*)(s_brow_lds + 4 * (u * kNrThreads + wave * 64)),
-                    16, 0, 0);
-            } else if (t < tot) {
-                reinterpret_cast<float4*>(s_brow)[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        }
-    }
 
     // ---- chunk commit: registers -> LDS + membership ---------------------------------------
     // needs the mask words zeroed (by the caller, one barrier earlier)
@@ -630,20 +648,17 @@ void k_non_reverse(NonRevArgs a) {
             }
         }
     };
-    fetch(0, cend(0));
+    if (!early_fetch) fetch(0, cend(0));
     for (uint32_t t = tid; t < kNrChunk * 4; t += kNrThreads) s_mask[t] = 0;
     __syncthreads();
-    commit();
-    __builtin_amdgcn_s_waitcnt(0);  // S(q)'s rows have landed in LDS
+    commit();  // S(q)'s rows and hash were made ready before the loop head's barrier
     __syncthreads();
-    prefetch_rs();
 
     // |approx - exact| <= (K + 8) 2^-23 (n_p + n_r) for the FP32 Gram over K = 4 ceil(d/4) terms
     const float margin_c = (float)(4 * d4 + 8) * (1.0f / 8388608.0f);
     const uint32_t ksteps = d4;
     const uint32_t ntile = L.kb16 / 16;
     const uint32_t lrow = lane & 15, lk = lane >> 4;
-    uint32_t n_scr = 0;
 
     for (uint32_t c0 = 0; c0 < ne;) {
         const uint32_t c1 = cend(c0);
@@ -756,6 +771,7 @@ void k_non_reverse(NonRevArgs a) {
             }
         }
         if (qn) flush();
+        if (c0 == 0) prefetch_rs();  // the next q's offsets and snapshot slots have had a chunk to land
         __syncthreads();
         if (n0 < ne) {
             for (uint32_t t = tid; t < kNrChunk * 4; t += kNrThreads) s_mask[t] = 0;
@@ -773,29 +789,55 @@ void k_non_reverse(NonRevArgs a) {
         c0 = c1;
     }
     // the chunk loop ends with a barrier: the list is complete
-    const uint32_t nsurv = s_cnt[0];
-    for (uint32_t t = tid; t < min(nsurv, kNrSurv); t += kNrThreads) {
-        const uint32_t pid = s_spid[t];
-        const uint32_t slot = atomicAdd(&a.ncnt[pid], 1u);
-        if (slot < a.cap) {
-            a.nbuf_d[(size_t)pid * a.cap + slot] = s_sd[t];
-            a.nbuf_id[(size_t)pid * a.cap + slot] = s_srid[t];
-        }
-    }
-    if (a.counters) {  // diagnostics only: two same-address atomics per workgroup
-        if (lane == 0 && n_scr) atomicAdd(&s_cnt[1], n_scr);
-        __syncthreads();
-        if (tid == 0) {
-            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)s_cnt[1]);
-            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[1]), (unsigned long long)nsurv);
-        }
-    }
+    nsurv = s_cnt[0];
     } else {
         prefetch_rs();
     }
-    __builtin_amdgcn_s_waitcnt(0);  // the next q's metadata has landed in LDS
+    // q's accepted offers go to the candidate buffers
+    auto append = [&]() {
+        for (uint32_t t = tid; t < min(nsurv, kNrSurv); t += kNrThreads) {
+            const uint32_t pid = s_spid[t];
+            const uint32_t slot = atomicAdd(&a.ncnt[pid], 1u);
+            if (slot < a.cap) {
+                a.nbuf_d[(size_t)pid * a.cap + slot] = s_sd[t];
+                a.nbuf_id[(size_t)pid * a.cap + slot] = s_srid[t];
+            }
+        }
+        if (a.counters) {  // diagnostics only: two same-address atomics per workgroup
+            if (lane == 0 && n_scr) atomicAdd(&s_cnt[1], n_scr);
+            __syncthreads();
+            if (tid == 0) {
+                atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)s_cnt[1]);
+                atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[1]), (unsigned long long)nsurv);
+            }
+        }
+    };
+    // this q's chunk registers are dead from here on: say so, so that they are not carried
+    // around the loop on the paths that do not fetch the next q's first chunk
+#pragma unroll
+    for (uint32_t u = 0; u < kNrPfRows; ++u) pf_row[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (uint32_t u = 0; u < kNrPfSnap; ++u) pf_snap[u] = 0u;
+    pf_id = kNoId;
+    pf_na = 0;
+    pf_bnd = pf_nrm = 0.f;
+    if (!has_next) {
+        append();
+        break;
+    }
+    // The next q: S ordered and hashed, its rows in flight to LDS, its first chunk fetched (always
+    // assigned, empty when there is none or it is fetched at the loop head, so that nothing of
+    // this q's registers stays live across the loop), and the position of the q after it.
+    __builtin_amdgcn_s_waitcnt(0);  // its snapshot slots, norms and reverse ids
+    prep_order(tid);
+    __syncthreads();
+    const uint32_t nrs1 = (uint32_t)(s_meta[2] - s_meta[1]);
+    if (nrs1 > 0) prep_rows(tid);
+    fetch_rows(tid, s_nrs, early_fetch ? min(nrs1, kNrChunk) : 0u);
+    if (lane == 0 && nidx + nslot < i_end) nq_l = a.order[(size_t)(nidx + nslot) * a.order_stride];
+    append();  // the atomics' round trips overlap the loads above
+    __builtin_amdgcn_s_waitcnt(0);  // the next q's rows, first chunk and position have landed
     __syncthreads();                // ... and the next q reuses the LDS
-    if (!has_next) break;
     idx = nidx;
     q = (uint32_t)s_meta[0];
     a0 = s_meta[1];
