@@ -191,16 +191,17 @@ __global__ void k_scan_total(const uint32_t* __restrict__ in, uint32_t n, uint64
 // ---- reverse screening kernel --------------------------------------------------------------------
 // One workgroup per q. FP32 rows of S[q] and 32-row chunks of RS[q] sit in LDS, each loaded as its
 // first ceil(d/4) float4 (the dims beyond, up to that bound, are zero in every buffer the builder
-// binds) and zero-filled to the LDS stride; an FP32 MFMA Gram (16x16x4) over ceil(d/4) steps
+// binds); an FP32 MFMA Gram (16x16x4) over ceil(d/4) steps
 // screens every pair with a rigorous margin; pairs with r already in S[p] are masked by
-// a per-chunk membership bitmap (S(q)'s ids sit in a bucketed hash table: one lookup is two
-// 16-byte LDS reads, with no probe loop for the lanes of a wave to diverge on); the survivors queue up per wave and are re-ranked exactly, eight
+// a per-chunk membership bitmap (S(q)'s ids sit in a bucketed hash table: one lookup is one
+// 16-byte LDS read and a few integer operations, with no probe loop for the lanes of a wave to
+// diverge on); the survivors queue up per wave and are re-ranked exactly, eight
 // lanes per survivor (lane j owns the reference's accumulator j, then the lanes are summed in
 // order). The next chunk's rows and snapshots are fetched into registers while the current chunk
 // is screened. Accepted offers collect in an LDS list and go to the candidate buffers once, after
 // the last chunk: an atomic that returns a value waits for every load issued before it, so
 // appending inside the chunk loop would stall on the next chunk's fetch.
-constexpr uint32_t kNrThreads = 512;  // 8 waves: 2 strips of the chunk x 4 groups of S[q]'s tiles
+constexpr uint32_t kNrThreads = 512;  // 8 waves, dealt (16-row strip, 16-column tile) jobs in turn
 constexpr uint32_t kNrChunk = 32;     // reverse-neighbour rows per chunk
 constexpr uint32_t kNrMaxK4 = 32;     // ld / 4 for ld <= 128 (larger ld takes the direct kernel)
 constexpr uint32_t kNrIds = 512;      // reverse-neighbour ids staged at a time
@@ -218,16 +219,16 @@ __host__ __device__ inline uint32_t nr_row_stride(uint32_t ld) { return ld % 8 =
 
 struct NrLds {
     uint32_t kb16, ldp;
-    uint32_t off_bid, off_bnrm, off_brow, off_hkey, off_hval, off_hcnt, off_rsid, off_rsid2, off_nsq, off_nbn,
+    uint32_t off_bid, off_bnrm, off_brow, off_hkey, off_hcnt, off_rsid, off_rsid2, off_nsq, off_nbn,
         off_meta;
     uint32_t off_aid, off_abnd, off_anrm, off_arow, off_mask, off_queue;
     uint32_t off_spid, off_srid, off_sd, off_cnt, total;
 };
 
-// S(q)'s ids (<= 128) hashed into 64 buckets of 8 slots. A key whose bucket is full is left out:
+// S(q)'s ids (<= 128) hashed into 128 buckets of 4 slots. A key whose bucket is full is left out:
 // membership then reads "absent", the offer goes through, and the list's id check rejects it.
-constexpr uint32_t kNrBuckets = 64;
-constexpr uint32_t kNrSlots = 8;
+constexpr uint32_t kNrBuckets = 128;
+constexpr uint32_t kNrSlots = 4;
 constexpr uint32_t kNrHash = kNrBuckets * kNrSlots;
 // S(q) rows staged per thread (k <= 128 rows of <= 33 float4)
 constexpr uint32_t kNrBRows = (128 * (kNrMaxK4 + 1) + kNrThreads - 1) / kNrThreads;
@@ -242,7 +243,6 @@ __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
     o = (o + 15) & ~15u;
     L.off_brow = o; o += L.kb16 * L.ldp * 4;
     L.off_hkey = o; o += kNrHash * 4;
-    L.off_hval = o; o += kNrHash;
     L.off_hcnt = o; o += kNrBuckets * 4;
     L.off_rsid = o; o += kNrIds * 4;
     L.off_rsid2 = o; o += kNrIds * 4;
@@ -267,7 +267,14 @@ __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
 
 size_t non_reverse_lds(uint32_t k, uint32_t ld) { return nr_layout(k, ld).total; }
 
-__device__ inline uint32_t nr_bucket(uint32_t id) { return (id * 0x9E3779B1u) >> 26; }
+// Membership hash of an id: h = id * odd constant is a bijection of 32-bit words, so the bucket
+// (h's top 7 bits) and h's low 25 bits identify the id. An entry holds (h << 7) | column (column
+// < 128); an empty slot holds 0xFFFFFFFF. For a query h, entry ^ (h << 7) is the column when the
+// entry is the query's, at least 128 for another id's entry and at least 127 for an empty slot, so
+// the minimum over the bucket's 4 slots, when below 127, is the column (a column of 127, k = 128
+// only, reads as absent, which is always safe). Id 0 is never inserted, and its query lands in
+// bucket 0 with h = 0, where no entry is below 128.
+__device__ inline uint32_t nr_hash(uint32_t id) { return id * 0x9E3779B1u; }
 
 // The 8 partial sums of a lane group (lanes 8g .. 8g+7) added in lane order into 0.0f, valid in
 // the group's first lane. A group never crosses a 16-lane DPP row, so lane 8g reads lane 8g + j
@@ -399,13 +406,14 @@ void k_non_reverse(NonRevArgs a) {
     if (idx >= i_end) return;
     const NrLds L = nr_layout(k, ld);
     const uint32_t ldp = L.ldp, d4 = (d + 3) / 4, ldp4 = ldp / 4;
+    // t / ldp4 = umulhi(t, ceil(2^32 / ldp4)) exactly for t < 2^32 / ldp4 (ldp4 > 1)
+    const uint32_t ldp4_inv = ldp4 > 1 ? (uint32_t)((0x100000000ull + ldp4 - 1) / ldp4) : 0u;
     uint32_t* s_bid = reinterpret_cast<uint32_t*>(lds + L.off_bid);
     float* s_bnrm = reinterpret_cast<float*>(lds + L.off_bnrm);
     float* s_brow = reinterpret_cast<float*>(lds + L.off_brow);
     __attribute__((address_space(3))) float* s_brow_lds =
         (__attribute__((address_space(3))) float*)(s_brow);
     uint32_t* s_hkey = reinterpret_cast<uint32_t*>(lds + L.off_hkey);
-    uint8_t* s_hval = lds + L.off_hval;
     uint32_t* s_hcnt = reinterpret_cast<uint32_t*>(lds + L.off_hcnt);
     // reverse ids of the current q and of the next one (the two swap every q)
     uint32_t* s_rsbuf0 = reinterpret_cast<uint32_t*>(lds + L.off_rsid);
@@ -497,12 +505,9 @@ void k_non_reverse(NonRevArgs a) {
         if (tid < k) {
             const uint32_t id = s_bid[tid];
             if (id != 0) {
-                const uint32_t b = nr_bucket(id);
+                const uint32_t h = nr_hash(id), b = h >> 25;
                 const uint32_t slot = atomicAdd(&s_hcnt[b], 1u);
-                if (slot < kNrSlots) {
-                    s_hkey[b * kNrSlots + slot] = id;
-                    s_hval[b * kNrSlots + slot] = (uint8_t)tid;
-                }
+                if (slot < kNrSlots) s_hkey[b * kNrSlots + slot] = (h << 7) | tid;
             }
         }
         // S's rows go from global memory straight into LDS (global_load_lds_dwordx4: lane i of a
@@ -516,7 +521,7 @@ void k_non_reverse(NonRevArgs a) {
 #pragma unroll
         for (uint32_t u = 0; u < kNrBRows; ++u) {
             const uint32_t t = tid + u * kNrThreads;
-            const uint32_t r = t / ldp4, q4 = t - r * ldp4;
+            const uint32_t r = ldp4 == 1 ? t : __umulhi(t, ldp4_inv), q4 = t - r * ldp4;
             if (t < tot && r < k && q4 < d4) {
                 __builtin_amdgcn_global_load_lds(
                     a.X + (size_t)s_bid[r] * ld + 4 * q4,
@@ -623,28 +628,21 @@ void k_non_reverse(NonRevArgs a) {
             const uint32_t q4 = sc + 16 * u;
             if (q4 < ldp4) reinterpret_cast<float4*>(s_arow)[sr * ldp4 + q4] = pf_row[u];
         }
-        // member[sr][j]: S(q)[j] is in S(p): the list's id check would reject it
+        // member[sr][j]: S(q)[j] is in S(p): the list's id check would reject it. Every bucket read
+        // is issued before the first is used.
         if (sr < pf_na) {
+            uint4 kb[kNrPfSnap];
+            uint32_t q7[kNrPfSnap];
 #pragma unroll
             for (uint32_t u = 0; u < kNrPfSnap; ++u) {
-                const uint32_t e = pf_snap[u] & kIdMask;
-                if (e == 0) continue;
-                const uint32_t b = nr_bucket(e);
-                const uint4 k0 = reinterpret_cast<const uint4*>(s_hkey)[2 * b];
-                const uint4 k1 = reinterpret_cast<const uint4*>(s_hkey)[2 * b + 1];
-                int f = -1;
-                f = k0.x == e ? 0 : f;
-                f = k0.y == e ? 1 : f;
-                f = k0.z == e ? 2 : f;
-                f = k0.w == e ? 3 : f;
-                f = k1.x == e ? 4 : f;
-                f = k1.y == e ? 5 : f;
-                f = k1.z == e ? 6 : f;
-                f = k1.w == e ? 7 : f;
-                if (f >= 0) {
-                    const uint32_t j = s_hval[b * kNrSlots + f];
-                    atomicOr(&s_mask[sr * 4 + (j >> 5)], 1u << (j & 31));
-                }
+                const uint32_t h = nr_hash(pf_snap[u] & kIdMask);
+                q7[u] = h << 7;
+                kb[u] = reinterpret_cast<const uint4*>(s_hkey)[h >> 25];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kNrPfSnap; ++u) {
+                const uint32_t m = min(min(kb[u].x ^ q7[u], kb[u].y ^ q7[u]), min(kb[u].z ^ q7[u], kb[u].w ^ q7[u]));
+                if (m < 127u) atomicOr(&s_mask[sr * 4 + (m >> 5)], 1u << (m & 31));
             }
         }
     };
