@@ -363,13 +363,14 @@ __device__ inline float dist_ref_fixed(const float* __restrict__ a, const float*
     return s;
 }
 
-// FP32 Gram of a 16x16 tile over K = 4 KS dims, four independent accumulation chains (the
-// screen's error bound holds for any summation order). KS is a compile-time constant for the
-// common widths; KS = 0 is the generic runtime loop.
+// FP32 Gram of a 16x16 tile over K = 4 KS dims, one accumulation chain (an MFMA that takes the
+// previous one's result as its accumulator issues back to back; the screen's error bound holds for
+// any summation order). KS is a compile-time constant for the common widths; KS = 0 is the generic
+// runtime loop.
 template <uint32_t KS>
 __device__ inline float4_t gram_tile_f32(const float* __restrict__ arow,
                                          const float* __restrict__ brow, uint32_t ksteps) {
-    float4_t c0 = {0.f, 0.f, 0.f, 0.f}, c1 = c0, c2 = c0, c3 = c0;
+    float4_t c0 = {0.f, 0.f, 0.f, 0.f};
     if constexpr (KS > 0) {
         float av[KS], bv[KS];
 #pragma unroll
@@ -378,17 +379,12 @@ __device__ inline float4_t gram_tile_f32(const float* __restrict__ arow,
             bv[kk] = brow[4 * kk];
         }
 #pragma unroll
-        for (uint32_t kk = 0; kk < KS; ++kk) {
-            if (kk % 4 == 0) c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], bv[kk], c0, 0, 0, 0);
-            if (kk % 4 == 1) c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], bv[kk], c1, 0, 0, 0);
-            if (kk % 4 == 2) c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], bv[kk], c2, 0, 0, 0);
-            if (kk % 4 == 3) c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], bv[kk], c3, 0, 0, 0);
-        }
+        for (uint32_t kk = 0; kk < KS; ++kk) c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], bv[kk], c0, 0, 0, 0);
     } else {
         for (uint32_t kk = 0; kk < ksteps; ++kk)
             c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * kk], brow[4 * kk], c0, 0, 0, 0);
     }
-    return (c0 + c1) + (c2 + c3);
+    return c0;
 }
 
 template <uint32_t KS>
@@ -733,9 +729,11 @@ void k_non_reverse(NonRevArgs a) {
             const uint32_t strip = nstrip == 2 ? (wv & 1u) : 0u;
             float rn[4], thr0[4];
             uint32_t rr[4], pid[4];
+            bool rok[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 rr[i] = 16 * strip + (lane >> 4) * 4 + i;
+                rok[i] = rr[i] < na;
                 rn[i] = s_anrm[rr[i]];
                 pid[i] = s_aid[rr[i]];
                 thr0[i] = s_abnd[rr[i]] * (1.0f + 1.0f / 65536.0f);
@@ -745,19 +743,24 @@ void k_non_reverse(NonRevArgs a) {
                 const uint32_t t = nstrip == 2 ? j >> 1 : j;
                 if (qn + 256 > kNrQueue) flush();
                 const float* brow = s_brow + (16 * t + lrow) * ldp + lk;
+                // membership words of this tile's columns (c >> 5 = t >> 1 for the whole tile)
+                uint32_t mw[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) mw[i] = s_mask[rr[i] * 4 + (t >> 1)];
                 const float4_t acc = gram_tile_f32<KS>(arow, brow, ksteps);
-                const uint32_t c = 16 * t + lrow;
-                const float cn = c < k ? s_bnrm[c] : 0.f;
+                const uint32_t c = 16 * t + lrow, cb = c & 31;
+                const bool cok = c < k;
+                const float cn = s_bnrm[c];  // zero past k
                 const uint32_t rid = s_bid[c];
                 // compact the survivors of this tile into the wave's queue as (row << 7 | col)
                 const uint32_t qn_before = qn;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float nsum = rn[i] + cn;
-                    const float approx = nsum - 2.0f * acc[i];
+                    const float approx = __builtin_fmaf(-2.0f, acc[i], nsum);  // 2 acc is exact
                     // <=: a distance equal to a zero bound can still win on its id
-                    bool ok = rr[i] < na && c < k && rid != pid[i] && approx <= thr0[i] + margin_c * nsum;
-                    if (ok) ok = !((s_mask[rr[i] * 4 + (c >> 5)] >> (c & 31)) & 1u);
+                    const bool ok = cok && rok[i] && rid != pid[i] && !((mw[i] >> cb) & 1u) &&
+                                    approx <= __builtin_fmaf(margin_c, nsum, thr0[i]);
                     const uint64_t bal = __ballot(ok);
                     if (ok) {
                         const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
