@@ -217,14 +217,6 @@ static_assert(kNrChunk == 32 && (kNrThreads / 64) % 2 == 0,
 
 __host__ __device__ inline uint32_t nr_row_stride(uint32_t ld) { return ld % 8 == 0 ? ld + 4 : ld; }
 
-struct NrLds {
-    uint32_t kb16, ldp;
-    uint32_t off_bid, off_bnrm, off_brow, off_hkey, off_hcnt, off_rsid, off_rsid2, off_nsq, off_nbn,
-        off_meta;
-    uint32_t off_aid, off_abnd, off_anrm, off_arow, off_mask, off_queue;
-    uint32_t off_spid, off_srid, off_sd, off_cnt, total;
-};
-
 // S(q)'s ids (<= 128) hashed into 128 buckets of 4 slots. A key whose bucket is full is left out:
 // membership then reads "absent", the offer goes through, and the list's id check rejects it.
 constexpr uint32_t kNrBuckets = 128;
@@ -233,34 +225,42 @@ constexpr uint32_t kNrHash = kNrBuckets * kNrSlots;
 // S(q) rows staged per thread (k <= 128 rows of <= 33 float4)
 constexpr uint32_t kNrBRows = (128 * (kNrMaxK4 + 1) + kNrThreads - 1) / kNrThreads;
 
+// LDS: the fixed-size arrays first, at compile-time offsets (they fold into the LDS instructions'
+// immediate offsets and take no scalar registers), then the arrays sized by k and ld.
+constexpr uint32_t kNrOffHkey = 0;                                   // hash entries (16-byte buckets)
+constexpr uint32_t kNrOffHcnt = kNrOffHkey + kNrHash * 4;            // bucket fill counts
+constexpr uint32_t kNrOffRsid = kNrOffHcnt + kNrBuckets * 4;         // reverse ids, current / next q
+constexpr uint32_t kNrOffRsid2 = kNrOffRsid + kNrIds * 4;
+constexpr uint32_t kNrOffNsq = kNrOffRsid2 + kNrIds * 4;             // next q's snapshot slots
+constexpr uint32_t kNrOffNbn = kNrOffNsq + 128 * 4;                  // ... and their norms
+constexpr uint32_t kNrOffMeta = kNrOffNbn + 128 * 4;                 // next q: q, roff[q], roff[q+1]
+constexpr uint32_t kNrOffAid = kNrOffMeta + 32;                      // chunk rows: ids, bounds, norms
+constexpr uint32_t kNrOffAbnd = kNrOffAid + kNrChunk * 4;
+constexpr uint32_t kNrOffAnrm = kNrOffAbnd + kNrChunk * 4;
+constexpr uint32_t kNrOffMask = kNrOffAnrm + kNrChunk * 4;           // membership words, 4 per row
+constexpr uint32_t kNrOffQueue = kNrOffMask + kNrChunk * 4 * 4;      // per-wave survivor queues
+constexpr uint32_t kNrOffSpid = (kNrOffQueue + (kNrThreads / 64) * kNrQueue * 2 + 15) & ~15u;
+constexpr uint32_t kNrOffSrid = kNrOffSpid + kNrSurv * 4;            // accepted offers
+constexpr uint32_t kNrOffSd = kNrOffSrid + kNrSurv * 4;
+constexpr uint32_t kNrOffCnt = kNrOffSd + kNrSurv * 4;               // [0] accepted, [1] screened, [2] new
+constexpr uint32_t kNrFixed = (kNrOffCnt + 16 + 15) & ~15u;
+static_assert(kNrOffMeta % 8 == 0, "s_meta holds 64-bit words");
+
+struct NrLds {
+    uint32_t kb16, ldp;
+    uint32_t off_bid, off_bnrm, off_brow, off_arow, total;
+};
+
 __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
     NrLds L;
     L.kb16 = (k + 15) & ~15u;
     L.ldp = nr_row_stride(ld);
-    uint32_t o = 0;
+    uint32_t o = kNrFixed;
     L.off_bid = o;  o += L.kb16 * 4;
     L.off_bnrm = o; o += L.kb16 * 4;
     o = (o + 15) & ~15u;
     L.off_brow = o; o += L.kb16 * L.ldp * 4;
-    L.off_hkey = o; o += kNrHash * 4;
-    L.off_hcnt = o; o += kNrBuckets * 4;
-    L.off_rsid = o; o += kNrIds * 4;
-    L.off_rsid2 = o; o += kNrIds * 4;
-    L.off_nsq = o;  o += 128 * 4;
-    L.off_nbn = o;  o += 128 * 4;
-    L.off_meta = o; o += 32;
-    L.off_aid = o;  o += kNrChunk * 4;
-    L.off_abnd = o; o += kNrChunk * 4;
-    L.off_anrm = o; o += kNrChunk * 4;
-    o = (o + 15) & ~15u;
     L.off_arow = o; o += kNrChunk * L.ldp * 4;
-    L.off_mask = o;  o += kNrChunk * 4 * 4;
-    L.off_queue = o; o += (kNrThreads / 64) * kNrQueue * 2;
-    o = (o + 15) & ~15u;
-    L.off_spid = o; o += kNrSurv * 4;
-    L.off_srid = o; o += kNrSurv * 4;
-    L.off_sd = o;   o += kNrSurv * 4;
-    L.off_cnt = o;  o += 16;
     L.total = (o + 15) & ~15u;
     return L;
 }
@@ -409,23 +409,23 @@ void k_non_reverse(NonRevArgs a) {
     float* s_brow = reinterpret_cast<float*>(lds + L.off_brow);
     __attribute__((address_space(3))) float* s_brow_lds =
         (__attribute__((address_space(3))) float*)(s_brow);
-    uint32_t* s_hkey = reinterpret_cast<uint32_t*>(lds + L.off_hkey);
-    uint32_t* s_hcnt = reinterpret_cast<uint32_t*>(lds + L.off_hcnt);
+    uint32_t* s_hkey = reinterpret_cast<uint32_t*>(lds + kNrOffHkey);
+    uint32_t* s_hcnt = reinterpret_cast<uint32_t*>(lds + kNrOffHcnt);
     // reverse ids of the current q and of the next one (the two swap every q)
-    uint32_t* s_rsbuf0 = reinterpret_cast<uint32_t*>(lds + L.off_rsid);
-    uint32_t* s_rsbuf1 = reinterpret_cast<uint32_t*>(lds + L.off_rsid2);
-    uint32_t* s_nsq = reinterpret_cast<uint32_t*>(lds + L.off_nsq);  // next S(q): snapshot slots
-    float* s_nbn = reinterpret_cast<float*>(lds + L.off_nbn);        // ... and their norms
-    uint64_t* s_meta = reinterpret_cast<uint64_t*>(lds + L.off_meta);  // next q: q, roff[q], roff[q+1]
-    uint32_t* s_aid = reinterpret_cast<uint32_t*>(lds + L.off_aid);
-    float* s_abnd = reinterpret_cast<float*>(lds + L.off_abnd);
-    float* s_anrm = reinterpret_cast<float*>(lds + L.off_anrm);
+    uint32_t* s_rsbuf0 = reinterpret_cast<uint32_t*>(lds + kNrOffRsid);
+    uint32_t* s_rsbuf1 = reinterpret_cast<uint32_t*>(lds + kNrOffRsid2);
+    uint32_t* s_nsq = reinterpret_cast<uint32_t*>(lds + kNrOffNsq);  // next S(q): snapshot slots
+    float* s_nbn = reinterpret_cast<float*>(lds + kNrOffNbn);        // ... and their norms
+    uint64_t* s_meta = reinterpret_cast<uint64_t*>(lds + kNrOffMeta);  // next q: q, roff[q], roff[q+1]
+    uint32_t* s_aid = reinterpret_cast<uint32_t*>(lds + kNrOffAid);
+    float* s_abnd = reinterpret_cast<float*>(lds + kNrOffAbnd);
+    float* s_anrm = reinterpret_cast<float*>(lds + kNrOffAnrm);
     float* s_arow = reinterpret_cast<float*>(lds + L.off_arow);
-    uint32_t* s_mask = reinterpret_cast<uint32_t*>(lds + L.off_mask);
-    uint32_t* s_spid = reinterpret_cast<uint32_t*>(lds + L.off_spid);
-    uint32_t* s_srid = reinterpret_cast<uint32_t*>(lds + L.off_srid);
-    float* s_sd = reinterpret_cast<float*>(lds + L.off_sd);
-    uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds + L.off_cnt);  // [0] accepted, [1] screened
+    uint32_t* s_mask = reinterpret_cast<uint32_t*>(lds + kNrOffMask);
+    uint32_t* s_spid = reinterpret_cast<uint32_t*>(lds + kNrOffSpid);
+    uint32_t* s_srid = reinterpret_cast<uint32_t*>(lds + kNrOffSrid);
+    float* s_sd = reinterpret_cast<float*>(lds + kNrOffSd);
+    uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds + kNrOffCnt);  // [0] accepted, [1] screened
     const uint32_t tid0 = threadIdx.x;
 
 
@@ -556,7 +556,7 @@ void k_non_reverse(NonRevArgs a) {
     uint32_t tid = tid0;
     asm volatile("" : "+v"(tid));
     const uint32_t lane = tid & 63, wave = tid >> 6;
-    uint16_t* s_queue = reinterpret_cast<uint16_t*>(lds + L.off_queue) + wave * kNrQueue;
+    uint16_t* s_queue = reinterpret_cast<uint16_t*>(lds + kNrOffQueue) + wave * kNrQueue;
     // commit map: 16 threads per chunk row
     const uint32_t sr = tid >> 4, sc = tid & 15;
     uint32_t* s_rsid = par ? s_rsbuf1 : s_rsbuf0;  // RS(q)'s first kNrIds entries, staged
