@@ -212,6 +212,7 @@ constexpr uint32_t kNrSurv = 512;     // accepted offers held in LDS per workgro
 typedef __attribute__((address_space(3))) uint32_t kLdsU32;
 constexpr uint32_t kNrOversub = 4;    // persistent workgroups per resident slot
 static_assert(kNrIds == kNrThreads, "a q's first reverse ids are staged one per thread");
+static_assert(kNrThreads / 16 == kNrChunk, "16 threads commit each chunk row (one wave holds a row)");
 static_assert(kNrChunk == 32 && (kNrThreads / 64) % 2 == 0,
               "a chunk is one or two 16-row strips, and a wave keeps its strip from job to job");
 
@@ -612,8 +613,10 @@ void k_non_reverse(NonRevArgs a) {
     auto cend = [&](uint32_t c) { return c < nn ? min(c + kNrChunk, nn) : min(c + kNrChunk, ne); };
 
     // ---- chunk commit: registers -> LDS + membership ---------------------------------------
-    // needs the mask words zeroed (by the caller, one barrier earlier)
+    // A row's 16 threads are one wave's lanes: they zero the row's membership words and then set
+    // bits in them, in the wave's LDS order, with no barrier between.
     auto commit = [&]() {
+        if (sc < 4) s_mask[sr * 4 + sc] = 0;
         if (sc == 0) {
             s_aid[sr] = pf_id;
             s_abnd[sr] = sr < pf_na ? pf_bnd : 0.f;
@@ -643,8 +646,6 @@ void k_non_reverse(NonRevArgs a) {
         }
     };
     if (!early_fetch) fetch(0, cend(0));
-    for (uint32_t t = tid; t < kNrChunk * 4; t += kNrThreads) s_mask[t] = 0;
-    __syncthreads();
     commit();  // S(q)'s rows and hash were made ready before the loop head's barrier
     __syncthreads();
 
@@ -775,7 +776,6 @@ void k_non_reverse(NonRevArgs a) {
         if (c0 == 0) prefetch_rs();  // the next q's offsets and snapshot slots have had a chunk to land
         __syncthreads();
         if (n0 < ne) {
-            for (uint32_t t = tid; t < kNrChunk * 4; t += kNrThreads) s_mask[t] = 0;
             if (!next_in_group) {  // next ids group, then a direct fetch
                 g0 = n0;
                 for (uint32_t t = tid; t < min(nrs - g0, kNrIds); t += kNrThreads)
@@ -783,7 +783,6 @@ void k_non_reverse(NonRevArgs a) {
                 __syncthreads();
                 fetch(0, n1 - n0);
             }
-            __syncthreads();
             commit();
             __syncthreads();
         }
