@@ -18,8 +18,9 @@
 
 namespace knng {
 
-constexpr uint32_t kLeafThreads = 512;
-constexpr uint32_t kLeafWaves = kLeafThreads / 64;
+// Waves per workgroup: 8, or 16 for buckets above 128 points whose LDS (one workgroup per CU
+// there) still fits, so that twice the waves hide the list and row loads of phase 2.
+constexpr uint32_t kLeafWavesSmall = 8, kLeafWavesLarge = 16;
 
 
 
@@ -30,7 +31,8 @@ struct LeafLds {
     uint32_t part_rows;  // FP32 rows that fit the row region at once (phase 2 re-rank)
 };
 
-__host__ __device__ inline LeafLds leaf_lds_layout(uint32_t max_leaf, uint32_t d, uint32_t ld) {
+__host__ __device__ inline LeafLds leaf_lds_layout(uint32_t max_leaf, uint32_t d, uint32_t ld,
+                                                  uint32_t waves = kLeafWavesSmall) {
     LeafLds L;
     L.b16 = (max_leaf + 15) & ~15u;
     L.kp = (d + 15) & ~15u;
@@ -46,24 +48,25 @@ __host__ __device__ inline LeafLds leaf_lds_layout(uint32_t max_leaf, uint32_t d
     L.off_rows = o;   o += L.b16 * L.rs * 2;
     o = (o + 15) & ~15u;
     L.part_rows = (L.b16 * L.rs * 2) / (ld * 4);
-    L.off_rowbuf = o; o += kLeafWaves * ld * 4;
-    L.off_cand = o;   o += kLeafWaves * L.b16 * 2;
+    L.off_rowbuf = o; o += waves * ld * 4;
+    L.off_cand = o;   o += waves * L.b16 * 2;
     o = (o + 15) & ~15u;
-    L.off_scr = o;    o += kLeafWaves * 128 * 8;  // per-wave list scratch of the batched offer
+    L.off_scr = o;    o += waves * 128 * 8;  // per-wave list scratch of the batched offer
     L.total = (o + 15) & ~15u;
     return L;
 }
 
 // KS = (d rounded up to 16) / 16 MFMA k-steps, a compile-time constant so the k-loop unrolls
 // without guards and its LDS loads pipeline.
-template <uint32_t HALF, uint32_t KS>
-__global__ __launch_bounds__(kLeafThreads) void k_leaf(LeafArgs a) {
+template <uint32_t HALF, uint32_t KS, uint32_t W>
+__global__ __launch_bounds__(W * 64) void k_leaf(LeafArgs a) {
+    constexpr uint32_t kLeafWaves = W, kLeafThreads = W * 64;
     extern __shared__ __align__(16) unsigned char lds[];
     const LeafDesc lf = a.leaves[blockIdx.x];
     const uint32_t b = lf.size;
     if (b < 2) return;
     const uint32_t d = a.d, ld = a.ld, k = a.k;
-    const LeafLds L = leaf_lds_layout(a.max_leaf, d, ld);
+    const LeafLds L = leaf_lds_layout(a.max_leaf, d, ld, W);
     const uint32_t b16 = (b + 15) & ~15u;
     const uint32_t nstrip = b16 / 16;
     uint32_t* s_ids = reinterpret_cast<uint32_t*>(lds + L.off_ids);
@@ -547,31 +550,38 @@ size_t leaf_lds_bytes(uint32_t max_leaf, uint32_t d, uint32_t ld) {
 }
 
 template <uint32_t HALF, uint32_t KS>
-static void launch_leaf_t(const LeafArgs& a, size_t lds, hipStream_t st) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf<HALF, KS>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    k_leaf<HALF, KS><<<a.nleaves, kLeafThreads, lds, st>>>(a);
+static void launch_leaf_t(const LeafArgs& a, hipStream_t st) {
+    const size_t lds8 = leaf_lds_bytes(a.max_leaf, a.d, a.ld);
+    const size_t lds16 = leaf_lds_layout(a.max_leaf, a.d, a.ld, kLeafWavesLarge).total;
+    if (a.max_leaf > 128 && lds16 <= 160 * 1024) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf<HALF, KS, kLeafWavesLarge>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds16);
+        k_leaf<HALF, KS, kLeafWavesLarge><<<a.nleaves, kLeafWavesLarge * 64, lds16, st>>>(a);
+    } else {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf<HALF, KS, kLeafWavesSmall>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds8);
+        k_leaf<HALF, KS, kLeafWavesSmall><<<a.nleaves, kLeafWavesSmall * 64, lds8, st>>>(a);
+    }
 }
 
 template <uint32_t HALF>
-static void launch_leaf_h(const LeafArgs& a, size_t lds, hipStream_t st) {
+static void launch_leaf_h(const LeafArgs& a, hipStream_t st) {
     switch ((a.d + 15) / 16) {
-        case 1: launch_leaf_t<HALF, 1>(a, lds, st); break;
-        case 2: launch_leaf_t<HALF, 2>(a, lds, st); break;
-        case 3: launch_leaf_t<HALF, 3>(a, lds, st); break;
-        case 4: launch_leaf_t<HALF, 4>(a, lds, st); break;
-        case 5: launch_leaf_t<HALF, 5>(a, lds, st); break;
-        case 6: launch_leaf_t<HALF, 6>(a, lds, st); break;
-        case 7: launch_leaf_t<HALF, 7>(a, lds, st); break;
-        default: launch_leaf_t<HALF, 8>(a, lds, st); break;
+        case 1: launch_leaf_t<HALF, 1>(a, st); break;
+        case 2: launch_leaf_t<HALF, 2>(a, st); break;
+        case 3: launch_leaf_t<HALF, 3>(a, st); break;
+        case 4: launch_leaf_t<HALF, 4>(a, st); break;
+        case 5: launch_leaf_t<HALF, 5>(a, st); break;
+        case 6: launch_leaf_t<HALF, 6>(a, st); break;
+        case 7: launch_leaf_t<HALF, 7>(a, st); break;
+        default: launch_leaf_t<HALF, 8>(a, st); break;
     }
 }
 
 void launch_leaf(const LeafArgs& a, hipStream_t st) {
     if (!a.nleaves) return;
-    const size_t lds = leaf_lds_bytes(a.max_leaf, a.d, a.ld);
-    if (a.half_kind == 0) launch_leaf_h<0>(a, lds, st);
-    else launch_leaf_h<1>(a, lds, st);
+    if (a.half_kind == 0) launch_leaf_h<0>(a, st);
+    else launch_leaf_h<1>(a, st);
 }
 
 // ---- top-k state helpers ----------------------------------------------------------------------
