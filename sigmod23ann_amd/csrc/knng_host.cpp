@@ -833,8 +833,15 @@ struct Builder {
         // later passes of a build screen only offers with a new hop (not in the ordered form: an
         // old entry can cross the rank threshold and be expanded for the first time)
         const bool incremental = have_prev && (prm.flags & KNNG_FLAG_INCREMENTAL_NON) && prm.non_rank_lo == 0;
+        const bool reverse = ld <= 128 && non_reverse_lds(k, ld) <= 160 * 1024 &&
+                             !(prm.flags & KNNG_FLAG_DIRECT_NON);
+        if (reverse) {  // the snapshot kernel also counts the reverse lists
+            KNNG_TRY(rcnt.ensure(n));
+            KNNG_TRY(rnew.ensure(n));
+        }
         launch_non_snapshot(knn_ids.p, snap.p, n, k, incremental ? snap_prev.p : nullptr,
-                            prm.non_rank_lo, st);
+                            prm.non_rank_lo, reverse ? rcnt.p : nullptr,
+                            reverse && incremental ? rnew.p : nullptr, st);
         NonArgs da{};
         da.X = X;
         da.ld = ld;
@@ -845,8 +852,6 @@ struct Builder {
         da.knn_ids = knn_ids.p;
         da.knn_d = knn_d.p;
         da.bound = bound.p;
-        const bool reverse = ld <= 128 && non_reverse_lds(k, ld) <= 160 * 1024 &&
-                             !(prm.flags & KNNG_FLAG_DIRECT_NON);
         if (!reverse) {
             launch_non_direct(da, nullptr, n, st);
             KNNG_HIP(hipGetLastError());

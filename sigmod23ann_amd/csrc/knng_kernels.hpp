@@ -149,13 +149,17 @@ void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float
 // Sorted snapshot; an entry absent from `prev` (nullable: all) carries kNewBit, an expanded one
 // kExpBit. rank_lo = 0: unfilled slots -> id 0, every entry expanded; rank_lo > 0: unfilled
 // slots -> the point's own id, filled entries at ranks >= rank_lo expanded.
+// rcnt (nullable): also count the reverse lists (their sizes for launch_reverse_lists), and rnew
+// (nullable) their new entries.
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
-                         const uint32_t* prev, uint32_t rank_lo, hipStream_t st);
+                         const uint32_t* prev, uint32_t rank_lo, uint32_t* rcnt, uint32_t* rnew,
+                         hipStream_t st);
 // plist == nullptr: points 0..np-1.
 void launch_non_direct(const NonArgs& a, const uint32_t* plist, uint32_t np, hipStream_t st);
 size_t scan_scratch_elems(uint32_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint64_t* out, uint64_t* scratch,
                            hipStream_t st);
+// rcnt (and rnew) hold the counts launch_non_snapshot made: offsets, then the lists
 void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t* rcnt,
                           uint32_t* rnew, uint64_t* roff, uint64_t* scratch, uint32_t* rfill,
                           uint32_t* rfill_old, uint32_t* rs, hipStream_t st);

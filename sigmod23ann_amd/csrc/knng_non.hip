@@ -34,11 +34,16 @@
 namespace knng {
 
 // ---- snapshot sorted by id -------------------------------------------------------------------
+// rcnt (nullable): each entry that goes to a reverse list (expanded, and not a repeat of an id
+// that sorts before it, i.e. the repeated id 0 of an unfilled list) is counted for RS[id], the
+// same entries k_rs_fill places; rnew (nullable) counts the new ones.
 __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __restrict__ ids,
                                                              uint32_t* __restrict__ snap,
                                                              uint32_t n, uint32_t k,
                                                              const uint32_t* __restrict__ prev,
-                                                             uint32_t rank_lo) {
+                                                             uint32_t rank_lo,
+                                                             uint32_t* __restrict__ rcnt,
+                                                             uint32_t* __restrict__ rnew) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (p >= n) return;
@@ -51,18 +56,23 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
     // slots are in (dist, id) order, so a slot is its rank
     const uint32_t x0 = (!rank_lo || (!un0 && lane >= rank_lo)) ? kExpBit : 0u;
     const uint32_t x1 = (!rank_lo || (!un1 && lane + 64 >= rank_lo)) ? kExpBit : 0u;
-    // rank = number of (value, slot) pairs before mine
+    // rank = number of (value, slot) pairs before mine; rep: an equal value sorts before mine
     uint32_t r0 = 0, r1 = 0;
+    bool rep0 = false, rep1 = false;
     const uint32_t e0 = lane, e1 = lane + 64;
     for (uint32_t j = 0; j < 64; ++j) {
         const uint32_t a = __shfl(v0, j), b = __shfl(v1, j);
         if (j < k) {
             r0 += (a < v0 || (a == v0 && j < e0));
             r1 += (a < v1 || (a == v1 && j < e1));
+            rep0 |= a == v0 && j < e0;
+            rep1 |= a == v1 && j < e1;
         }
         if (j + 64 < k) {
             r0 += (b < v0 || (b == v0 && j + 64 < e0));
             r1 += (b < v1 || (b == v1 && j + 64 < e1));
+            rep0 |= b == v0 && j + 64 < e0;
+            rep1 |= b == v1 && j + 64 < e1;
         }
     }
     // new: not in the previous snapshot of this point (all new without one)
@@ -83,22 +93,20 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
     uint32_t* dst = snap + (size_t)p * k;
     if (lane < k) dst[r0] = v0 | f0 | x0;
     if (lane + 64 < k) dst[r1] = v1 | f1 | x1;
+    if (rcnt) {
+        if (lane < k && x0 && !rep0) {
+            atomicAdd(&rcnt[v0], 1u);
+            if (rnew && f0) atomicAdd(&rnew[v0], 1u);
+        }
+        if (lane + 64 < k && x1 && !rep1) {
+            atomicAdd(&rcnt[v1], 1u);
+            if (rnew && f1) atomicAdd(&rnew[v1], 1u);
+        }
+    }
 }
 
 // ---- reverse lists: RS[q] = { p : q in S[p] } ----------------------------------------------------
 // RS[q] holds the p with a new entry q first (rnew[q] of them), then the rest.
-__global__ void k_rs_count(const uint32_t* __restrict__ snap, uint32_t n, uint32_t k,
-                           uint32_t* __restrict__ rcnt, uint32_t* __restrict__ rnew) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)n * k) return;
-    const uint32_t slot = (uint32_t)(i % k);
-    const uint32_t v = snap[i], q = v & kIdMask;
-    if (!(v & kExpBit)) return;
-    if (slot > 0 && (snap[i - 1] & kIdMask) == q) return;  // repeated id 0 of an unfilled list
-    atomicAdd(&rcnt[q], 1u);
-    if (rnew && (v & kNewBit)) atomicAdd(&rnew[q], 1u);
-}
-
 __global__ void k_rs_fill(const uint32_t* __restrict__ snap, uint32_t n, uint32_t k,
                           const uint64_t* __restrict__ roff, const uint32_t* __restrict__ rcnt,
                           uint32_t* __restrict__ rfill, uint32_t* __restrict__ rfill_old,
@@ -949,8 +957,13 @@ void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float
 
 // ---- launchers ----------------------------------------------------------------------------------------
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
-                         const uint32_t* prev, uint32_t rank_lo, hipStream_t st) {
-    if (n) k_non_snapshot_sorted<<<(n + 3) / 4, 256, 0, st>>>(ids, snap, n, k, prev, rank_lo);
+                         const uint32_t* prev, uint32_t rank_lo, uint32_t* rcnt, uint32_t* rnew,
+                         hipStream_t st) {
+    if (!n) return;
+    if (rcnt) (void)hipMemsetAsync(rcnt, 0, (size_t)n * 4, st);
+    if (rcnt && rnew) (void)hipMemsetAsync(rnew, 0, (size_t)n * 4, st);
+    k_non_snapshot_sorted<<<(n + 3) / 4, 256, 0, st>>>(ids, snap, n, k, prev, rank_lo, rcnt,
+                                                       rcnt ? rnew : nullptr);
 }
 
 void launch_non_direct(const NonArgs& a, const uint32_t* plist, uint32_t np, hipStream_t st) {
@@ -979,13 +992,8 @@ void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t
     if (!tot) return;
     const unsigned g = (unsigned)((tot + 255) / 256);
     // rnew == nullptr: no new/old split (every entry filled from the front)
-    (void)hipMemsetAsync(rcnt, 0, (size_t)n * 4, st);
     (void)hipMemsetAsync(rfill, 0, (size_t)n * 4, st);
-    if (rnew) {
-        (void)hipMemsetAsync(rnew, 0, (size_t)n * 4, st);
-        (void)hipMemsetAsync(rfill_old, 0, (size_t)n * 4, st);
-    }
-    k_rs_count<<<g, 256, 0, st>>>(snap, n, k, rcnt, rnew);
+    if (rnew) (void)hipMemsetAsync(rfill_old, 0, (size_t)n * 4, st);
     launch_exclusive_scan(rcnt, n, roff, scratch, st);
     k_rs_fill<<<g, 256, 0, st>>>(snap, n, k, roff, rcnt, rfill, rnew ? rfill_old : nullptr, rs);
 }
