@@ -107,19 +107,30 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
 
 // ---- reverse lists: RS[q] = { p : q in S[p] } ----------------------------------------------------
 // RS[q] holds the p with a new entry q first (rnew[q] of them), then the rest.
-__global__ void k_rs_fill(const uint32_t* __restrict__ snap, uint32_t n, uint32_t k,
-                          const uint64_t* __restrict__ roff, const uint32_t* __restrict__ rcnt,
-                          uint32_t* __restrict__ rfill, uint32_t* __restrict__ rfill_old,
-                          uint32_t* __restrict__ rs) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)n * k) return;
-    const uint32_t slot = (uint32_t)(i % k);
-    const uint32_t v = snap[i], q = v & kIdMask;
-    if (!(v & kExpBit)) return;
-    if (slot > 0 && (snap[i - 1] & kIdMask) == q) return;
-    const uint32_t pos = (!rfill_old || (v & kNewBit)) ? atomicAdd(&rfill[q], 1u)
-                                                       : rcnt[q] - 1u - atomicAdd(&rfill_old[q], 1u);
-    rs[roff[q] + pos] = (uint32_t)(i / k);
+// One wave per snapshot row (slots lane, lane + 64; k <= 128), four rows per workgroup. An entry
+// is placed when it is expanded and its id differs from the previous slot's (a repeated id 0 of
+// an unfilled list is placed once), the entries launch_non_snapshot counted.
+__global__ __launch_bounds__(256) void k_rs_fill(const uint32_t* __restrict__ snap, uint32_t n, uint32_t k,
+                                                 const uint64_t* __restrict__ roff, const uint32_t* __restrict__ rcnt,
+                                                 uint32_t* __restrict__ rfill, uint32_t* __restrict__ rfill_old,
+                                                 uint32_t* __restrict__ rs) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= n) return;
+    const uint32_t* row = snap + (size_t)p * k;
+    const uint32_t v0 = lane < k ? row[lane] : 0u, v1 = lane + 64 < k ? row[lane + 64] : 0u;
+    const uint32_t q0 = v0 & kIdMask, q1 = v1 & kIdMask;
+    // the previous slot's id: lane - 1 of the same half; slot 63 for slot 64
+    const uint32_t up0 = __shfl_up(q0, 1), up1 = __shfl_up(q1, 1), last0 = __shfl(q0, 63);
+    const bool ok0 = lane < k && (v0 & kExpBit) && !(lane > 0 && up0 == q0);
+    const bool ok1 = lane + 64 < k && (v1 & kExpBit) && (lane == 0 ? last0 : up1) != q1;
+    auto place = [&](uint32_t v, uint32_t q) {
+        const uint32_t pos = (!rfill_old || (v & kNewBit)) ? atomicAdd(&rfill[q], 1u)
+                                                           : rcnt[q] - 1u - atomicAdd(&rfill_old[q], 1u);
+        rs[roff[q] + pos] = p;
+    };
+    if (ok0) place(v0, q0);
+    if (ok1) place(v1, q1);
 }
 
 // ---- exclusive scan of n u32 counts into n+1 u64 offsets ----------------------------------------
@@ -988,9 +999,8 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint64_t* out, uint64
 void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t* rcnt,
                           uint32_t* rnew, uint64_t* roff, uint64_t* scratch, uint32_t* rfill,
                           uint32_t* rfill_old, uint32_t* rs, hipStream_t st) {
-    const size_t tot = (size_t)n * k;
-    if (!tot) return;
-    const unsigned g = (unsigned)((tot + 255) / 256);
+    if (!n || !k) return;
+    const unsigned g = (n + 3) / 4;
     // rnew == nullptr: no new/old split (every entry filled from the front)
     (void)hipMemsetAsync(rfill, 0, (size_t)n * 4, st);
     if (rnew) (void)hipMemsetAsync(rfill_old, 0, (size_t)n * 4, st);
