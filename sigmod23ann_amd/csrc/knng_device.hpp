@@ -102,6 +102,37 @@ __device__ inline float dist_ref(const float* __restrict__ a, const float* __res
     return s;
 }
 
+// dist_ref (the reference's order: eight accumulators over dims j, j+8, ... below floor(d/8)*8,
+// summed in order into 0.0f, then the tail with fused multiply-add) for a compile-time width D,
+// fully unrolled so that every LDS load of the two rows is issued at once. Rows are 16-byte
+// aligned.
+template <uint32_t D>
+__device__ inline float dist_ref_fixed(const float* __restrict__ a, const float* __restrict__ b) {
+    constexpr uint32_t kMain = D & ~7u;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (uint32_t i = 0; i < kMain; i += 4) {
+        const float4 x = *reinterpret_cast<const float4*>(a + i);
+        const float4 y = *reinterpret_cast<const float4*>(b + i);
+        const uint32_t j = i & 4u;
+        float t;
+        t = x.x - y.x; acc[j + 0] = __builtin_fmaf(t, t, acc[j + 0]);
+        t = x.y - y.y; acc[j + 1] = __builtin_fmaf(t, t, acc[j + 1]);
+        t = x.z - y.z; acc[j + 2] = __builtin_fmaf(t, t, acc[j + 2]);
+        t = x.w - y.w; acc[j + 3] = __builtin_fmaf(t, t, acc[j + 3]);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) s = s + acc[j];
+#pragma unroll
+    for (uint32_t i = kMain; i < D; ++i) {
+        const float t = a[i] - b[i];
+        s = __builtin_fmaf(t, t, s);
+    }
+    return s;
+}
+
+
 // Hyperplane between two centres (SolutionKmeans.hpp:162-164): between = 0.5f * (c1 + c2),
 // coef = c1 - between, offset = dot(between, coef). Called by a whole block; `between` is block
 // scratch of d floats; returns the offset to every thread.

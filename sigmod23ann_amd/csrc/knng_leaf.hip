@@ -233,7 +233,10 @@ __global__ __launch_bounds__(W * 64) void k_leaf(LeafArgs a) {
                 uint64_t x = WaveList::kEmpty;
                 if (c0 + lane < S) {
                     const uint32_t c = s_cand[c0 + lane];
-                    x = WaveList::key(dist_ref(s_rowbuf, s_f32 + (size_t)(c - c_lo) * ld, d), s_ids[c]);
+                    const float* cr = s_f32 + (size_t)(c - c_lo) * ld;
+                    // the reference's width (points[][112], 100 dims) fully unrolled
+                    x = WaveList::key(d == 100 ? dist_ref_fixed<100>(s_rowbuf, cr) : dist_ref(s_rowbuf, cr, d),
+                                      s_ids[c]);
                 }
                 n_ins += __popcll(__ballot((uint32_t)(x >> 32) < __float_as_uint(lst.bound)));
                 changed |= lst.offer64(x, k, lane, s_scr);
