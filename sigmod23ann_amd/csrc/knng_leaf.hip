@@ -178,15 +178,28 @@ __global__ __launch_bounds__(W * 64) void k_leaf(LeafArgs a) {
     // parts of part_rows rows when the whole bucket does not fit; each part re-ranks the survivors
     // whose column lies in it.
     float* s_f32 = reinterpret_cast<float*>(lds + L.off_rows);
+    __attribute__((address_space(3))) float* s_f32_lds = (__attribute__((address_space(3))) float*)(s_f32);
     const uint32_t ld4 = ld / 4;
+    // t / ld4 = umulhi(t, ceil(2^32 / ld4)) exactly for t < 2^32 / ld4 (ld4 > 1)
+    const uint32_t ld4_inv = ld4 > 1 ? (uint32_t)((0x100000000ull + ld4 - 1) / ld4) : 0u;
     uint32_t n_surv = 0, n_ins = 0;
     for (uint32_t c_lo = 0; c_lo < b; c_lo += L.part_rows) {
         const uint32_t c_hi = min(b, c_lo + L.part_rows);
         __syncthreads();
-        for (uint32_t t = tid; t < (c_hi - c_lo) * ld4; t += kLeafThreads) {
-            const uint32_t rr = t / ld4, q = t - rr * ld4;
-            reinterpret_cast<float4*>(s_f32)[t] =
-                reinterpret_cast<const float4*>(a.X + (size_t)s_ids[c_lo + rr] * ld)[q];
+        // the part's rows go from global memory straight into LDS (global_load_lds_dwordx4: lane i
+        // of a wave writes 16 bytes at the wave's base + 16 i), waited for before the barrier
+        {
+            const uint32_t tot4 = (c_hi - c_lo) * ld4;
+            for (uint32_t t0 = tid - lane; t0 < tot4; t0 += kLeafThreads) {
+                const uint32_t t = t0 + lane;
+                if (t < tot4) {
+                    const uint32_t rr = ld4 == 1 ? t : __umulhi(t, ld4_inv), q = t - rr * ld4;
+                    __builtin_amdgcn_global_load_lds(
+                        a.X + (size_t)s_ids[c_lo + rr] * ld + 4 * q,
+                        (__attribute__((address_space(3))) void*)(s_f32_lds + 4 * t0), 16, 0, 0);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
         }
         __syncthreads();
         // tiles overlapping [c_lo, c_hi): bits outside the part are masked off. The next row's
