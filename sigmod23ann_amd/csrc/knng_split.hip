@@ -113,14 +113,26 @@ __global__ __launch_bounds__(kSplitThreads) void k_lloyd_chunk(
     }
     atomicAdd(&nlo, mylo);
     __syncthreads();
-    // FP64 sums in entry order, one thread per (side, dim).
+    // FP64 sums in entry order, one thread per (side, dim). The loads of 16 entries are issued
+    // before their adds, which stay in entry order.
+    constexpr uint32_t kAhead = 16;
     for (uint32_t t = threadIdx.x; t < 2 * ld; t += blockDim.x) {
         const uint32_t side = t / ld, j = t - side * ld;
         const uint8_t want = side == 0 ? 1 : 0;
         double acc = 0.0;
         if (j < d) {
-            for (uint32_t e = 0; e < ch.count; ++e)
-                if (flags[e] == want) acc += (double)X[(size_t)rows[e] * ld + j];
+            const float* xj = X + j;
+            uint32_t e = 0;
+            for (; e + kAhead <= ch.count; e += kAhead) {
+                float v[kAhead];
+#pragma unroll
+                for (uint32_t u = 0; u < kAhead; ++u) v[u] = xj[(size_t)rows[e + u] * ld];
+#pragma unroll
+                for (uint32_t u = 0; u < kAhead; ++u)
+                    if (flags[e + u] == want) acc += (double)v[u];
+            }
+            for (; e < ch.count; ++e)
+                if (flags[e] == want) acc += (double)xj[(size_t)rows[e] * ld];
         }
         part[((size_t)blockIdx.x * 2 + side) * ld + j] = acc;
     }
