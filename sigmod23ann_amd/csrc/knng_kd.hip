@@ -47,15 +47,26 @@ __global__ __launch_bounds__(64) void k_kd_tile_sums(uint32_t mode, const float*
     const uint32_t base = nodes[tl.node].start + tl.begin;
     const double mu = mode ? mean[(size_t)tl.node * kKdStride + c] : 0.0;
     double s = 0.0;
-    for (uint32_t e = 0; e < tl.count; ++e) {
-        const double x = (double)rot[(size_t)perm[base + e] * kKdStride + c];
+    auto add = [&](float v) {
+        const double x = (double)v;
         if (mode) {
             const double t = x - mu;
             s = s + t * t;
         } else {
             s = s + x;
         }
+    };
+    // the loads of 16 positions are issued before their adds, which stay in position order
+    constexpr uint32_t kAhead = 16;
+    uint32_t e = 0;
+    for (; e + kAhead <= tl.count; e += kAhead) {
+        float v[kAhead];
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; ++u) v[u] = rot[(size_t)perm[base + e + u] * kKdStride + c];
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; ++u) add(v[u]);
     }
+    for (; e < tl.count; ++e) add(rot[(size_t)perm[base + e] * kKdStride + c]);
     part[(size_t)blockIdx.x * kKdStride + c] = s;
 }
 
