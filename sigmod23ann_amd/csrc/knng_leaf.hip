@@ -80,6 +80,9 @@ __global__ __launch_bounds__(W * 64) void k_leaf(LeafArgs a) {
     uint16_t* s_cand = reinterpret_cast<uint16_t*>(lds + L.off_cand) + wave * L.b16;
     uint64_t* s_scr = reinterpret_cast<uint64_t*>(lds + L.off_scr) + wave * 128;
 
+    __shared__ uint32_t s_stat[2];  // the workgroup's survivors and inserts
+    if (tid < 2) s_stat[tid] = 0;
+
     // ---- phase 0: gather, centre, scale ---------------------------------------------------------
     for (uint32_t t = tid; t < b16; t += kLeafThreads) {
         const uint32_t id = t < b ? a.perm[lf.start + t] : kNoId;
@@ -261,9 +264,16 @@ __global__ __launch_bounds__(W * 64) void k_leaf(LeafArgs a) {
             __builtin_amdgcn_wave_barrier();
         }
     }
-    if (a.counters && lane == 0) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)n_surv);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[2]), (unsigned long long)n_ins);
+    if (a.counters) {  // one pair of global atomics per workgroup: same-address atomics serialise
+        if (lane == 0) {
+            atomicAdd(&s_stat[0], n_surv);
+            atomicAdd(&s_stat[1], n_ins);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), (unsigned long long)s_stat[0]);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[2]), (unsigned long long)s_stat[1]);
+        }
     }
 }
 
