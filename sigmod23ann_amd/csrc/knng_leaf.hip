@@ -295,7 +295,7 @@ constexpr uint32_t kBigRowsPerWave = kBigRows / kBigWaves;  // 4
 struct BigLds {
     uint32_t rs, fs;  // 16-bit row stride (elements), FP32 row stride (floats)
     uint32_t off_arow, off_anrm, off_arinv, off_abnd, off_crow, off_cnrm, off_crinv, off_cid;
-    uint32_t off_af32, off_cf32, off_mask, off_scr, total;
+    uint32_t off_af32, off_cf32, off_mask, off_scr, off_pend, total;
 };
 
 __host__ __device__ inline BigLds big_lds_layout(uint32_t d, uint32_t ld) {
@@ -319,6 +319,7 @@ __host__ __device__ inline BigLds big_lds_layout(uint32_t d, uint32_t ld) {
     L.off_cf32 = o;  o += kBigCols * L.fs * 4;
     L.off_mask = o;  o += kBigRows * 8;
     L.off_scr = o;   o += kBigWaves * 128 * 8;
+    L.off_pend = o;  o += kBigWaves * kBigRowsPerWave * 64 * 8;  // pending offers, 64 per owned row
     L.total = (o + 15) & ~15u;
     return L;
 }
@@ -398,8 +399,16 @@ __global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigB
     const uint32_t lrow = lane & 15, lk = (lane >> 4) * 4;
     const uint32_t strip = wave >> 2, tile = wave & 3;  // phase 1: 4 strips x 4 tiles
     bool changed[kBigRowsPerWave];
+    // Offers below a row's bound wait in a pending batch of 64 (LDS) across column blocks and go
+    // to the list by offer64 when the next block's would not fit, and at the end: the list keeps
+    // the k smallest distinct keys offered, whatever the order and the bound they passed.
+    uint64_t* s_pend = reinterpret_cast<uint64_t*>(lds + L.off_pend) + wave * kBigRowsPerWave * 64;
+    uint32_t npend[kBigRowsPerWave];
 #pragma unroll
-    for (uint32_t j = 0; j < kBigRowsPerWave; ++j) changed[j] = false;
+    for (uint32_t j = 0; j < kBigRowsPerWave; ++j) {
+        changed[j] = false;
+        npend[j] = 0;
+    }
     uint32_t n_surv = 0, n_ins = 0;
 
     // column staging: each wave fetches 4 rows of the next block into registers (fetch), then
@@ -504,25 +513,53 @@ __global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigB
             const uint32_t rr = kBigRowsPerWave * wave + j;
             if (rr >= nrows) break;
             const uint64_t w64 = *reinterpret_cast<const uint64_t*>(s_mask + rr * 4);
-            if (!w64) continue;
-            // this row's state into fixed registers (constant-index selects keep the arrays in VGPRs)
-            WaveList cur = lst[0];
+            const bool last = c0 + kBigCols >= b;  // the last column block: the batch goes out
+            uint32_t np0 = npend[0];
 #pragma unroll
             for (uint32_t jj = 1; jj < kBigRowsPerWave; ++jj)
-                if (j == jj) cur = lst[jj];
+                if (j == jj) np0 = npend[jj];
+            if (!w64 && !(last && np0)) continue;
+            // this row's state into fixed registers (constant-index selects keep the arrays in VGPRs)
+            WaveList cur = lst[0];
+            uint32_t np = npend[0];
+#pragma unroll
+            for (uint32_t jj = 1; jj < kBigRowsPerWave; ++jj)
+                if (j == jj) {
+                    cur = lst[jj];
+                    np = npend[jj];
+                }
+            uint64_t* pq = s_pend + j * 64;
+            bool ch = false;
+            // the block adds at most popc(w64) offers: offer the batch first if they might not fit
+            // (before this block's distances, which then need no registers across the offer)
+            if (np + __popcll(w64) > 64) {
+                const uint64_t pd = lane < np ? pq[lane] : WaveList::kEmpty;
+                ch = cur.offer64(pd, k, lane, s_scr);
+                np = 0;
+            }
             uint64_t x = WaveList::kEmpty;
             if ((w64 >> lane) & 1ull)
                 x = WaveList::key(dist_ref(s_af32 + (size_t)rr * L.fs, s_cf32 + (size_t)lane * L.fs, d),
                                   s_cid[lane]);
             n_surv += __popcll(w64);
-            n_ins += __popcll(__ballot((uint32_t)(x >> 32) < __float_as_uint(cur.bound)));
-            const bool ch = cur.offer64(x, k, lane, s_scr);
+            const bool below = x < cur.kth;
+            const uint64_t m = __ballot(below);
+            n_ins += __popcll(m);
             __builtin_amdgcn_wave_barrier();
+            if (below) pq[np + __popcll(m & ((1ull << lane) - 1ull))] = x;
+            __builtin_amdgcn_wave_barrier();
+            np += __popcll(m);
+            if (last && np) {
+                const uint64_t pd = lane < np ? pq[lane] : WaveList::kEmpty;
+                ch |= cur.offer64(pd, k, lane, s_scr);
+                np = 0;
+            }
 #pragma unroll
             for (uint32_t jj = 0; jj < kBigRowsPerWave; ++jj)
                 if (j == jj) {
                     lst[jj] = cur;
                     changed[jj] |= ch;
+                    npend[jj] = np;
                 }
             if (lane == 0) s_abnd[rr] = cur.bound;
         }
