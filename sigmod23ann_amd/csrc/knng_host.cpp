@@ -279,6 +279,7 @@ struct Builder {
     DevBuf<uint32_t> ovf_flag;
     DevBuf<uint32_t> rcnt, rfill, rsl, ncnt, nbuf_id;
     DevBuf<uint32_t> snap_prev, rnew, rfill_old;
+    DevBuf<uint32_t> rslot;  // full passes only (incremental ones hold snap_prev instead)
     bool have_prev = false;  // snap_prev holds the previous pass's snapshot of the current lists
     uint32_t pass_no = 0;    // neighbours-of-neighbours passes so far in this build
     DevBuf<uint64_t> roff, scan_tmp;
@@ -835,13 +836,16 @@ struct Builder {
         const bool incremental = have_prev && (prm.flags & KNNG_FLAG_INCREMENTAL_NON) && prm.non_rank_lo == 0;
         const bool reverse = ld <= 128 && non_reverse_lds(k, ld) <= 160 * 1024 &&
                              !(prm.flags & KNNG_FLAG_DIRECT_NON);
-        if (reverse) {  // the snapshot kernel also counts the reverse lists
+        if (reverse) {  // the snapshot kernel also counts the reverse lists (and, in full
+                        // passes, gives each entry its slot)
             KNNG_TRY(rcnt.ensure(n));
             KNNG_TRY(rnew.ensure(n));
+            if (!incremental) KNNG_TRY(rslot.ensure(nk));
         }
         launch_non_snapshot(knn_ids.p, snap.p, n, k, incremental ? snap_prev.p : nullptr,
                             prm.non_rank_lo, reverse ? rcnt.p : nullptr,
-                            reverse && incremental ? rnew.p : nullptr, st);
+                            reverse && incremental ? rnew.p : nullptr,
+                            reverse && !incremental ? rslot.p : nullptr, st);
         NonArgs da{};
         da.X = X;
         da.ld = ld;
@@ -873,7 +877,7 @@ struct Builder {
         KNNG_TRY(perm.ensure(n));
         if (!perm_is_tree) launch_iota(perm.p, n, st);
         launch_reverse_lists(snap.p, n, k, rcnt.p, incremental ? rnew.p : nullptr, roff.p, scan_tmp.p,
-                             rfill.p, rfill_old.p, rsl.p, st);
+                             rfill.p, rfill_old.p, incremental ? nullptr : rslot.p, rsl.p, st);
         if (verbose) {
             std::vector<uint32_t> rc(n);
             KNNG_HIP(hipMemcpyAsync(rc.data(), rcnt.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));

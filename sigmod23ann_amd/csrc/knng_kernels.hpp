@@ -150,10 +150,11 @@ void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float
 // kExpBit. rank_lo = 0: unfilled slots -> id 0, every entry expanded; rank_lo > 0: unfilled
 // slots -> the point's own id, filled entries at ranks >= rank_lo expanded.
 // rcnt (nullable): also count the reverse lists (their sizes for launch_reverse_lists), and rnew
-// (nullable) their new entries.
+// (nullable) their new entries; rslot (nullable, n*k, used without rnew): each counted entry's
+// slot in its reverse list, for launch_reverse_lists to place it without an atomic.
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
                          const uint32_t* prev, uint32_t rank_lo, uint32_t* rcnt, uint32_t* rnew,
-                         hipStream_t st);
+                         uint32_t* rslot, hipStream_t st);
 // plist == nullptr: points 0..np-1.
 void launch_non_direct(const NonArgs& a, const uint32_t* plist, uint32_t np, hipStream_t st);
 size_t scan_scratch_elems(uint32_t n);
@@ -162,7 +163,7 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint64_t* out, uint64
 // rcnt (and rnew) hold the counts launch_non_snapshot made: offsets, then the lists
 void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t* rcnt,
                           uint32_t* rnew, uint64_t* roff, uint64_t* scratch, uint32_t* rfill,
-                          uint32_t* rfill_old, uint32_t* rs, hipStream_t st);
+                          uint32_t* rfill_old, const uint32_t* rslot, uint32_t* rs, hipStream_t st);
 size_t non_reverse_lds(uint32_t k, uint32_t ld);
 void launch_non_reverse(const NonRevArgs& a, hipStream_t st);
 void launch_non_merge(const NonRevArgs& a, uint32_t* ovf_cnt, uint32_t* ovf_list, uint32_t* ovf_flag,

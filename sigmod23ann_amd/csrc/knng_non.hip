@@ -36,14 +36,16 @@ namespace knng {
 // ---- snapshot sorted by id -------------------------------------------------------------------
 // rcnt (nullable): each entry that goes to a reverse list (expanded, and not a repeat of an id
 // that sorts before it, i.e. the repeated id 0 of an unfilled list) is counted for RS[id], the
-// same entries k_rs_fill places; rnew (nullable) counts the new ones.
+// same entries k_rs_fill places; rnew (nullable) counts the new ones; rslot (nullable, not with
+// rnew) receives each such entry's slot in RS[id] (the count before it), at its snapshot slot.
 __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __restrict__ ids,
                                                              uint32_t* __restrict__ snap,
                                                              uint32_t n, uint32_t k,
                                                              const uint32_t* __restrict__ prev,
                                                              uint32_t rank_lo,
                                                              uint32_t* __restrict__ rcnt,
-                                                             uint32_t* __restrict__ rnew) {
+                                                             uint32_t* __restrict__ rnew,
+                                                             uint32_t* __restrict__ rslot) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (p >= n) return;
@@ -95,11 +97,13 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
     if (lane + 64 < k) dst[r1] = v1 | f1 | x1;
     if (rcnt) {
         if (lane < k && x0 && !rep0) {
-            atomicAdd(&rcnt[v0], 1u);
+            const uint32_t at = atomicAdd(&rcnt[v0], 1u);
+            if (rslot) rslot[(size_t)p * k + r0] = at;
             if (rnew && f0) atomicAdd(&rnew[v0], 1u);
         }
         if (lane + 64 < k && x1 && !rep1) {
-            atomicAdd(&rcnt[v1], 1u);
+            const uint32_t at = atomicAdd(&rcnt[v1], 1u);
+            if (rslot) rslot[(size_t)p * k + r1] = at;
             if (rnew && f1) atomicAdd(&rnew[v1], 1u);
         }
     }
@@ -113,7 +117,7 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
 __global__ __launch_bounds__(256) void k_rs_fill(const uint32_t* __restrict__ snap, uint32_t n, uint32_t k,
                                                  const uint64_t* __restrict__ roff, const uint32_t* __restrict__ rcnt,
                                                  uint32_t* __restrict__ rfill, uint32_t* __restrict__ rfill_old,
-                                                 uint32_t* __restrict__ rs) {
+                                                 const uint32_t* __restrict__ rslot, uint32_t* __restrict__ rs) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (p >= n) return;
@@ -124,13 +128,15 @@ __global__ __launch_bounds__(256) void k_rs_fill(const uint32_t* __restrict__ sn
     const uint32_t up0 = __shfl_up(q0, 1), up1 = __shfl_up(q1, 1), last0 = __shfl(q0, 63);
     const bool ok0 = lane < k && (v0 & kExpBit) && !(lane > 0 && up0 == q0);
     const bool ok1 = lane + 64 < k && (v1 & kExpBit) && (lane == 0 ? last0 : up1) != q1;
-    auto place = [&](uint32_t v, uint32_t q) {
-        const uint32_t pos = (!rfill_old || (v & kNewBit)) ? atomicAdd(&rfill[q], 1u)
-                                                           : rcnt[q] - 1u - atomicAdd(&rfill_old[q], 1u);
+    // rslot: the slot the snapshot kernel's count gave the entry; else one taken here
+    auto place = [&](uint32_t v, uint32_t q, uint32_t e) {
+        const uint32_t pos = rslot ? rslot[(size_t)p * k + e]
+                             : (!rfill_old || (v & kNewBit)) ? atomicAdd(&rfill[q], 1u)
+                                                             : rcnt[q] - 1u - atomicAdd(&rfill_old[q], 1u);
         rs[roff[q] + pos] = p;
     };
-    if (ok0) place(v0, q0);
-    if (ok1) place(v1, q1);
+    if (ok0) place(v0, q0, lane);
+    if (ok1) place(v1, q1, lane + 64);
 }
 
 // ---- exclusive scan of n u32 counts into n+1 u64 offsets ----------------------------------------
@@ -939,12 +945,13 @@ void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float
 // ---- launchers ----------------------------------------------------------------------------------------
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
                          const uint32_t* prev, uint32_t rank_lo, uint32_t* rcnt, uint32_t* rnew,
-                         hipStream_t st) {
+                         uint32_t* rslot, hipStream_t st) {
     if (!n) return;
     if (rcnt) (void)hipMemsetAsync(rcnt, 0, (size_t)n * 4, st);
     if (rcnt && rnew) (void)hipMemsetAsync(rnew, 0, (size_t)n * 4, st);
     k_non_snapshot_sorted<<<(n + 3) / 4, 256, 0, st>>>(ids, snap, n, k, prev, rank_lo, rcnt,
-                                                       rcnt ? rnew : nullptr);
+                                                       rcnt ? rnew : nullptr,
+                                                       rcnt && !rnew ? rslot : nullptr);
 }
 
 void launch_non_direct(const NonArgs& a, const uint32_t* plist, uint32_t np, hipStream_t st) {
@@ -968,14 +975,15 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint64_t* out, uint64
 
 void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t* rcnt,
                           uint32_t* rnew, uint64_t* roff, uint64_t* scratch, uint32_t* rfill,
-                          uint32_t* rfill_old, uint32_t* rs, hipStream_t st) {
+                          uint32_t* rfill_old, const uint32_t* rslot, uint32_t* rs, hipStream_t st) {
     if (!n || !k) return;
     const unsigned g = (n + 3) / 4;
     // rnew == nullptr: no new/old split (every entry filled from the front)
-    (void)hipMemsetAsync(rfill, 0, (size_t)n * 4, st);
+    if (rnew) rslot = nullptr;
+    if (!rslot) (void)hipMemsetAsync(rfill, 0, (size_t)n * 4, st);
     if (rnew) (void)hipMemsetAsync(rfill_old, 0, (size_t)n * 4, st);
     launch_exclusive_scan(rcnt, n, roff, scratch, st);
-    k_rs_fill<<<g, 256, 0, st>>>(snap, n, k, roff, rcnt, rfill, rnew ? rfill_old : nullptr, rs);
+    k_rs_fill<<<g, 256, 0, st>>>(snap, n, k, roff, rcnt, rfill, rnew ? rfill_old : nullptr, rslot, rs);
 }
 
 void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {
