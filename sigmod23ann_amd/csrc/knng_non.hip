@@ -849,24 +849,33 @@ __global__ __launch_bounds__(256) void k_non_merge(NonRevArgs a, uint32_t* __res
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (p >= a.n) return;
+    // the count, the list and the first 64 candidate slots are loaded together (slots past the
+    // count hold stale values and are masked below), so a point waits out one memory latency
+    const uint32_t k = a.k;
+    uint32_t* gid = a.knn_ids + (size_t)p * k;
+    float* gd = a.knn_d + (size_t)p * k;
+    const size_t cb = (size_t)p * a.cap;
     const uint32_t m = a.ncnt[p];
+    const uint32_t id_l = lane < a.cap ? a.nbuf_id[cb + lane] : 0u;
+    const float d_l = lane < a.cap ? a.nbuf_d[cb + lane] : 0.f;
+    const uint32_t lid0 = lane < k ? gid[lane] : kNoId, lid1 = lane + 64 < k ? gid[lane + 64] : kNoId;
+    const float ld0 = lane < k ? gd[lane] : kFltMax, ld1 = lane + 64 < k ? gd[lane + 64] : kFltMax;
     if (m == 0) return;
     if (m > a.cap) {
         if (lane == 0 && atomicExch(&ovf_flag[p], 1u) == 0u) ovf_list[atomicAdd(ovf_cnt, 1u)] = p;
         return;
     }
-    const uint32_t k = a.k;
-    uint32_t* gid = a.knn_ids + (size_t)p * k;
-    float* gd = a.knn_d + (size_t)p * k;
     __shared__ uint64_t scr[4][128];
     WaveList lst;
-    lst.load(gid, gd, k, lane);
+    lst.k0 = lane < k ? WaveList::key(ld0, lid0) : WaveList::kEmpty;
+    lst.k1 = lane + 64 < k ? WaveList::key(ld1, lid1) : WaveList::kEmpty;
+    lst.set_bound(k);
     bool changed = false;
     for (uint32_t c0 = 0; c0 < m; c0 += 64) {
         uint64_t x = WaveList::kEmpty;
         if (c0 + lane < m)
-            x = WaveList::key(a.nbuf_d[(size_t)p * a.cap + c0 + lane],
-                              a.nbuf_id[(size_t)p * a.cap + c0 + lane]);
+            x = c0 == 0 ? WaveList::key(d_l, id_l)
+                        : WaveList::key(a.nbuf_d[cb + c0 + lane], a.nbuf_id[cb + c0 + lane]);
         changed |= lst.offer64(x, k, lane, scr[threadIdx.x >> 6]);
     }
     if (changed) {
