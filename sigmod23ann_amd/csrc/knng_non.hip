@@ -62,8 +62,10 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
     uint32_t r0 = 0, r1 = 0;
     bool rep0 = false, rep1 = false;
     const uint32_t e0 = lane, e1 = lane + 64;
+    // slot j's values broadcast by readlane (j is uniform): no LDS round trip per step
     for (uint32_t j = 0; j < 64; ++j) {
-        const uint32_t a = __shfl(v0, j), b = __shfl(v1, j);
+        const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v0, (int)j);
+        const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v1, (int)j);
         if (j < k) {
             r0 += (a < v0 || (a == v0 && j < e0));
             r1 += (a < v1 || (a == v1 && j < e1));
@@ -85,7 +87,8 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
         const uint32_t q1 = lane + 64 < k ? pp[lane + 64] & kIdMask : 0xFFFFFFFFu;
         bool h0 = false, h1 = false;
         for (uint32_t j = 0; j < 64; ++j) {
-            const uint32_t a = __shfl(q0, j), b = __shfl(q1, j);
+            const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)q0, (int)j);
+            const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)q1, (int)j);
             h0 |= (a == v0) | (b == v0);
             h1 |= (a == v1) | (b == v1);
         }
