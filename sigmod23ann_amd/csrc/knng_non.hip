@@ -10,8 +10,8 @@
 // an unfilled slot reads as the point's own id, which every membership test rejects.
 //
 // Reverse form (the fast path). The offers (p <- r) with q in S[p] and r in S[q] are grouped by q:
-// one workgroup per q screens all pairs (p in RS[q], r in S[q]) with an FP32 MFMA Gram, drops
-// pairs with r already in S[p] (binary search in p's id-sorted snapshot) and pairs that
+// a workgroup screens all pairs (p in RS[q], r in S[q]) of a q with an FP32 MFMA Gram, drops
+// pairs with r already in S[p] (a membership mask from a hash of S[q]'s ids) and pairs that
 // cannot beat p's bound, re-ranks the rest in exact reference-order FP32 and appends the ones below
 // the bound to p's candidate buffer. A second kernel merges each buffer into its list (the list
 // insert rejects the duplicates that arrive through several q). The pre-pass bound admits a
@@ -217,15 +217,16 @@ __global__ void k_scan_total(const uint32_t* __restrict__ in, uint32_t n, uint64
 }
 
 // ---- reverse screening kernel --------------------------------------------------------------------
-// One workgroup per q. FP32 rows of S[q] and 32-row chunks of RS[q] sit in LDS, each loaded as its
+// Persistent workgroups, each taking its q one at a time. FP32 rows of S[q] and 32-row chunks of
+// RS[q] sit in LDS, each loaded as its
 // first ceil(d/4) float4 (the dims beyond, up to that bound, are zero in every buffer the builder
 // binds); an FP32 MFMA Gram (16x16x4) over ceil(d/4) steps
 // screens every pair with a rigorous margin; pairs with r already in S[p] are masked by
 // a per-chunk membership bitmap (S(q)'s ids sit in a bucketed hash table: one lookup is one
 // 16-byte LDS read and a few integer operations, with no probe loop for the lanes of a wave to
-// diverge on); the survivors queue up per wave and are re-ranked exactly, eight
+// diverge on); the survivors queue up per wave and are re-ranked exactly: a short queue eight
 // lanes per survivor (lane j owns the reference's accumulator j, then the lanes are summed in
-// order). The next chunk's rows and snapshots are fetched into registers while the current chunk
+// order), a longer one a lane per survivor. The next chunk's rows and snapshots are fetched into registers while the current chunk
 // is screened. Accepted offers collect in an LDS list and go to the candidate buffers once, after
 // the last chunk: an atomic that returns a value waits for every load issued before it, so
 // appending inside the chunk loop would stall on the next chunk's fetch.
