@@ -1,7 +1,7 @@
 // Leaf stage for gfx950: all pairs inside each leaf bucket (addCandidatesLessThan,
 // KnnSets.hpp:131-160) merged into each point's top-k set (KnnSetScannableSimd, :26-127).
 //
-// One workgroup (8 waves) per bucket, rows resident in LDS:
+// One workgroup (8 or 16 waves) per bucket, rows resident in LDS:
 //  phase 0  gather the bucket's rows, centre them on the bucket's first row, scale each row by a
 //           power of two into fp16 (or bf16) range; FP32 squared norms of the centred rows.
 //  phase 1  screening Gram on MFMA (16x16x16, fp32 accumulate), one 16-row strip per wave over all
