@@ -239,7 +239,10 @@ constexpr uint32_t kNrPfRows = kNrChunk * kNrMaxK4 / kNrThreads;  // float4 per 
 constexpr uint32_t kNrPfSnap = kNrChunk * 128 / kNrThreads;        // snapshot ids per thread
 constexpr uint32_t kNrSurv = 512;     // accepted offers held in LDS per workgroup before appending
 typedef __attribute__((address_space(3))) uint32_t kLdsU32;
-constexpr uint32_t kNrOversub = 4;    // persistent workgroups per resident slot
+// persistent workgroups per resident slot: the dispatcher hands a freed slot the next workgroup, so
+// smaller shares balance the end of a launch (NoN per 10M build: 4 -> 1606 ms, 16 -> 1581, 32 -> 1574,
+// 64 -> 1575)
+constexpr uint32_t kNrOversub = 32;
 static_assert(kNrIds == kNrThreads, "a q's first reverse ids are staged one per thread");
 static_assert(kNrThreads / 16 == kNrChunk, "16 threads commit each chunk row (one wave holds a row)");
 static_assert(kNrChunk == 32 && (kNrThreads / 64) % 2 == 0,
