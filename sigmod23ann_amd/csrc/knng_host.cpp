@@ -284,6 +284,10 @@ struct Builder {
     uint32_t pass_no = 0;    // neighbours-of-neighbours passes so far in this build
     DevBuf<uint64_t> roff, scan_tmp;
     DevBuf<float> nbuf_d, norms32;
+    // the second candidate-buffer set: sub-passes alternate, so a sub-pass's merge (on sst) runs
+    // beside the next sub-pass's screen
+    DevBuf<uint32_t> ncnt2, nbuf_id2;
+    DevBuf<float> nbuf_d2;
 
     // generator scratch
     DevBuf<float> gen_a, gen_b, gen_c;
@@ -872,6 +876,9 @@ struct Builder {
         KNNG_TRY(ncnt.ensure(n));
         KNNG_TRY(nbuf_d.ensure((size_t)n * non_cap()));
         KNNG_TRY(nbuf_id.ensure((size_t)n * non_cap()));
+        KNNG_TRY(ncnt2.ensure(n));
+        KNNG_TRY(nbuf_d2.ensure((size_t)n * non_cap()));
+        KNNG_TRY(nbuf_id2.ensure((size_t)n * non_cap()));
         KNNG_TRY(ovf.ensure((size_t)n + 1));
         KNNG_TRY(ovf_flag.ensure(n));
         KNNG_TRY(perm.ensure(n));
@@ -921,22 +928,44 @@ struct Builder {
         // pass needs more, to keep the candidate buffers within their capacity (at 10M points:
         // 6 sub-passes and 512 slots take 100 ms less than 16 and 192, with the same screen time)
         const uint32_t subs = std::max(1u, std::min(pass_no ? kNonSubpassesLater : kNonSubpasses, n / 1024 + 1));
+        // Sub-pass sp screens into candidate set sp % 2 on st; its merge runs on sst once the screen
+        // is done, beside the next screen, which may read a bound before or after the merge lowers
+        // it (a larger bound only admits candidates the merge rejects). The next screen into the
+        // same set waits for that set's merge; the pass ends when the last merge has.
         float t_rev = 0.f, t_merge = 0.f, ms = 0.f;
+        std::vector<hipEvent_t> spev(4 * subs, nullptr);
+        struct EvGuard {
+            std::vector<hipEvent_t>& v;
+            ~EvGuard() {
+                for (auto e : v)
+                    if (e) (void)hipEventDestroy(e);
+            }
+        } spguard{spev};
+        for (auto& e : spev) KNNG_HIP(hipEventCreate(&e));
         for (uint32_t sp = 0; sp < subs; ++sp) {
             a.order = perm.p + sp;
             a.order_stride = subs;
             a.nq = (n - sp + subs - 1) / subs;
-            KNNG_HIP(hipMemsetAsync(ncnt.p, 0, (size_t)n * 4, st));
-            KNNG_HIP(hipEventRecord(ev[2], st));
+            a.ncnt = (sp & 1) ? ncnt2.p : ncnt.p;
+            a.nbuf_d = (sp & 1) ? nbuf_d2.p : nbuf_d.p;
+            a.nbuf_id = (sp & 1) ? nbuf_id2.p : nbuf_id.p;
+            if (sp >= 2) KNNG_HIP(hipStreamWaitEvent(st, spev[4 * (sp - 2) + 3], 0));
+            KNNG_HIP(hipMemsetAsync(a.ncnt, 0, (size_t)n * 4, st));
+            KNNG_HIP(hipEventRecord(spev[4 * sp], st));
             launch_non_reverse(a, st);
-            KNNG_HIP(hipEventRecord(ev[3], st));
-            launch_non_merge(a, ovf.p, ovf.p + 1, ovf_flag.p, st);
-            KNNG_HIP(hipEventRecord(ev[4], st));
+            KNNG_HIP(hipEventRecord(spev[4 * sp + 1], st));
+            KNNG_HIP(hipStreamWaitEvent(sst, spev[4 * sp + 1], 0));
+            KNNG_HIP(hipEventRecord(spev[4 * sp + 2], sst));
+            launch_non_merge(a, ovf.p, ovf.p + 1, ovf_flag.p, sst);
+            KNNG_HIP(hipEventRecord(spev[4 * sp + 3], sst));
             KNNG_HIP(hipGetLastError());
-            KNNG_HIP(hipEventSynchronize(ev[4]));
-            KNNG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+        }
+        KNNG_HIP(hipStreamWaitEvent(st, spev[4 * (subs - 1) + 3], 0));
+        KNNG_HIP(hipEventSynchronize(spev[4 * (subs - 1) + 3]));
+        for (uint32_t sp = 0; sp < subs; ++sp) {
+            KNNG_HIP(hipEventElapsedTime(&ms, spev[4 * sp], spev[4 * sp + 1]));
             t_rev += ms;
-            KNNG_HIP(hipEventElapsedTime(&ms, ev[3], ev[4]));
+            KNNG_HIP(hipEventElapsedTime(&ms, spev[4 * sp + 2], spev[4 * sp + 3]));
             t_merge += ms;
         }
         uint32_t novf = 0;
@@ -1583,9 +1612,9 @@ double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_c
 uint64_t knng_job_footprint(uint32_t n, uint32_t d, const knng_params* params) {
     if (!params) return 0;
     const uint64_t ld = (d + 3) & ~3u, k = std::max(params->k, params->search_k);
-    // points copy, top-k ids/dists, outputs, two snapshots + reverse lists, NoN candidate buffers,
-    // per-point scalars (bounds, permutations, flags, offsets, counters, norms)
-    const uint64_t per_point = 4 * ld + 8 * k + 8 * k + 12 * k + 8 * Builder::kNonCap + 72;
+    // points copy, top-k ids/dists, outputs, two snapshots + reverse lists, two sets of NoN
+    // candidate buffers, per-point scalars (bounds, permutations, flags, offsets, counters, norms)
+    const uint64_t per_point = 4 * ld + 8 * k + 8 * k + 12 * k + 16 * Builder::kNonCap + 76;
     return (uint64_t)n * per_point + (64ull << 20);
 }
 

@@ -60,8 +60,8 @@ def test_server_admission_and_validation_without_gpu():
     f3 = K.job_footprint(1_000_000, 100, K.params(k=10))
     assert f2 > f1 > 0 and f3 < f2
     # per point: points (4 ld) + lists, outputs (8k each), two snapshots and reverse lists (12k)
-    # + NoN buffers
-    assert f2 - f1 == 900_000 * (4 * 100 + 28 * 100 + 8 * 512 + 72)
+    # + two sets of NoN buffers
+    assert f2 - f1 == 900_000 * (4 * 100 + 28 * 100 + 16 * 512 + 76)
     # a job larger than the budget is refused before anything is queued
     srv = K.Server(workers=2, hbm_budget=f1)
     X = np.zeros((1000, 100), np.float32)
