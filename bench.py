@@ -392,9 +392,10 @@ def main():
             relerr = np.concatenate(rels)
             del Xh, ds_all
             prec_block = {"precision": a.precision,
-                          "screen": {"fp32": "fp16 Gram with exact error margin",
-                                     "fp16": "fp16 Gram, no margin",
-                                     "bf16": "bf16 Gram, no margin"}[a.precision],
+                          "screen": {"fp32": "leaf: fp16 Gram with exact error margin",
+                                     "fp16": "leaf: fp16 Gram, no margin",
+                                     "bf16": "leaf: bf16 Gram, no margin"}[a.precision]
+                                    + "; neighbours of neighbours: fp32 Gram with exact error margin",
                           "rerank": "exact reference-order FP32 for every screen survivor",
                           "dist_abs_err_vs_fp64": {"max": float(err.max()), "mean": float(err.mean())},
                           "dist_rel_err_vs_fp64": {"max": float(relerr.max()),

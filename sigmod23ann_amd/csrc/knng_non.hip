@@ -367,25 +367,43 @@ __device__ inline float dist_ref_lane8_fixed(const float* __restrict__ a, const 
 }
 
 // FP32 Gram of a 16x16 tile over K = 4 KS dims, one accumulation chain (an MFMA that takes the
-// previous one's result as its accumulator issues back to back; the screen's error bound holds for
-// any summation order). KS is a compile-time constant for the common widths; KS = 0 is the generic
-// runtime loop.
+// previous one's result as its accumulator issues back to back). arow and brow are the lane's rows
+// (row lane & 15) in LDS; k-step s of the MFMA sums the products of lane group lk = lane >> 4. Any
+// assignment of the dims to (k-step, lane group) that takes each dim once gives the same Gram up to
+// rounding, and the screen's error bound holds for every summation order. With KS a compile-time
+// constant, group lk takes dims 16m + 4lk .. 16m + 4lk + 3 for m < KS / 4 (one 16-byte LDS read per
+// operand for four k-steps) and dim 16 (KS / 4) + 4s + lk for the last KS % 4 k-steps s; KS = 0 is
+// the generic runtime loop over dims 4s + lk.
 template <uint32_t KS>
 __device__ inline float4_t gram_tile_f32(const float* __restrict__ arow,
-                                         const float* __restrict__ brow, uint32_t ksteps) {
+                                         const float* __restrict__ brow, uint32_t lk, uint32_t ksteps) {
     float4_t c0 = {0.f, 0.f, 0.f, 0.f};
     if constexpr (KS > 0) {
-        float av[KS], bv[KS];
+        constexpr uint32_t M = KS / 4, R = KS % 4;
+        float4 av[M], bv[M];
+        float ar[R > 0 ? R : 1], br[R > 0 ? R : 1];
 #pragma unroll
-        for (uint32_t kk = 0; kk < KS; ++kk) {
-            av[kk] = arow[4 * kk];
-            bv[kk] = brow[4 * kk];
+        for (uint32_t m = 0; m < M; ++m) {
+            av[m] = *reinterpret_cast<const float4*>(arow + 16 * m + 4 * lk);
+            bv[m] = *reinterpret_cast<const float4*>(brow + 16 * m + 4 * lk);
         }
 #pragma unroll
-        for (uint32_t kk = 0; kk < KS; ++kk) c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], bv[kk], c0, 0, 0, 0);
+        for (uint32_t r = 0; r < R; ++r) {
+            ar[r] = arow[16 * M + 4 * r + lk];
+            br[r] = brow[16 * M + 4 * r + lk];
+        }
+#pragma unroll
+        for (uint32_t m = 0; m < M; ++m) {
+            c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[m].x, bv[m].x, c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[m].y, bv[m].y, c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[m].z, bv[m].z, c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[m].w, bv[m].w, c0, 0, 0, 0);
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[r], br[r], c0, 0, 0, 0);
     } else {
         for (uint32_t kk = 0; kk < ksteps; ++kk)
-            c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * kk], brow[4 * kk], c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * kk + lk], brow[4 * kk + lk], c0, 0, 0, 0);
     }
     return c0;
 }
@@ -741,16 +759,16 @@ void k_non_reverse(NonRevArgs a) {
                 pid[i] = s_aid[rr[i]];
                 thr0[i] = s_abnd[rr[i]] * (1.0f + 1.0f / 65536.0f);
             }
-            const float* arow = s_arow + (16 * strip + lrow) * ldp + lk;
+            const float* arow = s_arow + (16 * strip + lrow) * ldp;
             for (uint32_t j = wv; j < nstrip * tt; j += kNrThreads / 64) {
                 const uint32_t t = nstrip == 2 ? j >> 1 : j;
                 if (qn + 256 > kNrQueue) flush();
-                const float* brow = s_brow + (16 * t + lrow) * ldp + lk;
+                const float* brow = s_brow + (16 * t + lrow) * ldp;
                 // membership words of this tile's columns (c >> 5 = t >> 1 for the whole tile)
                 uint32_t mw[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) mw[i] = s_mask[rr[i] * 4 + (t >> 1)];
-                const float4_t acc = gram_tile_f32<KS>(arow, brow, ksteps);
+                const float4_t acc = gram_tile_f32<KS>(arow, brow, lk, ksteps);
                 const uint32_t c = 16 * t + lrow, cb = c & 31;
                 const bool cok = c < k;
                 const float cn = s_bnrm[c];  // zero past k
