@@ -58,6 +58,9 @@ def parse():
                     help="kmeans: SolutionKmeans' 2-means split; kd: SolutionRandomKD's "
                          "random-rotation split (the reference runs it with 5 rounds, leaves < 400, "
                          "no NoN: --splitter kd --rounds 5 --leaf 399 --non 0)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="leaf workgroup shape picked by timing both shapes on the first launch of "
+                         "each size bin (the warmup builds pay for it)")
     ap.add_argument("--seed", type=int, default=2023)
     ap.add_argument("--recall-queries", type=int, default=100_000)
     ap.add_argument("--cpu-sample", type=int, default=250_000)
@@ -110,7 +113,8 @@ def zipf_round(a, K):
     leaf = a.leaf if a.leaf != 400 else 4096
     ld = (d + 3) & ~3
     prm = K.params(k=k, leaf_max=leaf, small_max=3000, iterations=a.iterations, rounds=1,
-                   non_passes=0, precision=K.PREC_FP32, seed=a.seed)
+                   non_passes=0, precision=K.PREC_FP32, seed=a.seed,
+                   flags=K.FLAG_AUTOTUNE if a.autotune else 0)
     b = K.Builder(prm)
     log(f"[bench] generating {n}x{d} Zipf-mixture points (20,000 clusters) on the device")
     dX = K.DeviceArray(n * ld * 4)
@@ -284,7 +288,7 @@ def main():
         copy_calibration(a.copy_calibration_mib)
 
     prec = {"fp32": K.PREC_FP32, "fp16": K.PREC_FP16, "bf16": K.PREC_BF16}[a.precision]
-    flags = K.FLAG_KD_SPLIT if a.splitter == "kd" else 0
+    flags = (K.FLAG_KD_SPLIT if a.splitter == "kd" else 0) | (K.FLAG_AUTOTUNE if a.autotune else 0)
     prm = K.params(k=a.k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
                    rounds=a.rounds, non_passes=a.non, precision=prec, seed=a.seed, flags=flags)
     b = K.Builder(prm)
@@ -470,7 +474,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"low-rank {n}x{d} (24 latent, unit norm), k={k}, leaf<={a.leaf}",
                    "n": n, "d": d, "k": k, "rounds": a.rounds, "iterations": a.iterations,
-                   "non_passes": a.non, "precision": a.precision, "splitter": a.splitter,
+                   "non_passes": a.non, "precision": a.precision, "splitter": a.splitter, "autotune": a.autotune,
                    "recall_at_k": None if rec is None else round(rec, 4),
                    "recall_queries": nq, "parallelism": "single GPU"},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),

@@ -43,6 +43,10 @@ extern "C" {
 #define KNNG_FLAG_SMALL_NON_CAP 8u   /* tests: 16 neighbours-of-neighbours candidate slots per point
                                         instead of 512, so the overflow hand-off to the per-point
                                         direct kernel runs */
+#define KNNG_FLAG_AUTOTUNE 32u       /* leaf stage: the first launch of each size bin above 128
+                                        points (per device, precision and width, once per process)
+                                        also times both workgroup shapes, 8 and 16 waves, and the
+                                        faster is kept; the lists end as without the flag */
 
 /* Build parameters. The reference hard-codes k = dims = 100 (src/Constants.hpp:23-24),
  * maxGroupSize 200/400 alternating (src/SolutionKmeans.hpp:582-587), knnIterations = 1 (:591),

@@ -794,6 +794,7 @@ struct Builder {
         a.rerank_cap = prm.rerank;
         a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
         a.counters = counters.p;
+        a.tune = (prm.flags & KNNG_FLAG_AUTOTUNE) ? 1u : 0u;
         KNNG_HIP(hipEventRecord(e0, st));
         off = 0;
         for (auto& v : bin) {

@@ -97,6 +97,7 @@ struct LeafArgs {
     uint32_t half_kind;    // 0 = fp16, 1 = bf16 screening Gram
     uint32_t max_leaf;     // largest leaf size in this launch
     uint64_t* counters;    // [0] survivors screened, [1] exact distances, [2] inserts (nullable)
+    uint32_t tune;         // 1: pick the workgroup shape by timing both (KNNG_FLAG_AUTOTUNE)
 };
 void launch_leaf(const LeafArgs& a, hipStream_t st);
 

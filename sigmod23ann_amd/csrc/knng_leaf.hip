@@ -16,6 +16,10 @@
 #include "knng_device.hpp"
 #include "knng_kernels.hpp"
 
+#include <cstdio>
+#include <mutex>
+#include <unordered_map>
+
 namespace knng {
 
 // Waves per workgroup: 8, or 16 for buckets above 128 points whose LDS (one workgroup per CU
@@ -612,19 +616,77 @@ size_t leaf_lds_bytes(uint32_t max_leaf, uint32_t d, uint32_t ld) {
     return leaf_lds_layout(max_leaf, d, ld).total;
 }
 
+template <uint32_t HALF, uint32_t KS, uint32_t W>
+static void launch_leaf_w(const LeafArgs& a, size_t lds, hipStream_t st) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf<HALF, KS, W>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_leaf<HALF, KS, W><<<a.nleaves, W * 64, lds, st>>>(a);
+}
+
+// Workgroup-shape tuning (LeafArgs::tune). Bins above 128 points run as 8 or 16 waves per
+// workgroup, with the same lists either way. The first launch of a (device, precision, width, bin)
+// key does the real pass in the 16-wave shape, then one timed pass in each shape on the same
+// leaves. A repeated pass of the exact screen offers the same (id, distance) pairs again, which the
+// lists' id check rejects, so both timed passes see the same lists and leave them unchanged. The
+// faster shape is kept for the rest of the process, and the two times are reported on stderr.
+namespace {
+std::mutex g_tune_mu;
+std::unordered_map<uint64_t, uint32_t> g_tune;  // key -> waves per workgroup
+}  // namespace
+
 template <uint32_t HALF, uint32_t KS>
 static void launch_leaf_t(const LeafArgs& a, hipStream_t st) {
     const size_t lds8 = leaf_lds_bytes(a.max_leaf, a.d, a.ld);
     const size_t lds16 = leaf_lds_layout(a.max_leaf, a.d, a.ld, kLeafWavesLarge).total;
-    if (a.max_leaf > 128 && lds16 <= 160 * 1024) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf<HALF, KS, kLeafWavesLarge>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds16);
-        k_leaf<HALF, KS, kLeafWavesLarge><<<a.nleaves, kLeafWavesLarge * 64, lds16, st>>>(a);
-    } else {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf<HALF, KS, kLeafWavesSmall>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds8);
-        k_leaf<HALF, KS, kLeafWavesSmall><<<a.nleaves, kLeafWavesSmall * 64, lds8, st>>>(a);
+    if (!(a.max_leaf > 128 && lds16 <= 160 * 1024)) {
+        launch_leaf_w<HALF, KS, kLeafWavesSmall>(a, lds8, st);
+        return;
     }
+    if (!a.tune || a.screen != kScreenExact) {
+        launch_leaf_w<HALF, KS, kLeafWavesLarge>(a, lds16, st);
+        return;
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const uint32_t bin = a.max_leaf > 256 ? 1u : 0u;
+    const uint64_t key = ((uint64_t)(uint32_t)dev << 32) | (HALF << 16) | (KS << 8) | bin;
+    uint32_t w = 0;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        const auto it = g_tune.find(key);
+        if (it != g_tune.end()) w = it->second;
+    }
+    if (w == kLeafWavesSmall) {
+        launch_leaf_w<HALF, KS, kLeafWavesSmall>(a, lds8, st);
+        return;
+    }
+    launch_leaf_w<HALF, KS, kLeafWavesLarge>(a, lds16, st);
+    if (w) return;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool ok = true;
+    for (auto& e : ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    float t8 = 0.f, t16 = 0.f;
+    if (ok) {
+        LeafArgs b = a;
+        b.counters = nullptr;  // the statistics count the real pass only
+        ok = hipEventRecord(ev[0], st) == hipSuccess;
+        launch_leaf_w<HALF, KS, kLeafWavesSmall>(b, lds8, st);
+        ok = ok && hipEventRecord(ev[1], st) == hipSuccess;
+        launch_leaf_w<HALF, KS, kLeafWavesLarge>(b, lds16, st);
+        ok = ok && hipEventRecord(ev[2], st) == hipSuccess && hipEventSynchronize(ev[2]) == hipSuccess &&
+             hipEventElapsedTime(&t8, ev[0], ev[1]) == hipSuccess &&
+             hipEventElapsedTime(&t16, ev[1], ev[2]) == hipSuccess;
+    }
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (!ok) return;  // no decision: the next launch of this key times again
+    w = t8 < t16 ? kLeafWavesSmall : kLeafWavesLarge;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune.emplace(key, w);
+    }
+    std::fprintf(stderr, "[knng] leaf shape (%s, %u k-steps, leaves <= %u): 8 waves %.2f ms, 16 waves %.2f ms; %u kept\n",
+                 HALF ? "bf16" : "fp16", KS, bin ? 512u : 256u, t8, t16, w);
 }
 
 template <uint32_t HALF>

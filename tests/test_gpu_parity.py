@@ -213,6 +213,28 @@ def test_leaf_stage_lowrank_unit_norm():
     assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
 
 
+def test_leaf_stage_autotune_matches_default_shape():
+    # with KNNG_FLAG_AUTOTUNE the first launch of each bin above 128 points also times both
+    # workgroup shapes and keeps the faster; the lists must equal the default shape's, on the
+    # tuning call and on later calls that use the kept shape
+    X = lowrank(20000, 100, 8)
+    kw = dict(k=100, leaf_max=400, small_max=3000, iterations=1, seed=4)
+    perm, ls, lz = oracle.split_round(X, oracle.make_params(**kw), 0)
+    assert lz.max() > 256 and ((lz > 128) & (lz <= 256)).any()  # both tuned bins are launched
+    ref_ids, ref_ds = K.Builder(K.params(**kw)).leaf_stage(X, perm, ls, lz)
+    tuned = K.Builder(K.params(flags=K.FLAG_AUTOTUNE, **kw))
+    for _ in range(2):
+        ids, ds = tuned.leaf_stage(X, perm, ls, lz)
+        np.testing.assert_array_equal(ids, ref_ids)
+        np.testing.assert_array_equal(ds, ref_ds)
+    bkw = dict(k=50, leaf_max=400, small_max=3000, iterations=1, rounds=2, non_passes=1, seed=9)
+    Xb = X[:12000]
+    ids0, ds0 = K.Builder(K.params(**bkw)).build(Xb, return_dists=True)
+    ids1, ds1 = K.Builder(K.params(flags=K.FLAG_AUTOTUNE, **bkw)).build(Xb, return_dists=True)
+    np.testing.assert_array_equal(ids1, ids0)
+    np.testing.assert_array_equal(ds1, ds0)
+
+
 @pytest.mark.parametrize("flags", [0, K.FLAG_DIRECT_NON])
 def test_non_matches_oracle(flags):
     X = mixture(6000, 100, 30, 31)
