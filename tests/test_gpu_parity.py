@@ -452,3 +452,52 @@ def test_build_two_non_passes_matches_oracle():
     ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
     o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(**kw))
     assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+
+
+@pytest.mark.timeout(600)
+def test_full_size_lowrank_build_properties():
+    # The headline workload (10M x 100 low-rank unit-norm points generated on the device, k = 100,
+    # leaf <= 400, 6 rounds, 2 NoN passes), checked through properties that hold at any size:
+    # rows sorted by (dist, id) with distinct ids, no self and no padding; sampled rows' distances
+    # equal to the oracle's reference-order FP32 distance bit for bit; recall@100 against exact
+    # FP64 k-NN (the device kernel that test_exact_knn_gpu_matches_oracle pins) above 0.94.
+    n, d, k = 10_000_000, 100, 100
+    b = K.Builder(K.params(k=k, leaf_max=400, small_max=3000, iterations=1, rounds=6,
+                           non_passes=2, seed=2023))
+    dX = K.DeviceArray(n * d * 4)
+    b.generate_device(K.GEN_LOWRANK, n, d, d, 2023, dX.ptr)
+    dI = K.DeviceArray(n * k * 4)
+    dD = K.DeviceArray(n * k * 4)
+    b.build_device(dX.ptr, n, d, d, dI.ptr, dD.ptr)
+    ids = dI.download((n, k), np.uint32)
+    ds = dD.download((n, k), np.float32)
+    del dD
+    step = 1_000_000
+    for r0 in range(0, n, step):
+        i, s = ids[r0:r0 + step], ds[r0:r0 + step]
+        assert np.isfinite(s).all() and (i < n).all()
+        assert (s[:, 1:] >= s[:, :-1]).all()
+        tie = s[:, 1:] == s[:, :-1]
+        assert (i[:, 1:][tie] > i[:, :-1][tie]).all()
+        srt = np.sort(i, axis=1)
+        assert (srt[:, 1:] != srt[:, :-1]).all()
+        assert (i != np.arange(r0, r0 + i.shape[0], dtype=np.uint32)[:, None]).all()
+    X = dX.download((n, d), np.float32)
+    rng = np.random.default_rng(3)
+    for r in rng.choice(n, 200, replace=False):
+        got = np.array([oracle.dist(X[r], X[j]) for j in ids[r]], np.float32)
+        np.testing.assert_array_equal(ds[r], got)
+    q = np.sort(rng.choice(n, 4000, replace=False)).astype(np.uint32)
+    nq = q.size
+    dq = K.DeviceArray(nq * 4)
+    dq.upload(q)
+    dl = K.DeviceArray(nq * k * 4)
+    dl.upload(np.ascontiguousarray(ids[q]))
+    dub = K.DeviceArray(nq * 8)
+    b.exact_upper_from_lists(dX.ptr, n, d, d, dq.ptr, nq, dl.ptr, k, dub.ptr)
+    dti = K.DeviceArray(nq * k * 4)
+    dtd = K.DeviceArray(nq * k * 8)
+    b.exact_knn_device(dX.ptr, n, d, d, dq.ptr, nq, k, dub.ptr, 4 * k + 256, dti.ptr, dtd.ptr)
+    rec = K.recall(ids[q], dti.download((nq, k), np.uint32))
+    assert rec > 0.94, rec
+    del b, dX, dI
