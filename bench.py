@@ -482,6 +482,7 @@ def main():
         "stages_ms": {"split": round(st["split_ms"], 2), "side": round(st["side_ms"], 2),
                       "leaf": round(st["leaf_ms"], 2), "non": round(st["non_ms"], 2),
                       "total": round(st["total_ms"], 2)},
+        "non_overflow_points": int(st["non_overflow"]),
         "precision_form": prec_block,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -269,12 +269,12 @@ struct Builder {
     DevBuf<uint32_t> cand_id, qcount, ovf;
 
     // neighbours-of-neighbours scratch
-    // candidate slots per point before the direct path (41 GB at 10M points); with 6 sub-passes
-    // in the first pass about 1e-5 of the points overflow
+    // candidate slots per point before the direct path (41 GB at 10M points per set); the points
+    // that overflow them go through the direct kernel
     static constexpr uint32_t kNonCap = 512;
     static constexpr uint32_t kNonCapSmall = 16;       // KNNG_FLAG_SMALL_NON_CAP (tests)
-    static constexpr uint32_t kNonSubpasses = 6;       // first pass of a build
-    static constexpr uint32_t kNonSubpassesLater = 2;  // later passes: lists are close, buffers small
+    static constexpr uint32_t kNonSubpasses = 5;       // first pass of a build
+    static constexpr uint32_t kNonSubpassesLater = 1;  // later passes: lists are close, buffers small
     uint32_t non_cap() const { return (prm.flags & KNNG_FLAG_SMALL_NON_CAP) ? kNonCapSmall : kNonCap; }
     DevBuf<uint32_t> ovf_flag;
     DevBuf<uint32_t> rcnt, rfill, rsl, ncnt, nbuf_id;
@@ -926,8 +926,9 @@ struct Builder {
         // spread over all of them), each merged before the next: the bounds tighten as the pass
         // goes, which keeps the candidate buffers small.
         // a sub-pass merges every point's list once, so fewer of them save list traffic; the first
-        // pass needs more, to keep the candidate buffers within their capacity (at 10M points:
-        // 6 sub-passes and 512 slots take 100 ms less than 16 and 192, with the same screen time)
+        // pass needs more, to keep the candidate buffers within their capacity (at 10M points,
+        // with each merge beside the next screen: 5 + 1 sub-passes build in 1952 ms, 6 + 1 in
+        // 1955, 6 + 2 in 1964, 8 + 2 in 1968, 6 + 3 in 1978, means of 2-5 builds)
         const uint32_t subs = std::max(1u, std::min(pass_no ? kNonSubpassesLater : kNonSubpasses, n / 1024 + 1));
         // Sub-pass sp screens into candidate set sp % 2 on st; its merge runs on sst once the screen
         // is done, beside the next screen, which may read a bound before or after the merge lowers
