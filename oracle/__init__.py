@@ -43,6 +43,27 @@ class Params(C.Structure):
     ]
 
 
+class SplitTrace(C.Structure):
+    """One node's split decisions (the oracle's SplitTrace): the successful attempt, the start
+    sample rows and farthest pair (indices into ``rows``), the Lloyd sample size ``m``, the final
+    centres, the side-test hyperplane and the lo size."""
+    _fields_ = [
+        ("attempt", C.c_uint32),
+        ("s", C.c_uint32),
+        ("bi", C.c_uint32),
+        ("bj", C.c_uint32),
+        ("m", C.c_uint32),
+        ("nlo", C.c_uint32),
+        ("fallback", C.c_uint32),
+        ("pad_", C.c_uint32),
+        ("rows", C.c_uint32 * 256),
+        ("c1", C.c_float * 512),
+        ("c2", C.c_float * 512),
+        ("coef", C.c_float * 512),
+        ("offset", C.c_float),
+    ]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("split_s", C.c_double),
@@ -110,6 +131,12 @@ def lib():
         L.orc_contains.restype = C.c_int
         L.orc_split_round.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, _P(Params),
                                       C.c_uint32, _u32p, _u32p, _u32p, _u32p, C.c_int]
+        L.orc_farthest_pair.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, _u32p,
+                                        C.c_uint32, _u32p, _u32p]
+        L.orc_farthest_pair.restype = C.c_int
+        L.orc_split_root_trace.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, _P(Params),
+                                           C.c_uint32, _u32p, _P(SplitTrace), C.c_int]
+        L.orc_split_root_trace.restype = C.c_int
         L.orc_partition_injected.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, _u32p,
                                              C.c_uint32, _u32p, _u32p, _f32p, _f32p, _u32p]
         L.orc_hyperplane.argtypes = [_f32p, _f32p, C.c_uint32, _f32p]
@@ -237,6 +264,36 @@ def split_round(X, params: Params, round_idx: int, threads: int = 0):
     lib().orc_split_round(_fp(X), n, d, d, C.byref(params), round_idx, _up(perm), _up(ls),
                           _up(lz), C.byref(nl), threads)
     return perm, ls[: nl.value].copy(), lz[: nl.value].copy()
+
+
+def farthest_pair(X, rows):
+    """kmeansStartVecs' pair search (SolutionKmeans.hpp:76-99) over X[rows]: the first pair
+    (i, j), i < j, in loop order whose distance is the maximum and beyond FLT_MIN, as indices into
+    ``rows``; None when there is none (the reference dereferences null there)."""
+    X, n, d = _points(X)
+    r = np.ascontiguousarray(rows, np.uint32)
+    bi, bj = C.c_uint32(0), C.c_uint32(0)
+    ok = lib().orc_farthest_pair(_fp(X), n, d, d, _up(r), r.shape[0], C.byref(bi), C.byref(bj))
+    return (bi.value, bj.value) if ok else None
+
+
+def split_root_trace(X, params: Params, round_idx: int = 0, threads: int = 0):
+    """Splits the root node of a round once (SolutionKmeans.hpp:142-311) from the identity
+    permutation. Returns (perm, trace dict): attempt, start rows, (bi, bj), m, c1, c2, coef,
+    offset, nlo, fallback."""
+    X, n, d = _points(X)
+    perm = np.empty(n, np.uint32)
+    tr = SplitTrace()
+    rc = lib().orc_split_root_trace(_fp(X), n, d, d, C.byref(params), round_idx, _up(perm),
+                                    C.byref(tr), threads)
+    if rc != 0:
+        raise ValueError("split_root_trace needs 2 <= n and d <= 512")
+    t = dict(attempt=tr.attempt, s=tr.s, bi=tr.bi, bj=tr.bj, m=tr.m, nlo=tr.nlo,
+             fallback=bool(tr.fallback), offset=np.float32(tr.offset),
+             rows=np.array(tr.rows[: min(tr.s, 256)], np.uint32),
+             c1=np.array(tr.c1[:d], np.float32), c2=np.array(tr.c2[:d], np.float32),
+             coef=np.array(tr.coef[:d], np.float32))
+    return perm, t
 
 
 def kd_directions(seed: int, round_idx: int, m: int, d: int) -> np.ndarray:

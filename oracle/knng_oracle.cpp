@@ -269,6 +269,18 @@ struct SplitStats {
     uint64_t fallbacks = 0;
 };
 
+// What one node's split decided, for the known-answer tests of the split: the attempt that
+// succeeded, its start sample and farthest pair, the centres the last Lloyd step produced, the
+// hyperplane of the final side test and the lo size. Rows are stored only for d <= kTraceDims.
+constexpr uint32_t kTraceDims = 512;
+constexpr uint32_t kTraceRows = 256;
+struct SplitTrace {
+    uint32_t attempt, s, bi, bj, m, nlo, fallback, pad_;
+    uint32_t rows[kTraceRows];
+    float c1[kTraceDims], c2[kTraceDims], coef[kTraceDims];
+    float offset;
+};
+
 // kmeansStartVecs, SolutionKmeans.hpp:68-101 / :103-138: farthest pair among the s sampled rows,
 // first maximum in (i, j) order, starting from FLT_MIN with strict >. Returns false when no pair
 // exceeds FLT_MIN (the reference dereferences null there).
@@ -294,7 +306,7 @@ static bool farthest_pair(const Points& P, const std::vector<uint32_t>& rows, ui
 // placed first). `threads` parallelises the full-range side test and the sample sums.
 static uint32_t split_node(const Points& P, const Params& prm, uint32_t* perm, uint32_t start,
                            uint32_t n, uint64_t key, uint32_t threads, SplitStats& st,
-                           const Hyperplane* injected) {
+                           const Hyperplane* injected, SplitTrace* tr = nullptr) {
     const uint32_t d = P.d;
     const bool large = n >= prm.small_max;
     std::vector<uint8_t> flags(n);
@@ -314,6 +326,13 @@ static uint32_t split_node(const Points& P, const Params& prm, uint32_t* perm, u
             if (!farthest_pair(P, rows, bi, bj)) {
                 st.attempts_failed++;
                 continue;
+            }
+            if (tr) {
+                tr->s = s;
+                tr->m = m;
+                tr->bi = bi;
+                tr->bj = bj;
+                for (uint32_t i = 0; i < s && i < kTraceRows; ++i) tr->rows[i] = rows[i];
             }
             std::vector<float> c1(P.row(rows[bi]), P.row(rows[bi]) + d);
             std::vector<float> c2(P.row(rows[bj]), P.row(rows[bj]) + d);
@@ -377,6 +396,10 @@ static uint32_t split_node(const Points& P, const Params& prm, uint32_t* perm, u
                 st.attempts_failed++;
                 continue;
             }
+            if (tr && d <= kTraceDims) {
+                std::memcpy(tr->c1, c1.data(), d * sizeof(float));
+                std::memcpy(tr->c2, c2.data(), d * sizeof(float));
+            }
             h = make_hyperplane(c1, c2, d);
         }
         // Final side test over the full range (SolutionKmeans.hpp:189-197, :262-289).
@@ -393,6 +416,13 @@ static uint32_t split_node(const Points& P, const Params& prm, uint32_t* perm, u
             st.attempts_failed++;
             continue;
         }
+        if (tr) {
+            tr->attempt = attempt;
+            tr->nlo = nlo;
+            tr->fallback = 0;
+            tr->offset = h.offset;
+            if (d <= kTraceDims) std::memcpy(tr->coef, h.coef.data(), d * sizeof(float));
+        }
         std::vector<uint32_t> tmp(perm + start, perm + start + n);
         uint32_t a = 0, b = nlo;
         for (uint32_t i = 0; i < n; ++i) {
@@ -402,6 +432,11 @@ static uint32_t split_node(const Points& P, const Params& prm, uint32_t* perm, u
         return nlo;
     }
     st.fallbacks++;
+    if (tr) {
+        tr->attempt = kMaxAttempts;
+        tr->nlo = n / 2;
+        tr->fallback = 1;
+    }
     return n / 2;
 }
 
@@ -804,6 +839,35 @@ int orc_contains(const uint32_t* ids, uint32_t n, uint32_t cand) {
 }
 
 // --- Split ----------------------------------------------------------------------------------
+// kmeansStartVecs' pair search (SolutionKmeans.hpp:76-99) over the rows X[rows[0..s)]. Returns 0
+// when no pair lies beyond FLT_MIN.
+int orc_farthest_pair(const float* X, uint32_t n, uint32_t d, uint32_t ld, const uint32_t* rows,
+                      uint32_t s, uint32_t* bi, uint32_t* bj) {
+    Points P{X, n, d, ld};
+    std::vector<uint32_t> r(rows, rows + s);
+    *bi = 0;
+    *bj = 0;
+    return farthest_pair(P, r, *bi, *bj) ? 1 : 0;
+}
+
+// The root node of round `round` split once (splitKmeansBinaryProcess, SolutionKmeans.hpp:142-311)
+// from the identity permutation, recording what it decided. perm_out receives the partitioned
+// permutation. The node's branch (small or large) follows prm->small_max as in split_round.
+int orc_split_root_trace(const float* X, uint32_t n, uint32_t d, uint32_t ld,
+                         const orc_params* prm, uint32_t round, uint32_t* perm_out,
+                         SplitTrace* trace, int threads) {
+    if (d > kTraceDims || n < 2) return -1;
+    Points P{X, n, d, ld};
+    Params p{prm->k, prm->leaf_max, prm->leaf_max_odd, prm->small_max, prm->iterations,
+             prm->rounds, prm->non_passes, prm->seed, prm->non_rank_lo, prm->splitter};
+    for (uint32_t i = 0; i < n; ++i) perm_out[i] = i;
+    std::memset(trace, 0, sizeof(SplitTrace));
+    SplitStats st;
+    split_node(P, p, perm_out, 0, n, round_key(p.seed, round), pick_threads(threads), st, nullptr,
+               trace);
+    return 0;
+}
+
 int orc_split_round(const float* X, uint32_t n, uint32_t d, uint32_t ld, const orc_params* prm,
                     uint32_t round, uint32_t* perm_out, uint32_t* leaf_start, uint32_t* leaf_size,
                     uint32_t* n_leaves, int threads) {

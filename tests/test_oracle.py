@@ -305,6 +305,168 @@ def _mix64(z):
     return z ^ (z >> 31)
 
 
+# ----------------------------------------------------------------------------------------------
+# 2-means split decisions: start pair, Lloyd step, sample-only Lloyd
+# (SolutionKmeans.hpp:68-138, :161-186, :218-260)
+# ----------------------------------------------------------------------------------------------
+def test_farthest_pair_first_maximum_wins():
+    # Unit square corners: the diagonals (0,3) and (1,2) tie at distance 2; the first in (i, j)
+    # loop order wins (strict >, SolutionKmeans.hpp:88-93).
+    X = np.array([[0, 0], [1, 0], [0, 1], [1, 1]], np.float32)
+    assert oracle.farthest_pair(X, [0, 1, 2, 3]) == (0, 3)
+    assert oracle.farthest_pair(X, [3, 2, 1, 0]) == (0, 3)
+    assert oracle.farthest_pair(X, [1, 0, 3, 2]) == (0, 3)
+    # a repeated row: the pair of two copies never wins over a real distance
+    assert oracle.farthest_pair(X, [0, 0, 1]) == (0, 2)
+    # every distance 0: no pair beyond FLT_MIN (the reference would dereference null)
+    assert oracle.farthest_pair(X, [2, 2, 2, 2]) is None
+    # a subnormal distance (1e-40 < FLT_MIN) does not count either: the search starts at FLT_MIN
+    Y = np.array([[0.0, 0.0], [1e-20, 0.0]], np.float32)
+    assert 0.0 < oracle.dist(Y[0], Y[1]) < np.finfo(np.float32).tiny
+    assert oracle.farthest_pair(Y, [0, 1]) is None
+    # brute force on random rows, with a duplicated maximal pair planted later in the order
+    rng = np.random.default_rng(3)
+    Z = rng.normal(size=(40, 9)).astype(np.float32)
+    rows = list(rng.permutation(40)[:30])
+    best, arg = np.float32(np.finfo(np.float32).tiny), None
+    for i in range(len(rows)):
+        for j in range(i + 1, len(rows)):
+            dd = oracle.dist(Z[rows[i]], Z[rows[j]])
+            if dd > best:
+                best, arg = dd, (i, j)
+    assert oracle.farthest_pair(Z, rows) == arg
+    i, j = arg
+    assert oracle.farthest_pair(Z, rows + [rows[i], rows[j]]) == arg
+
+
+_PHI = 0x9E3779B97F4A7C15
+
+
+def _draw_below(akey, i, n):
+    h = _mix64(akey + _PHI * (i + 1))
+    return ((h >> 32) * n) >> 32
+
+
+def _attempt_key(seed, rnd, attempt):
+    key = _mix64(seed + _PHI * (rnd + 1))
+    return _mix64(key ^ ((0xD1B54A32D192ED03 * (attempt + 1)) & _M64))
+
+
+def _first_max_pair(X, rows):
+    best, arg = np.float32(np.finfo(np.float32).tiny), None
+    for i in range(len(rows)):
+        for j in range(i + 1, len(rows)):
+            dd = oracle.dist(X[rows[i]], X[rows[j]])
+            if dd > best:
+                best, arg = dd, (i, j)
+    return arg
+
+
+def _two_integer_clusters(rng, per, d, spread):
+    """Two clusters of 2*per points with integer centres a, b; offsets come in +e/-e pairs, so
+    each cluster's mean is its centre exactly. Points are interleaved: even rows in a."""
+    a = rng.integers(-10, 11, d).astype(np.float64)
+    b = a + rng.choice([-1, 1], d) * rng.integers(40, 60, d)
+    e = rng.integers(1, spread + 1, (per, d)) * rng.choice([-1, 1], (per, d))
+    A = np.concatenate([a + e, a - e])
+    B = np.concatenate([b + e[::-1], b - e[::-1]])
+    X = np.empty((4 * per, d), np.float32)
+    X[0::2] = A
+    X[1::2] = B
+    return X, a, b
+
+
+@pytest.mark.parametrize("iterations", [1, 2])
+def test_split_root_lloyd_step_closed_form(iterations):
+    # Small branch (n < small_max), full-range Lloyd with FP64 sums (SolutionKmeans.hpp:161-186).
+    # After one step the centres are the two cluster means exactly, so the hyperplane is
+    # coef = (m1 - m2) / 2, offset = dot((m1 + m2) / 2, (m1 - m2) / 2), with m1 the mean of the
+    # cluster holding the first start point; the lo child is that cluster, in index order.
+    d, seed = 24, 17
+    X, a, b = _two_integer_clusters(np.random.default_rng(11), 32, d, 3)
+    n = X.shape[0]
+    p = oracle.make_params(leaf_max=32, small_max=3000, iterations=iterations, seed=seed)
+    perm, t = oracle.split_root_trace(X, p, 0)
+    # the start sample is the first s draws of attempt 0 (SolutionKmeans.hpp:70-74, :103-110)
+    assert t["attempt"] == 0 and not t["fallback"]
+    s = oracle.start_count(n)
+    rows = [_draw_below(_attempt_key(seed, 0, 0), i, n) for i in range(s)]
+    np.testing.assert_array_equal(t["rows"], rows)
+    bi, bj = _first_max_pair(X, rows)
+    assert (t["bi"], t["bj"]) == (bi, bj)
+    ri, rj = rows[bi], rows[bj]
+    assert ri % 2 != rj % 2, "precondition: the start pair straddles the clusters"
+    m1, m2 = (a, b) if ri % 2 == 0 else (b, a)
+    np.testing.assert_array_equal(t["c1"], m1.astype(np.float32))
+    np.testing.assert_array_equal(t["c2"], m2.astype(np.float32))
+    np.testing.assert_array_equal(t["coef"], ((m1 - m2) / 2).astype(np.float32))
+    assert t["offset"] == np.float32(np.dot((m1 + m2) / 2, (m1 - m2) / 2))   # exact: quarter-integers
+    lo = np.arange(ri % 2, n, 2)
+    hi = np.arange(1 - ri % 2, n, 2)
+    assert t["nlo"] == lo.size
+    np.testing.assert_array_equal(perm, np.concatenate([lo, hi]))
+
+
+def test_split_root_start_pair_hyperplane_without_lloyd():
+    # iterations = 0: the hyperplane is the bisector of the farthest start pair itself.
+    rng = np.random.default_rng(5)
+    X = rng.integers(-8, 9, (700, 12)).astype(np.float32)
+    seed = 3
+    p = oracle.make_params(leaf_max=64, small_max=3000, iterations=0, seed=seed)
+    perm, t = oracle.split_root_trace(X, p, 2)
+    n = X.shape[0]
+    rows = [_draw_below(_attempt_key(seed, 2, t["attempt"]), i, n) for i in range(oracle.start_count(n))]
+    np.testing.assert_array_equal(t["rows"], rows)
+    bi, bj = _first_max_pair(X, rows)
+    assert (t["bi"], t["bj"]) == (bi, bj)
+    xa, xb = X[rows[bi]].astype(np.float64), X[rows[bj]].astype(np.float64)
+    np.testing.assert_array_equal(t["coef"], ((xa - xb) / 2).astype(np.float32))
+    assert t["offset"] == np.float32(np.dot((xa + xb) / 2, (xa - xb) / 2))
+
+
+def test_split_root_large_branch_lloyd_uses_the_sample_only():
+    # Large branch (n >= small_max): the Lloyd step sums only the m sample draws, with
+    # replacement (SolutionKmeans.hpp:218-249); the final side test covers the whole range
+    # (:262-289). Integer coordinates keep every FP64 sum and every start-plane dot exact, so the
+    # expected centres are computed here independently of the summation order.
+    rng = np.random.default_rng(21)
+    n, d, seed = 5000, 16, 8
+    X = rng.integers(-4, 5, (n, d)).astype(np.float32)
+    p = oracle.make_params(leaf_max=400, small_max=3000, iterations=1, seed=seed)
+    perm, t = oracle.split_root_trace(X, p, 0)
+    assert t["attempt"] == 0 and not t["fallback"]
+    m = oracle.sample_count(n)
+    assert t["m"] == m and m < n
+    ak = _attempt_key(seed, 0, 0)
+    draws = np.array([_draw_below(ak, i, n) for i in range(m)])
+    s = min(oracle.start_count(n), m)
+    np.testing.assert_array_equal(t["rows"], draws[:s])
+    bi, bj = _first_max_pair(X, list(draws[:s]))
+    assert (t["bi"], t["bj"]) == (bi, bj)
+    xa, xb = X[draws[bi]].astype(np.float64), X[draws[bj]].astype(np.float64)
+    coef0, off0 = (xa - xb) / 2, np.dot((xa + xb) / 2, (xa - xb) / 2)
+
+    def means(idx):
+        Y = X[idx].astype(np.float64)
+        side = Y @ coef0 >= off0
+        return ((Y[side].sum(0) / side.sum()).astype(np.float32),
+                (Y[~side].sum(0) / (~side).sum()).astype(np.float32))
+
+    c1, c2 = means(draws)
+    f1, f2 = means(np.arange(n))
+    assert not np.array_equal(c1, f1), "precondition: sample and full-range means differ"
+    np.testing.assert_array_equal(t["c1"], c1)
+    np.testing.assert_array_equal(t["c2"], c2)
+    between = np.float32(0.5) * (c1 + c2)
+    coef = c1 - between
+    np.testing.assert_array_equal(t["coef"], coef)
+    assert t["offset"] == oracle.dot(between, coef)
+    side = np.array([oracle.dot(coef, x) >= t["offset"] for x in X])
+    idx = np.arange(n)
+    np.testing.assert_array_equal(perm, np.concatenate([idx[side], idx[~side]]))
+    assert t["nlo"] == side.sum()
+
+
 def _kd_key(seed, rnd):
     rk = _mix64(seed + 0x9E3779B97F4A7C15 * (rnd + 1))
     return _mix64(rk ^ 0x6A09E667F3BCC909)
