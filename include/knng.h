@@ -181,13 +181,17 @@ int knng_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int knng_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int knng_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int knng_synchronize(void);
+/* Free and total device memory of the current device (hipMemGetInfo). */
+int knng_mem_get_info(size_t* free_bytes, size_t* total_bytes);
 
 /* ---- serving: continuous batching of build jobs ---------------------------------------------
  * No counterpart in the reference (a one-shot CLI, src/main.cpp:39-71). A server owns `workers`
  * builders, each with its own HIP stream and device buffers reused across jobs. Jobs are admitted
- * in arrival order as soon as a worker is free and the sum of the running jobs' device footprints
- * (knng_job_footprint) stays within hbm_budget bytes; admitted jobs run concurrently on the GPU.
- * A build is deterministic, so a job's result does not depend on what else is running. */
+ * in arrival order as soon as a worker is free and the device memory accounted to all workers
+ * stays within hbm_budget bytes: what each idle worker's builder keeps from earlier jobs, plus,
+ * for each running job, its footprint (knng_job_footprint) on top of what its worker kept. When
+ * the queue's head does not fit, idle workers free what they keep. Admitted jobs run concurrently
+ * on the GPU. A build is deterministic, so a job's result does not depend on what else runs. */
 typedef struct knng_server knng_server;
 
 typedef struct {
@@ -207,7 +211,19 @@ typedef struct {
     double latency_ms;          /* submit -> results in the caller's buffers */
     uint64_t footprint;         /* bytes the admission accounted for */
     uint32_t worker;
+    double start_ms;            /* when a worker took the job, ms since the server was created */
+    double done_ms;             /* when its results were in the caller's buffers, same clock */
+    uint64_t worker_device_bytes; /* device memory the worker kept after the job */
 } knng_job_result;
+
+typedef struct {
+    uint64_t budget;            /* hbm_budget of knng_server_create */
+    uint64_t in_use;            /* device bytes accounted now: what idle workers keep, plus
+                                   footprints of running jobs */
+    uint64_t peak_in_use;       /* largest in_use so far; never above budget */
+    uint64_t trims;             /* times an idle worker freed its memory for a job that did not fit */
+    uint64_t queued;            /* jobs waiting for admission */
+} knng_server_stats;
 
 /* Device bytes a job needs: points, top-k state, NoN scratch, outputs. */
 uint64_t knng_job_footprint(uint32_t n, uint32_t d, const knng_params* params);
@@ -218,6 +234,8 @@ int knng_server_submit(knng_server* s, const knng_job* job, uint64_t* job_id);
 int knng_server_poll(knng_server* s, uint64_t job_id, int* done);
 /* Blocks until the job is done, fills `out` (nullable) and forgets the job. */
 int knng_server_wait(knng_server* s, uint64_t job_id, knng_job_result* out);
+/* Admission accounting of the server (a snapshot). */
+int knng_server_get_stats(knng_server* s, knng_server_stats* out);
 /* Runs every queued job to completion, then stops the workers. */
 void knng_server_destroy(knng_server* s);
 

@@ -52,8 +52,8 @@ EXPORTS = [
     "knng_partition_injected", "knng_leaf_stage", "knng_non", "knng_exact_knn_device",
     "knng_exact_upper_from_lists", "knng_recall", "knng_generate_device", "knng_device_alloc",
     "knng_device_free", "knng_memcpy_h2d", "knng_memcpy_d2h", "knng_memcpy_d2d",
-    "knng_synchronize", "knng_job_footprint", "knng_server_create", "knng_server_submit",
-    "knng_server_poll", "knng_server_wait", "knng_server_destroy",
+    "knng_synchronize", "knng_mem_get_info", "knng_job_footprint", "knng_server_create", "knng_server_submit",
+    "knng_server_poll", "knng_server_wait", "knng_server_get_stats", "knng_server_destroy",
 ]
 
 
@@ -131,6 +131,22 @@ class JobResult(C.Structure):
         ("latency_ms", C.c_double),
         ("footprint", C.c_uint64),
         ("worker", C.c_uint32),
+        ("start_ms", C.c_double),
+        ("done_ms", C.c_double),
+        ("worker_device_bytes", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("budget", C.c_uint64),
+        ("in_use", C.c_uint64),
+        ("peak_in_use", C.c_uint64),
+        ("trims", C.c_uint64),
+        ("queued", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -193,6 +209,8 @@ def lib():
         L.knng_server_submit.argtypes = [_vp, _P(Job), _P(C.c_uint64)]
         L.knng_server_poll.argtypes = [_vp, C.c_uint64, _P(C.c_int)]
         L.knng_server_wait.argtypes = [_vp, C.c_uint64, _P(JobResult)]
+        L.knng_server_get_stats.argtypes = [_vp, _P(ServerStats)]
+        L.knng_mem_get_info.argtypes = [_P(C.c_size_t), _P(C.c_size_t)]
         L.knng_server_destroy.argtypes = [_vp]
         L.knng_server_destroy.restype = None
         _lib = L
@@ -380,6 +398,13 @@ class Builder:
         _check(lib().knng_generate_device(self.h, kind, n, d, row_stride, seed, d_points))
 
 
+def mem_get_info():
+    """(free, total) device bytes of the current device."""
+    f, t = C.c_size_t(), C.c_size_t()
+    _check(lib().knng_mem_get_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
 def recall(result_ids, truth_ids) -> float:
     """grade.cpp:61-95: mean |truth ∩ result| per row, divided by k."""
     r = np.ascontiguousarray(result_ids, np.uint32)
@@ -403,24 +428,29 @@ def job_footprint(n, d, prm: Params) -> int:
 
 
 class Ticket:
-    """A submitted job: its id and the buffers that must outlive it."""
+    """A submitted job's id. The server, not the ticket, holds the job's buffers (the host points
+    and the output arrays the worker writes), from submit until wait() returns or close() has
+    drained the job, so dropping a ticket early frees nothing the worker still uses."""
 
-    def __init__(self, jid, ids, dists, keep):
-        self.id, self.ids, self.dists, self._keep = jid, ids, dists, keep
+    def __init__(self, jid):
+        self.id = jid
 
 
 class Server:
     """Continuous batching of build jobs (knng_server_*): `workers` builders on their own HIP
-    streams, jobs admitted in arrival order while their device footprints fit `hbm_budget`."""
+    streams, jobs admitted in arrival order while the device memory accounted to the workers (what
+    each keeps between jobs plus the running jobs' footprints) fits `hbm_budget`."""
 
     def __init__(self, workers=4, hbm_budget=250 << 30):
         h = _vp()
         _check(lib().knng_server_create(workers, hbm_budget, C.byref(h)))
         self.h = h.value
+        self._jobs = {}   # job id -> (ids, dists, kept input); released by wait() or close()
 
     def submit(self, prm: Params, X=None, device_ptr=None, n=None, d=None, row_stride=None,
                return_dists=False) -> Ticket:
-        """Host points X (n, d), or device rows at device_ptr (n rows of row_stride floats)."""
+        """Host points X (n, d), or device rows at device_ptr (n rows of row_stride floats; the
+        caller keeps that memory valid until the job is waited for or the server closed)."""
         if X is not None:
             X, n, d = _points(X)
             ptr, on_dev, stride, keep = X.ctypes.data, 0, d, X
@@ -432,7 +462,8 @@ class Server:
                   ds.ctypes.data if ds is not None else None)
         jid = C.c_uint64()
         _check(lib().knng_server_submit(self.h, C.byref(job), C.byref(jid)))
-        return Ticket(jid.value, ids, ds, keep)
+        self._jobs[jid.value] = (ids, ds, keep)
+        return Ticket(jid.value)
 
     def done(self, t: Ticket) -> bool:
         flag = C.c_int()
@@ -442,13 +473,23 @@ class Server:
     def wait(self, t: Ticket):
         """(ids, dists or None, result dict); raises if the job failed."""
         r = JobResult()
-        _check(lib().knng_server_wait(self.h, t.id, C.byref(r)))
-        return t.ids, t.dists, r.as_dict()
+        code = lib().knng_server_wait(self.h, t.id, C.byref(r))
+        ids, ds, _ = self._jobs.pop(t.id, (None, None, None))
+        _check(code)
+        return ids, ds, r.as_dict()
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib().knng_server_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
 
     def close(self):
+        """Runs every submitted job to completion, stops the workers, then releases the buffers of
+        jobs nobody waited for."""
         if getattr(self, "h", None):
             lib().knng_server_destroy(self.h)
             self.h = None
+        self._jobs = {}
 
     def __del__(self):
         try:

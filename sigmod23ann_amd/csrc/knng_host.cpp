@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -57,31 +58,41 @@ int fail(int code, const std::string& msg) {
         if (r_ != KNNG_OK) return r_; \
     } while (0)
 
+// Device bytes held by the buffers a thread allocates, when the thread names a counter (a serving
+// worker does, so that admission sees what its builder keeps between jobs). A buffer remembers the
+// counter it was allocated under and gives its bytes back to it when freed.
+thread_local std::atomic<uint64_t>* tl_device_account = nullptr;
+
 // Grow-only device buffer.
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t cap = 0;
+    std::atomic<uint64_t>* acct = nullptr;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
+    ~DevBuf() { release(); }
     void swap(DevBuf& o) {
         std::swap(p, o.p);
         std::swap(cap, o.cap);
+        std::swap(acct, o.acct);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        if (acct) *acct -= cap * sizeof(T);
+        p = nullptr;
+        cap = 0;
+        acct = nullptr;
     }
     int ensure(size_t count) {
         if (count <= cap && p) return KNNG_OK;
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-            cap = 0;
-        }
+        release();
         size_t c = std::max<size_t>(count, 1);
         KNNG_HIP(hipMalloc(&p, c * sizeof(T)));
         cap = c;
+        acct = tl_device_account;
+        if (acct) *acct += cap * sizeof(T);
         return KNNG_OK;
     }
 };
@@ -1503,6 +1514,12 @@ int knng_generate_device(knng_handle* h, int kind, uint32_t n, uint32_t d, size_
     return KNNG_OK;
 }
 
+int knng_mem_get_info(size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(KNNG_ERR_INVALID, "null argument");
+    KNNG_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return KNNG_OK;
+}
+
 int knng_device_alloc(size_t bytes, void** out) {
     if (!out) return fail(KNNG_ERR_INVALID, "null out");
     KNNG_HIP(hipMalloc(out, std::max<size_t>(bytes, 1)));
@@ -1541,12 +1558,18 @@ struct ServerJob {
     bool done = false;
     int status = KNNG_OK;
     uint32_t worker = 0;
+    uint64_t held_after = 0;
 };
 
 }  // namespace knng
 
 struct knng_server {
-    uint64_t budget = 0, in_use = 0;
+    // Admission accounting, under mu. A worker's charge is the device memory its builder keeps
+    // between jobs; while it runs a job, that plus the job's footprint (grow-only buffers free the
+    // old allocation before taking a larger one, so a job's peak is at most the two together).
+    // in_use is the sum of the charges and never exceeds budget.
+    uint64_t budget = 0, in_use = 0, peak_in_use = 0, trims = 0;
+    std::vector<uint64_t> charge;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<knng::ServerJob*> queue;
@@ -1554,51 +1577,94 @@ struct knng_server {
     uint64_t next_id = 1;
     bool stop = false;
     std::vector<std::thread> workers;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 };
 
 namespace {
 
-// One worker: a builder (stream, device buffers) reused across jobs.
-void server_worker(knng_server* s, uint32_t w) {
-    knng_params p;
-    knng_default_params(&p);
+// What a serving worker keeps between jobs: a builder (stream, device buffers) and the device
+// outputs it downloads from.
+struct WorkerState {
     Builder b;
-    const int init_rc = b.init(p);
     knng::DevBuf<uint32_t> oid;
     knng::DevBuf<float> od;
+    int init_rc = KNNG_OK;
+    WorkerState() {
+        knng_params p;
+        knng_default_params(&p);
+        init_rc = b.init(p);
+    }
+};
+
+// One worker. It takes the queue's head job when the charges of all workers plus the job's
+// footprint fit the budget. When the head does not fit and the worker holds device memory, it
+// frees that memory first (a fresh builder), so memory kept by idle workers never blocks a job.
+void server_worker(knng_server* s, uint32_t w) {
+    std::atomic<uint64_t> held{0};  // device bytes this thread's buffers hold
+    knng::tl_device_account = &held;
+    auto ws = std::make_unique<WorkerState>();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->in_use += held - s->charge[w];
+        s->charge[w] = held;
+        s->peak_in_use = std::max(s->peak_in_use, s->in_use);
+    }
     for (;;) {
         knng::ServerJob* j = nullptr;
+        bool trim = false;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] {
-                return (s->stop && s->queue.empty()) ||
-                       (!s->queue.empty() && s->in_use + s->queue.front()->footprint <= s->budget);
+                if (s->queue.empty()) return s->stop;
+                return s->in_use + s->queue.front()->footprint <= s->budget || s->charge[w] > 0;
             });
             if (s->queue.empty()) return;  // stopping
-            j = s->queue.front();
-            s->queue.pop_front();
-            s->in_use += j->footprint;
-            j->t_start = std::chrono::steady_clock::now();
-            j->worker = w;
+            if (s->in_use + s->queue.front()->footprint <= s->budget) {
+                j = s->queue.front();
+                s->queue.pop_front();
+                s->charge[w] += j->footprint;
+                s->in_use += j->footprint;
+                s->peak_in_use = std::max(s->peak_in_use, s->in_use);
+                j->t_start = std::chrono::steady_clock::now();
+                j->worker = w;
+            } else {
+                trim = true;  // the charge stays counted until the memory is actually freed
+            }
         }
-        int rc = init_rc;
+        if (trim) {
+            ws.reset();
+            ws = std::make_unique<WorkerState>();
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->in_use = s->in_use - s->charge[w] + held;
+                s->charge[w] = held;
+                s->trims++;
+            }
+            s->cv_work.notify_all();
+            continue;
+        }
+        int rc = ws->init_rc;
         if (rc == KNNG_OK) {
+            Builder& b = ws->b;
             const knng_job& jb = j->job;
             const size_t tot = (size_t)jb.n * jb.params.k;
             b.prm = jb.params;
             rc = b.bind_points(jb.points, jb.points_on_device != 0, jb.n, jb.d, jb.row_stride);
-            if (rc == KNNG_OK) rc = oid.ensure(std::max<size_t>(tot, 1));
-            if (rc == KNNG_OK && jb.out_dists) rc = od.ensure(std::max<size_t>(tot, 1));
-            if (rc == KNNG_OK) rc = b.build(oid.p, jb.out_dists ? od.p : nullptr);
-            if (rc == KNNG_OK) rc = b.download(jb.out_ids, oid.p, tot * 4);
-            if (rc == KNNG_OK && jb.out_dists) rc = b.download(jb.out_dists, od.p, tot * 4);
+            if (rc == KNNG_OK) rc = ws->oid.ensure(std::max<size_t>(tot, 1));
+            if (rc == KNNG_OK && jb.out_dists) rc = ws->od.ensure(std::max<size_t>(tot, 1));
+            if (rc == KNNG_OK) rc = b.build(ws->oid.p, jb.out_dists ? ws->od.p : nullptr);
+            if (rc == KNNG_OK) rc = b.download(jb.out_ids, ws->oid.p, tot * 4);
+            if (rc == KNNG_OK && jb.out_dists) rc = b.download(jb.out_dists, ws->od.p, tot * 4);
         }
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            s->in_use -= j->footprint;
+            s->in_use = s->in_use - s->charge[w] + held;
+            s->charge[w] = held;
+            s->peak_in_use = std::max(s->peak_in_use, s->in_use);
             j->status = rc;
             j->done = true;
             j->t_done = std::chrono::steady_clock::now();
+            j->held_after = held;
         }
         s->cv_done.notify_all();
         s->cv_work.notify_all();
@@ -1614,16 +1680,35 @@ double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_c
 uint64_t knng_job_footprint(uint32_t n, uint32_t d, const knng_params* params) {
     if (!params) return 0;
     const uint64_t ld = (d + 3) & ~3u, k = std::max(params->k, params->search_k);
-    // points copy, top-k ids/dists, outputs, two snapshots + reverse lists, two sets of NoN
-    // candidate buffers, per-point scalars (bounds, permutations, flags, offsets, counters, norms)
-    const uint64_t per_point = 4 * ld + 8 * k + 8 * k + 12 * k + 16 * Builder::kNonCap + 76;
-    return (uint64_t)n * per_point + (64ull << 20);
+    const bool incremental = (params->flags & KNNG_FLAG_INCREMENTAL_NON) != 0;
+    const bool kd = (params->flags & KNNG_FLAG_KD_SPLIT) != 0;
+    // Per point: the points copy; top-k ids/dists; outputs; the snapshot, its reverse-list slots
+    // and the reverse lists (plus the previous snapshot in incremental passes); two sets of NoN
+    // candidate buffers; three permutations, side flags, bounds, norms, seven per-point counters,
+    // overflow list, reverse offsets; two sets of leaf descriptors (at most one leaf per point);
+    // projected rows for the KD splitter.
+    uint64_t per_point = 4 * ld + 8 * k + 8 * k + (incremental ? 16 : 12) * k +
+                         16 * (uint64_t)Builder::kNonCap + 12 + 1 + 4 + 4 + 28 + 4 + 8 + 16;
+    if (kd) per_point += 4 * (uint64_t)knng::kKdStride;
+    // Split tables of one level: at most one node per leaf_max + 1 points, each with its
+    // descriptor, three centre/hyperplane rows, an offset and two FP64 partial rows per chunk
+    // (one chunk per node plus one per 256 sample draws), and one tile per 1024 positions.
+    const uint64_t lm = std::max<uint32_t>(1, std::min(params->leaf_max, params->leaf_max_odd
+                                                                            ? params->leaf_max_odd
+                                                                            : params->leaf_max));
+    const uint64_t nodes = (uint64_t)n / (lm + 1) + 1;
+    const uint64_t node_bytes = sizeof(knng::NodeDesc) + 12 * ld + 4 + 2 * (16 * ld + 8 + sizeof(knng::ChunkDesc));
+    const uint64_t tiles = (uint64_t)n / 1024 + nodes + 1;
+    const uint64_t scan = ((uint64_t)n / 1024 + 2) * 8;
+    return (uint64_t)n * per_point + nodes * node_bytes + tiles * (sizeof(knng::TileDesc) + 8) + scan +
+           (64ull << 20);
 }
 
 int knng_server_create(uint32_t workers, uint64_t hbm_budget, knng_server** out) {
     if (!out || workers == 0 || workers > 64) return fail(KNNG_ERR_INVALID, "workers must be in [1, 64]");
     auto* s = new knng_server();
     s->budget = hbm_budget;
+    s->charge.assign(workers, 0);
     for (uint32_t w = 0; w < workers; ++w) s->workers.emplace_back(server_worker, s, w);
     *out = s;
     return KNNG_OK;
@@ -1680,8 +1765,22 @@ int knng_server_wait(knng_server* s, uint64_t job_id, knng_job_result* out) {
         out->latency_ms = ms_between(j->t_submit, j->t_done);
         out->footprint = j->footprint;
         out->worker = j->worker;
+        out->start_ms = ms_between(s->t0, j->t_start);
+        out->done_ms = ms_between(s->t0, j->t_done);
+        out->worker_device_bytes = j->held_after;
     }
     return j->status == KNNG_OK ? KNNG_OK : fail(j->status, "job failed");
+}
+
+int knng_server_get_stats(knng_server* s, knng_server_stats* out) {
+    if (!s || !out) return fail(KNNG_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    out->budget = s->budget;
+    out->in_use = s->in_use;
+    out->peak_in_use = s->peak_in_use;
+    out->trims = s->trims;
+    out->queued = s->queue.size();
+    return KNNG_OK;
 }
 
 void knng_server_destroy(knng_server* s) {

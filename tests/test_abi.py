@@ -59,9 +59,15 @@ def test_server_admission_and_validation_without_gpu():
     f2 = K.job_footprint(1_000_000, 100, prm)
     f3 = K.job_footprint(1_000_000, 100, K.params(k=10))
     assert f2 > f1 > 0 and f3 < f2
-    # per point: points (4 ld) + lists, outputs (8k each), two snapshots and reverse lists (12k)
-    # + two sets of NoN buffers
-    assert f2 - f1 == 900_000 * (4 * 100 + 28 * 100 + 16 * 512 + 76)
+    # per point: points (4 ld) + lists, outputs (8k each), the snapshot, its reverse-list slots
+    # and the reverse lists (12k) + two sets of NoN buffers, then per-point scalars and split
+    # tables (a few hundred bytes at most)
+    core = 4 * 100 + 28 * 100 + 16 * 512
+    assert 900_000 * core < f2 - f1 < 900_000 * (core + 200)
+    # the KD splitter's projected rows (32 floats) and the incremental passes' second snapshot
+    assert K.job_footprint(1_000_000, 100, K.params(k=100, flags=K.FLAG_KD_SPLIT)) - f2 == 128 * 1_000_000
+    assert (K.job_footprint(1_000_000, 100, K.params(k=100, flags=K.FLAG_INCREMENTAL_NON)) - f2
+            == 4 * 100 * 1_000_000)
     # a job larger than the budget is refused before anything is queued
     srv = K.Server(workers=2, hbm_budget=f1)
     X = np.zeros((1000, 100), np.float32)
@@ -74,3 +80,27 @@ def test_server_admission_and_validation_without_gpu():
     srv.close()
     h = ctypes.c_void_p()
     assert K.lib().knng_server_create(0, 1 << 30, ctypes.byref(h)) == K.ERR_INVALID
+
+
+def test_server_keeps_job_buffers_after_the_ticket_is_dropped():
+    # The worker writes a job's ids into the submitted arrays whenever it runs: the server, not
+    # the ticket, must keep them (and the host points) alive until the job is collected.
+    import gc
+    import weakref
+
+    import numpy as np
+    if not os.path.exists(K.LIB_PATH):
+        pytest.skip("libknng.so not built")
+    srv = K.Server(workers=1, hbm_budget=1 << 40)
+    X = np.random.default_rng(0).normal(size=(500, 16)).astype(np.float32)
+    t = srv.submit(K.params(k=10, leaf_max=64, rounds=1, non_passes=0), X=X)
+    jid = t.id
+    ids_ref = weakref.ref(srv._jobs[jid][0])
+    x_ref = weakref.ref(srv._jobs[jid][2])
+    del t, X
+    gc.collect()
+    assert ids_ref() is not None and x_ref() is not None
+    srv.close()                       # drains the job (it fails without a GPU), then lets go
+    assert srv._jobs == {}
+    gc.collect()
+    assert ids_ref() is None and x_ref() is None

@@ -29,16 +29,24 @@ def lowrank(n, d, seed, latent=24):
     return (X / np.linalg.norm(X, axis=1, keepdims=True)).astype(np.float32)
 
 
-def assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds):
-    """Same sorted distances; same ids wherever the distance is not tied with the k-th."""
+def assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds):
+    """Same sorted distances; same ids wherever the distance is not tied with the k-th. In a row
+    whose ids differ at the tied k-th distance, the row's ids are distinct, exclude the point
+    itself, and each tied id lies at exactly the tied distance (reference-order distance)."""
     ids = np.asarray(ids)
     o_ids = np.asarray(o_ids)
     np.testing.assert_array_equal(ds, o_ds)
     boundary = ds == ds[:, -1:]
     bad = (ids != o_ids) & ~boundary
     assert not bad.any(), f"{int(bad.sum())} ids differ away from ties, first rows {np.where(bad.any(1))[0][:5]}"
+    fmax = np.finfo(np.float32).max
     for r in np.where((ids != o_ids).any(1))[0]:
-        assert set(ids[r][boundary[r]]) <= set(range(ids.shape[0]))
+        valid = ds[r] < fmax
+        row = ids[r][valid]
+        assert len(set(row.tolist())) == row.size, f"row {r}: repeated id"
+        assert r not in row, f"row {r}: lists itself"
+        for j in ids[r][boundary[r] & valid]:
+            assert j < X.shape[0] and oracle.dist(X[r], X[j]) == ds[r, -1], f"row {r}: id {j} not at the tied distance"
 
 
 def assert_valid_lists(X, ids, ds):
@@ -151,12 +159,12 @@ def test_leaf_stage_matches_oracle(k, leaf, d):
     perm, ls, lz = oracle.split_round(X, o_prm, 0)
     ids, ds = b.leaf_stage(X, perm, ls, lz)
     o_ids, o_ds, o_sz = oracle.leaf_stage(X, k, perm, ls, lz)
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
     # second round merges into the existing sets
     perm2, ls2, lz2 = oracle.split_round(X, o_prm, 1)
     ids2, ds2 = b.leaf_stage(X, perm2, ls2, lz2, state=(ids, ds))
     o2 = oracle.leaf_stage(X, k, perm2, ls2, lz2, state=(o_ids, o_ds, o_sz))
-    assert_lists_equal_up_to_ties(ids2, ds2, o2[0], o2[1])
+    assert_lists_equal_up_to_ties(X, ids2, ds2, o2[0], o2[1])
 
 
 @pytest.mark.parametrize("k,leaf,d", [(100, 256, 100), (10, 400, 100), (32, 128, 64), (100, 400, 128)])
@@ -169,11 +177,11 @@ def test_leaf_stage_big_kernel_matches_oracle(k, leaf, d):
     perm, ls, lz = oracle.split_round(X, o_prm, 0)
     ids, ds = b.leaf_stage(X, perm, ls, lz)
     o_ids, o_ds, o_sz = oracle.leaf_stage(X, k, perm, ls, lz)
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
     perm2, ls2, lz2 = oracle.split_round(X, o_prm, 1)
     ids2, ds2 = b.leaf_stage(X, perm2, ls2, lz2, state=(ids, ds))
     o2 = oracle.leaf_stage(X, k, perm2, ls2, lz2, state=(o_ids, o_ds, o_sz))
-    assert_lists_equal_up_to_ties(ids2, ds2, o2[0], o2[1])
+    assert_lists_equal_up_to_ties(X, ids2, ds2, o2[0], o2[1])
 
 
 @pytest.mark.parametrize("d,leaf", [(100, 2000), (128, 1500), (37, 3000)])
@@ -187,11 +195,11 @@ def test_leaf_stage_large_buckets(d, leaf):
     b = K.Builder(K.params(**kw))
     ids, ds = b.leaf_stage(X, perm, ls, lz)
     o_ids, o_ds, o_sz = oracle.leaf_stage(X, 100, perm, ls, lz)
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
     perm2, ls2, lz2 = oracle.split_round(X, o_prm, 1)
     ids2, ds2 = b.leaf_stage(X, perm2, ls2, lz2, state=(ids, ds))
     o2 = oracle.leaf_stage(X, 100, perm2, ls2, lz2, state=(o_ids, o_ds, o_sz))
-    assert_lists_equal_up_to_ties(ids2, ds2, o2[0], o2[1])
+    assert_lists_equal_up_to_ties(X, ids2, ds2, o2[0], o2[1])
 
 
 def test_build_large_leaves_matches_oracle():
@@ -199,7 +207,7 @@ def test_build_large_leaves_matches_oracle():
     kw = dict(k=50, leaf_max=1500, small_max=1000, iterations=1, rounds=2, non_passes=1, seed=19)
     ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
     o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(**kw))
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
 def test_leaf_stage_lowrank_unit_norm():
@@ -210,7 +218,7 @@ def test_leaf_stage_lowrank_unit_norm():
     perm, ls, lz = oracle.split_round(X, o_prm, 0)
     ids, ds = K.Builder(K.params(**kw)).leaf_stage(X, perm, ls, lz)
     o_ids, o_ds, _ = oracle.leaf_stage(X, 100, perm, ls, lz)
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
 def test_leaf_stage_autotune_matches_default_shape():
@@ -245,7 +253,7 @@ def test_non_matches_oracle(flags):
     st = oracle.leaf_stage(X, k, perm, ls, lz)
     ids, ds = K.Builder(K.params(flags=flags, **kw)).non(X, (st[0], st[1]))
     o_ids, o_ds, _ = oracle.non(X, k, st)
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
 def test_non_overflow_path_matches_oracle():
@@ -261,7 +269,7 @@ def test_non_overflow_path_matches_oracle():
     ids, ds = b.non(X, (st[0], st[1]))
     assert b.stats()["non_overflow"] > 0
     o_ids, o_ds, _ = oracle.non(X, k, st)
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
 @pytest.mark.parametrize("flags,leaf", [(0, 100), (K.FLAG_DIRECT_NON, 100), (0, 15), (K.FLAG_DIRECT_NON, 15)])
@@ -276,7 +284,7 @@ def test_non_ordered_matches_oracle(flags, leaf):
     st = oracle.leaf_stage(X, k, perm, ls, lz)
     ids, ds = K.Builder(K.params(flags=flags, non_rank_lo=10, **kw)).non(X, (st[0], st[1]))
     o_ids, o_ds, _ = oracle.non(X, k, st, rank_lo=10)
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
 def test_build_ordered_non_matches_oracle():
@@ -285,7 +293,7 @@ def test_build_ordered_non_matches_oracle():
               non_rank_lo=50, seed=59)
     ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
     o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(**kw))
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
 def test_non_k100_lowrank():
@@ -299,7 +307,7 @@ def test_non_k100_lowrank():
     b = K.Builder(K.params(**kw))
     ids, ds = b.non(X, (st[0], st[1]))
     o_ids, o_ds, _ = oracle.non(X, k, st)
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
 def test_non_with_unfilled_lists():
@@ -313,7 +321,7 @@ def test_non_with_unfilled_lists():
     assert (st[2] < k).all()
     ids, ds = K.Builder(K.params(**kw)).non(X, (st[0], st[1]))
     o_ids, o_ds, _ = oracle.non(X, k, st)
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
 @pytest.mark.parametrize("k,rounds,non", [(20, 3, 1), (100, 2, 1), (10, 4, 0)])
@@ -322,7 +330,7 @@ def test_build_end_to_end_matches_oracle(k, rounds, non):
     kw = dict(k=k, leaf_max=256, small_max=3000, iterations=2, rounds=rounds, non_passes=non, seed=17)
     ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
     o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(**kw))
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
 @pytest.mark.parametrize("non", [0, 1])
@@ -331,7 +339,7 @@ def test_kd_build_matches_oracle(non):
     kw = dict(k=20, leaf_max=399, rounds=3, non_passes=non, seed=9)
     ids, ds = K.Builder(K.params(flags=K.FLAG_KD_SPLIT, **kw)).build(X, return_dists=True)
     o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(splitter=1, **kw))
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
 def test_build_reference_stride_and_padding():
@@ -345,7 +353,7 @@ def test_build_reference_stride_and_padding():
     K._check(K.lib().knng_build(b.h, K._f(Xs), 2000, 100, 112, K._u(ids), K._f(ds)))
     o_ids, o_ds, o_sz, _ = oracle.build_graph(X, oracle.make_params(k=30, leaf_max=10, small_max=200,
                                                                      rounds=1, seed=1))
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
     short = o_sz < 30
     assert short.any()
     for r in np.where(short)[0]:
@@ -451,7 +459,7 @@ def test_build_two_non_passes_matches_oracle():
     kw = dict(k=40, leaf_max=128, small_max=3000, iterations=1, rounds=2, non_passes=2, seed=41)
     ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
     o_ids, o_ds, _, _ = oracle.build_graph(X, oracle.make_params(**kw))
-    assert_lists_equal_up_to_ties(ids, ds, o_ids, o_ds)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
 @pytest.mark.timeout(600)
