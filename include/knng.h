@@ -139,6 +139,24 @@ int knng_leaf_stage(knng_handle* h, const float* points, uint32_t n, uint32_t d,
                     const uint32_t* leaf_size, uint32_t n_leaves, uint32_t* ids, float* dists,
                     int start_empty);
 
+/* No counterpart in the reference: knng_leaf_stage for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Points, permutation and lists are device memory the caller owns: d_points n rows of row_stride
+ * floats used in place (d and row_stride multiples of 4, d <= 128, 16-byte aligned), d_perm n ids,
+ * d_ids/d_dists n*w (w = max(k, search_k) of the handle's parameters) updated in place, or
+ * initialised first when start_empty; leaf_start/leaf_size are host arrays. Its scratch (the
+ * per-point bounds, the binned leaf tables and counters) is a fixed 64 MiB device array per GPU,
+ * loaded with the library and held until the process ends; no device or pinned memory is
+ * allocated or freed. Needs 4 bytes per point plus 8 per leaf and 12 per 64 rows of leaves above
+ * 512 points (rounded up to 256-byte blocks); a call that would need more returns
+ * KNNG_ERR_UNSUPPORTED. All calls on one GPU share the array: calls that overlap on one GPU race.
+ * Returns after the stage completed. */
+int knng_leaf_stage_device_noalloc(knng_handle* h, const float* d_points, uint32_t n, uint32_t d,
+                                   size_t row_stride, const uint32_t* d_perm,
+                                   const uint32_t* leaf_start, const uint32_t* leaf_size,
+                                   uint32_t n_leaves, uint32_t* d_ids, float* d_dists,
+                                   int start_empty);
+
 /* topUpSingle (src/SolutionKmeans.hpp:480-555) on a state as above, in place. */
 int knng_non(knng_handle* h, const float* points, uint32_t n, uint32_t d, size_t row_stride,
              uint32_t* ids, float* dists);

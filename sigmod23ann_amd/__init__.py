@@ -49,7 +49,7 @@ GEN_ZIPF = 2
 EXPORTS = [
     "knng_last_error", "knng_default_params", "knng_create", "knng_destroy", "knng_set_params",
     "knng_get_stats", "knng_build", "knng_build_device", "knng_split_round",
-    "knng_partition_injected", "knng_leaf_stage", "knng_non", "knng_exact_knn_device",
+    "knng_partition_injected", "knng_leaf_stage", "knng_leaf_stage_device_noalloc", "knng_non", "knng_exact_knn_device",
     "knng_exact_upper_from_lists", "knng_recall", "knng_generate_device", "knng_device_alloc",
     "knng_device_free", "knng_memcpy_h2d", "knng_memcpy_d2h", "knng_memcpy_d2d",
     "knng_synchronize", "knng_mem_get_info", "knng_job_footprint", "knng_server_create", "knng_server_submit",
@@ -190,6 +190,8 @@ def lib():
                                               _u32p, _f32p, _f32p, _u32p]
         L.knng_leaf_stage.argtypes = [_vp, _f32p, _u32, _u32, _sz, _u32p, _u32p, _u32p, _u32,
                                       _u32p, _f32p, C.c_int]
+        L.knng_leaf_stage_device_noalloc.argtypes = [_vp, _vp, _u32, _u32, _sz, _vp, _u32p, _u32p,
+                                                     _u32, _vp, _vp, C.c_int]
         L.knng_non.argtypes = [_vp, _f32p, _u32, _u32, _sz, _u32p, _f32p]
         L.knng_exact_knn_device.argtypes = [_vp, _vp, _u32, _u32, _sz, _vp, _u32, _u32, _vp,
                                             _u32, _vp, _vp]
@@ -360,6 +362,16 @@ class Builder:
         _check(lib().knng_leaf_stage(self.h, _f(X), n, d, d, _u(perm), _u(ls), _u(lz), ls.shape[0],
                                      _u(ids), _f(ds), empty))
         return ids, ds
+
+    def leaf_stage_device_noalloc(self, d_points, n, d, row_stride, d_perm, leaf_start,
+                                  leaf_size, d_ids, d_dists, start_empty):
+        """knng_leaf_stage on device points, permutation and lists, with the library's fixed
+        scratch instead of allocated buffers (leaf tables on the host)."""
+        ls = np.ascontiguousarray(leaf_start, np.uint32)
+        lz = np.ascontiguousarray(leaf_size, np.uint32)
+        _check(lib().knng_leaf_stage_device_noalloc(self.h, d_points, n, d, row_stride, d_perm,
+                                                    _u(ls), _u(lz), ls.shape[0], d_ids, d_dists,
+                                                    1 if start_empty else 0))
 
     def non(self, X, state):
         X, n, d = _points(X)

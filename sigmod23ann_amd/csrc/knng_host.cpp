@@ -742,77 +742,66 @@ struct Builder {
     // Leaf stage of one round on `st`, reading the round's permutation `pm` and its descriptors
     // through set `slot`. With sync, the host waits and the stage's time is added to the stats;
     // without, the caller times it from the events e0/e1 recorded around its kernels.
-    int leaf_stage(const std::vector<LeafDesc>& lv, const uint32_t* pm, uint32_t slot, bool sync,
-                   hipEvent_t e0, hipEvent_t e1) {
-        if (lv.empty()) {
-            KNNG_HIP(hipEventRecord(e0, st));
-            KNNG_HIP(hipEventRecord(e1, st));
-            return KNNG_OK;
-        }
-        if (d > 128) return fail(KNNG_ERR_UNSUPPORTED, "leaf stage supports d <= 128");
-        static constexpr uint32_t kEdges[4] = {64, 128, 256, kMaxLeaf};
+    // A leaf stage's launches: the leaves of each LDS size bin (in bin order) and the row blocks
+    // of the leaves that go to the big-bucket kernel.
+    struct LeafPlan {
         std::vector<LeafDesc> bin[4];
         std::vector<BigBlock> big;
-        uint64_t pairs = 0;
+        uint64_t pairs = 0, nleaves = 0;
+        size_t nl = 0;  // leaves over all bins
+    };
+
+    int plan_leaves(const std::vector<LeafDesc>& lv, LeafPlan& pl) const {
+        if (d > 128) return fail(KNNG_ERR_UNSUPPORTED, "leaf stage supports d <= 128");
+        static constexpr uint32_t kEdges[4] = {64, 128, 256, kMaxLeaf};
         const bool force_big = (prm.flags & KNNG_FLAG_BIG_LEAF) != 0;
+        pl.nleaves = lv.size();
         for (auto& l : lv) {
-            pairs += (uint64_t)l.size * (l.size - 1) / 2;
+            pl.pairs += (uint64_t)l.size * (l.size - 1) / 2;
             if (l.size < 2) continue;
             int bi = 0;
             while (bi < 4 && l.size > kEdges[bi]) ++bi;
             // a bin's LDS is sized by its largest leaf, so the test is on the leaf's own size
             if (force_big || (bi < 4 && leaf_lds_bytes(l.size, d, ld) > 160 * 1024)) bi = 4;
             if (bi < 4) {
-                bin[bi].push_back(l);
+                pl.bin[bi].push_back(l);
             } else {
-                for (uint32_t r0 = 0; r0 < l.size; r0 += kLeafBigRows) big.push_back({l.start, l.size, r0});
+                for (uint32_t r0 = 0; r0 < l.size; r0 += kLeafBigRows) pl.big.push_back({l.start, l.size, r0});
             }
         }
-        if (!big.empty() && leaf_big_lds_bytes(d, ld) > 160 * 1024)
+        if (!pl.big.empty() && leaf_big_lds_bytes(d, ld) > 160 * 1024)
             return fail(KNNG_ERR_UNSUPPORTED, "big-bucket leaf kernel does not fit LDS at this d");
-        size_t nl = 0;
-        for (auto& v : bin) nl += v.size();
-        // the pinned set's previous upload (two rounds ago) must have been read before it is refilled
-        KNNG_HIP(hipEventSynchronize(up_ev[slot]));
-        KNNG_TRY(h_leaves[slot].ensure(std::max<size_t>(nl, 1)));
-        KNNG_TRY(leaves[slot].ensure(std::max<size_t>(nl, 1)));
-        size_t off = 0;
-        for (auto& v : bin) {
-            if (!v.empty()) std::memcpy(h_leaves[slot].p + off, v.data(), v.size() * sizeof(LeafDesc));
-            off += v.size();
-        }
-        if (nl)
-            KNNG_HIP(hipMemcpyAsync(leaves[slot].p, h_leaves[slot].p, nl * sizeof(LeafDesc),
-                                    hipMemcpyHostToDevice, st));
-        if (!big.empty()) {
-            KNNG_TRY(h_big[slot].ensure(big.size()));
-            std::memcpy(h_big[slot].p, big.data(), big.size() * sizeof(BigBlock));
-            KNNG_TRY(bigblocks[slot].ensure(big.size()));
-            KNNG_HIP(hipMemcpyAsync(bigblocks[slot].p, h_big[slot].p, big.size() * sizeof(BigBlock),
-                                    hipMemcpyHostToDevice, st));
-        }
-        KNNG_HIP(hipEventRecord(up_ev[slot], st));
+        for (auto& v : pl.bin) pl.nl += v.size();
+        return KNNG_OK;
+    }
+
+    // Launches a planned leaf stage on stream st. The plan's bins sit back to back at d_leaves and
+    // its big-bucket blocks at d_big; ids/dists/bnd are the lists and their bounds; cnt the
+    // statistics counters. Events e0/e1 bracket the launches (timed into stats when sync).
+    int launch_leaf_plan(const LeafPlan& pl, const LeafDesc* d_leaves, const BigBlock* d_big,
+                         const uint32_t* pm, uint32_t* ids, float* dists, float* bnd, uint64_t* cnt,
+                         bool tune, bool sync, hipEvent_t e0, hipEvent_t e1) {
         LeafArgs a{};
         a.X = X;
         a.ld = ld;
         a.d = d;
         a.k = kl();
         a.perm = pm;
-        a.knn_ids = knn_ids.p;
-        a.knn_d = knn_d.p;
-        a.bound = bound.p;
+        a.knn_ids = ids;
+        a.knn_d = dists;
+        a.bound = bnd;
         a.screen = prm.precision == KNNG_PREC_FP32 ? kScreenExact : kScreenApprox;
         a.rerank_cap = prm.rerank;
         a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
-        a.counters = counters.p;
-        a.tune = (prm.flags & KNNG_FLAG_AUTOTUNE) ? 1u : 0u;
+        a.counters = cnt;
+        a.tune = tune ? 1u : 0u;
         KNNG_HIP(hipEventRecord(e0, st));
-        off = 0;
-        for (auto& v : bin) {
+        size_t off = 0;
+        for (auto& v : pl.bin) {
             if (!v.empty()) {
                 uint32_t mx = 0;
                 for (auto& l : v) mx = std::max(mx, l.size);
-                a.leaves = leaves[slot].p + off;
+                a.leaves = d_leaves + off;
                 a.nleaves = (uint32_t)v.size();
                 a.max_leaf = mx;
                 launch_leaf(a, st);
@@ -820,10 +809,10 @@ struct Builder {
             }
             off += v.size();
         }
-        if (!big.empty()) {
+        if (!pl.big.empty()) {
             a.leaves = nullptr;
             a.nleaves = 0;
-            launch_leaf_big(a, bigblocks[slot].p, (uint32_t)big.size(), st);
+            launch_leaf_big(a, d_big, (uint32_t)pl.big.size(), st);
             stats.leaf_launches++;
         }
         KNNG_HIP(hipGetLastError());
@@ -834,9 +823,43 @@ struct Builder {
             KNNG_HIP(hipEventElapsedTime(&ms, e0, e1));
             stats.leaf_ms += ms;
         }
-        stats.leaves += lv.size();
-        stats.leaf_pairs += pairs;
+        stats.leaves += pl.nleaves;
+        stats.leaf_pairs += pl.pairs;
         return KNNG_OK;
+    }
+
+    int leaf_stage(const std::vector<LeafDesc>& lv, const uint32_t* pm, uint32_t slot, bool sync,
+                   hipEvent_t e0, hipEvent_t e1) {
+        if (lv.empty()) {
+            KNNG_HIP(hipEventRecord(e0, st));
+            KNNG_HIP(hipEventRecord(e1, st));
+            return KNNG_OK;
+        }
+        LeafPlan pl;
+        KNNG_TRY(plan_leaves(lv, pl));
+        // the pinned set's previous upload (two rounds ago) must have been read before it is refilled
+        KNNG_HIP(hipEventSynchronize(up_ev[slot]));
+        KNNG_TRY(h_leaves[slot].ensure(std::max<size_t>(pl.nl, 1)));
+        KNNG_TRY(leaves[slot].ensure(std::max<size_t>(pl.nl, 1)));
+        size_t off = 0;
+        for (auto& v : pl.bin) {
+            if (!v.empty()) std::memcpy(h_leaves[slot].p + off, v.data(), v.size() * sizeof(LeafDesc));
+            off += v.size();
+        }
+        if (pl.nl)
+            KNNG_HIP(hipMemcpyAsync(leaves[slot].p, h_leaves[slot].p, pl.nl * sizeof(LeafDesc),
+                                    hipMemcpyHostToDevice, st));
+        if (!pl.big.empty()) {
+            KNNG_TRY(h_big[slot].ensure(pl.big.size()));
+            std::memcpy(h_big[slot].p, pl.big.data(), pl.big.size() * sizeof(BigBlock));
+            KNNG_TRY(bigblocks[slot].ensure(pl.big.size()));
+            KNNG_HIP(hipMemcpyAsync(bigblocks[slot].p, h_big[slot].p, pl.big.size() * sizeof(BigBlock),
+                                    hipMemcpyHostToDevice, st));
+        }
+        KNNG_HIP(hipEventRecord(up_ev[slot], st));
+        return launch_leaf_plan(pl, leaves[slot].p, bigblocks[slot].p, pm, knn_ids.p, knn_d.p,
+                                bound.p, counters.p, (prm.flags & KNNG_FLAG_AUTOTUNE) != 0, sync, e0,
+                                e1);
     }
 
     // Permutation of the last split round; processing order of the reverse NoN kernel.
@@ -1199,11 +1222,44 @@ void knng_default_params(knng_params* p) {
     p->seed = 12345;
 }
 
+// ---- device scratch of the no-allocation leaf stage ------------------------------------------
+// One array per GPU, loaded with the library's code object and held until the process ends. Host
+// code never uses the name `scratch` as an address: each device's copy is found once with
+// hipGetSymbolAddress (with that device current) and that address is passed to the kernels.
+// Calls that overlap on one GPU share this memory and race.
+static constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's scratch address, looked up on the first call for that device.
+int device_scratch(unsigned char** out) {
+    int dev = -1;
+    KNNG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(KNNG_ERR_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(KNNG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return KNNG_OK;
+}
+}  // namespace
+
 int knng_create(const knng_params* params, knng_handle** out) {
     if (!params || !out) return fail(KNNG_ERR_INVALID, "null argument");
     *out = nullptr;
     auto* h = new knng_handle();
     int r = h->b.init(*params);
+    unsigned char* sp = nullptr;
+    if (r == KNNG_OK) r = device_scratch(&sp);  // this device's scratch address, set up once
     if (r != KNNG_OK) {
         delete h;
         return r;
@@ -1348,6 +1404,70 @@ int knng_leaf_stage(knng_handle* h, const float* points, uint32_t n, uint32_t d,
     KNNG_HIP(hipMemsetAsync(b.counters.p, 0, 4 * sizeof(uint64_t), b.st));
     KNNG_TRY(b.leaf_stage(lv, b.perm.p, 0, true, b.ev[4], b.ev[5]));
     return download_state(b, ids, dists);
+}
+
+int knng_leaf_stage_device_noalloc(knng_handle* h, const float* d_points, uint32_t n, uint32_t d,
+                                   size_t row_stride, const uint32_t* d_perm,
+                                   const uint32_t* leaf_start, const uint32_t* leaf_size,
+                                   uint32_t n_leaves, uint32_t* d_ids, float* d_dists,
+                                   int start_empty) {
+    KNNG_TRY(check_handle(h));
+    if (!d_perm || !d_ids || !d_dists || (n && !d_points) || (n_leaves && (!leaf_start || !leaf_size)))
+        return fail(KNNG_ERR_INVALID, "null argument");
+    if (d < 1 || d > 128 || d % 4 != 0 || row_stride < d || row_stride % 4 != 0 ||
+        (reinterpret_cast<uintptr_t>(d_points) & 15) != 0)
+        return fail(KNNG_ERR_UNSUPPORTED,
+                    "the no-allocation leaf stage reads the rows in place: d a multiple of 4 up to "
+                    "128, row_stride a multiple of 4, 16-byte aligned rows");
+    Builder& b = h->b;
+    std::vector<knng::LeafDesc> lv(n_leaves);
+    for (uint32_t i = 0; i < n_leaves; ++i) {
+        if ((uint64_t)leaf_start[i] + leaf_size[i] > n) return fail(KNNG_ERR_INVALID, "leaf outside [0, n)");
+        if (leaf_size[i] > knng::kMaxLeafAny) return fail(KNNG_ERR_UNSUPPORTED, "leaf larger than 65536");
+        lv[i] = {leaf_start[i], leaf_size[i]};
+    }
+    KNNG_TRY(b.bind_points(d_points, true, n, d, row_stride));  // in place: allocates nothing
+    Builder::LeafPlan pl;
+    KNNG_TRY(b.plan_leaves(lv, pl));
+    // scratch layout: bounds (n floats), 4 counters, the binned leaves, the big-bucket blocks
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_cnt = up((size_t)n * sizeof(float));
+    const size_t o_leaves = o_cnt + up(4 * sizeof(uint64_t));
+    const size_t o_big = o_leaves + up(pl.nl * sizeof(knng::LeafDesc));
+    const size_t need = o_big + up(pl.big.size() * sizeof(knng::BigBlock));
+    if (need > kNoAllocScratchBytes)
+        return fail(KNNG_ERR_UNSUPPORTED,
+                    "the no-allocation leaf stage needs " + std::to_string(need) +
+                        " bytes of scratch (4 per point plus the leaf tables); it holds " +
+                        std::to_string(kNoAllocScratchBytes));
+    unsigned char* sp = nullptr;
+    KNNG_TRY(device_scratch(&sp));
+    float* bnd = reinterpret_cast<float*>(sp);
+    uint64_t* cnt = reinterpret_cast<uint64_t*>(sp + o_cnt);
+    knng::LeafDesc* dl = reinterpret_cast<knng::LeafDesc*>(sp + o_leaves);
+    knng::BigBlock* dbig = reinterpret_cast<knng::BigBlock*>(sp + o_big);
+    const uint32_t k = b.kl();
+    if (start_empty) {
+        knng::launch_init_knn(d_ids, d_dists, bnd, n, k, b.st);
+        knng::launch_zero_u64(cnt, 4, b.st);
+    } else {
+        knng::launch_bound_from_lists(d_dists, n, k, bnd, cnt, 4, b.st);
+    }
+    // descriptors from ordinary host memory; the stream is synchronised before they go out of scope
+    std::vector<knng::LeafDesc> flat;
+    flat.reserve(pl.nl);
+    for (auto& v : pl.bin) flat.insert(flat.end(), v.begin(), v.end());
+    if (!flat.empty())
+        KNNG_HIP(hipMemcpyAsync(dl, flat.data(), flat.size() * sizeof(knng::LeafDesc),
+                                hipMemcpyHostToDevice, b.st));
+    if (!pl.big.empty())
+        KNNG_HIP(hipMemcpyAsync(dbig, pl.big.data(), pl.big.size() * sizeof(knng::BigBlock),
+                                hipMemcpyHostToDevice, b.st));
+    const int rc = n_leaves ? b.launch_leaf_plan(pl, dl, dbig, d_perm, d_ids, d_dists, bnd, cnt,
+                                                 false, true, b.ev[4], b.ev[5])
+                            : KNNG_OK;
+    KNNG_HIP(hipStreamSynchronize(b.st));
+    return rc;
 }
 
 int knng_non(knng_handle* h, const float* points, uint32_t n, uint32_t d, size_t row_stride,

@@ -110,6 +110,10 @@ void launch_leaf_big(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks
 
 void launch_init_knn(uint32_t* ids, float* dists, float* bound, uint32_t n, uint32_t k,
                      hipStream_t st);
+// bound[p] = dists[p * k + k - 1] for p < n, and counters[0, ncounters) = 0.
+void launch_bound_from_lists(const float* dists, uint32_t n, uint32_t k, float* bound,
+                             uint64_t* counters, uint32_t ncounters, hipStream_t st);
+void launch_zero_u64(uint64_t* p, uint32_t count, hipStream_t st);
 // First k of each width-kl list -> out (padding id 1 for empty slots), dists -> out_d (nullable).
 void launch_finalize(const uint32_t* ids, const float* dists, uint32_t n, uint32_t kl, uint32_t k,
                      uint32_t* out_ids, float* out_d, hipStream_t st);

@@ -725,6 +725,30 @@ void launch_init_knn(uint32_t* ids, float* dists, float* bound, uint32_t n, uint
     if (tot) k_init_knn<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(ids, dists, bound, n, k);
 }
 
+// bound[p] = the k-th distance of p's sorted list; and the statistics counters set to zero.
+__global__ void k_bound_from_lists(const float* dists, uint32_t n, uint32_t k, float* bound,
+                                   uint64_t* counters, uint32_t ncounters) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) bound[i] = dists[i * k + k - 1];
+    if (i < ncounters) counters[i] = 0;
+}
+
+__global__ void k_zero_u64(uint64_t* p, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
+}
+
+void launch_bound_from_lists(const float* dists, uint32_t n, uint32_t k, float* bound,
+                             uint64_t* counters, uint32_t ncounters, hipStream_t st) {
+    const size_t tot = std::max<size_t>(n, ncounters);
+    if (tot) k_bound_from_lists<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(dists, n, k, bound,
+                                                                               counters, ncounters);
+}
+
+void launch_zero_u64(uint64_t* p, uint32_t count, hipStream_t st) {
+    if (count) k_zero_u64<<<(count + 255) / 256, 256, 0, st>>>(p, count);
+}
+
 // finalize + padResult (KnnSets.hpp:113-126, :334-344): lists are already sorted by (dist, id);
 // empty slots become id 1.
 // Lists of width kl -> outputs of width k <= kl (each list's first k: the k nearest found).
