@@ -58,6 +58,9 @@ def parse():
                     help="kmeans: SolutionKmeans' 2-means split; kd: SolutionRandomKD's "
                          "random-rotation split (the reference runs it with 5 rounds, leaves < 400, "
                          "no NoN: --splitter kd --rounds 5 --leaf 399 --non 0)")
+    ap.add_argument("--non-screen16", action="store_true",
+                    help="neighbours-of-neighbours screen on a 16-bit Gram of rows centred on each "
+                         "q instead of the FP32 Gram of the rows")
     ap.add_argument("--autotune", action="store_true",
                     help="leaf workgroup shape picked by timing both shapes on the first launch of "
                          "each size bin (the warmup builds pay for it)")
@@ -288,7 +291,8 @@ def main():
         copy_calibration(a.copy_calibration_mib)
 
     prec = {"fp32": K.PREC_FP32, "fp16": K.PREC_FP16, "bf16": K.PREC_BF16}[a.precision]
-    flags = (K.FLAG_KD_SPLIT if a.splitter == "kd" else 0) | (K.FLAG_AUTOTUNE if a.autotune else 0)
+    flags = ((K.FLAG_KD_SPLIT if a.splitter == "kd" else 0) | (K.FLAG_AUTOTUNE if a.autotune else 0)
+             | (K.FLAG_NON_SCREEN16 if a.non_screen16 else 0))
     prm = K.params(k=a.k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
                    rounds=a.rounds, non_passes=a.non, precision=prec, seed=a.seed, flags=flags)
     b = K.Builder(prm)

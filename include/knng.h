@@ -43,6 +43,12 @@ extern "C" {
 #define KNNG_FLAG_SMALL_NON_CAP 8u   /* tests: 16 neighbours-of-neighbours candidate slots per point
                                         instead of 512, so the overflow hand-off to the per-point
                                         direct kernel runs */
+#define KNNG_FLAG_NON_SCREEN16 64u   /* neighbours-of-neighbours screen on a 16-bit Gram (16x16x32
+                                        MFMA) of rows centred on each q instead of the FP32 Gram
+                                        (d of 61-64, 97-100 or 125-128; fp16 with the exact margin
+                                        in the FP32 form, so the same lists; bf16/fp16 without a
+                                        margin in those forms). Measured slower at 10M x 100: the
+                                        exact re-rank then reads S(q)'s rows from global memory */
 #define KNNG_FLAG_AUTOTUNE 32u       /* leaf stage: the first launch of each size bin above 128
                                         points (per device, precision and width, once per process)
                                         also times both workgroup shapes, 8 and 16 waves, and the

@@ -873,7 +873,10 @@ struct Builder {
         // later passes of a build screen only offers with a new hop (not in the ordered form: an
         // old entry can cross the rank threshold and be expanded for the first time)
         const bool incremental = have_prev && (prm.flags & KNNG_FLAG_INCREMENTAL_NON) && prm.non_rank_lo == 0;
-        const bool reverse = ld <= 128 && non_reverse_lds(k, ld) <= 160 * 1024 &&
+        const uint32_t screen16 = non_screen16_supported(d) && (prm.flags & KNNG_FLAG_NON_SCREEN16)
+                                      ? (prm.precision == KNNG_PREC_BF16 ? 2u : 1u)
+                                      : 0u;
+        const bool reverse = ld <= 128 && non_reverse_lds(k, ld, d, screen16) <= 160 * 1024 &&
                              !(prm.flags & KNNG_FLAG_DIRECT_NON);
         if (reverse) {  // the snapshot kernel also counts the reverse lists (and, in full
                         // passes, gives each entry its slot)
@@ -952,6 +955,8 @@ struct Builder {
         a.nbuf_id = nbuf_id.p;
         a.cap = non_cap();
         a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
+        a.screen16 = screen16;
+        a.exact = prm.precision == KNNG_PREC_FP32 ? 1u : 0u;
         a.counters = verbose ? counters.p : nullptr;  // same-address atomics: diagnostics only
         KNNG_TRY(norms32.ensure(n));
         launch_row_norms(X, ld, d, n, norms32.p, st);

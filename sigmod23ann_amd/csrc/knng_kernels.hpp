@@ -145,6 +145,8 @@ struct NonRevArgs {
     uint32_t* nbuf_id;
     uint32_t cap;
     uint32_t half_kind;
+    uint32_t screen16;      // 0: FP32 Gram of the rows; 1: fp16, 2: bf16 Gram of rows centred on q
+    uint32_t exact;         // 16-bit screens: 1 = rigorous error margin, 0 = plain approximate test
     const float* norms;     // n FP32 squared norms
     uint64_t* counters;     // [0] screened pairs, [1] appended (nullable)
 };
@@ -169,7 +171,9 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint64_t* out, uint64
 void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t* rcnt,
                           uint32_t* rnew, uint64_t* roff, uint64_t* scratch, uint32_t* rfill,
                           uint32_t* rfill_old, const uint32_t* rslot, uint32_t* rs, hipStream_t st);
-size_t non_reverse_lds(uint32_t k, uint32_t ld);
+size_t non_reverse_lds(uint32_t k, uint32_t ld, uint32_t d, uint32_t screen16);
+// The reverse kernel's 16-bit centred screens cover d with 4 ceil(d / 4) in {64, 100, 128}.
+bool non_screen16_supported(uint32_t d);
 void launch_non_reverse(const NonRevArgs& a, hipStream_t st);
 void launch_non_merge(const NonRevArgs& a, uint32_t* ovf_cnt, uint32_t* ovf_list, uint32_t* ovf_flag,
                       hipStream_t st);

@@ -276,29 +276,51 @@ constexpr uint32_t kNrOffSpid = (kNrOffQueue + (kNrThreads / 64) * kNrQueue * 2 
 constexpr uint32_t kNrOffSrid = kNrOffSpid + kNrSurv * 4;            // accepted offers
 constexpr uint32_t kNrOffSd = kNrOffSrid + kNrSurv * 4;
 constexpr uint32_t kNrOffCnt = kNrOffSd + kNrSurv * 4;               // [0] accepted, [1] screened, [2] new
-constexpr uint32_t kNrFixed = (kNrOffCnt + 16 + 15) & ~15u;
+constexpr uint32_t kNrOffXq = (kNrOffCnt + 16 + 15) & ~15u;         // q's FP32 row (16-bit screens)
+constexpr uint32_t kNrFixed = kNrOffXq + 128 * 4;
 static_assert(kNrOffMeta % 8 == 0, "s_meta holds 64-bit words");
 
 struct NrLds {
-    uint32_t kb16, ldp;
-    uint32_t off_bid, off_bnrm, off_brow, off_arow, total;
+    uint32_t kb16, ldp, k32, rs16;
+    uint32_t off_bid, off_bnrm, off_brow, off_arow, off_brinv, off_b16, off_a16, off_arinv, total;
 };
 
-__host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld) {
+// FP32 Gram (screen16 = 0): S(q)'s FP32 rows sit in LDS for the Gram and the exact re-rank.
+// 16-bit screens: S(q)'s rows sit in LDS only as 16-bit rows centred on q (the exact re-rank reads
+// their FP32 rows from global memory), beside the chunk's FP32 and 16-bit rows, so two workgroups
+// still share a CU.
+__host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld, uint32_t d, uint32_t screen16) {
     NrLds L;
     L.kb16 = (k + 15) & ~15u;
     L.ldp = nr_row_stride(ld);
+    L.k32 = (d + 31) / 32;
+    L.rs16 = 32 * L.k32 + 8;  // halves; +8 staggers the banks of consecutive rows
     uint32_t o = kNrFixed;
     L.off_bid = o;  o += L.kb16 * 4;
     L.off_bnrm = o; o += L.kb16 * 4;
+    L.off_brinv = o; o += screen16 ? L.kb16 * 4 : 0;
     o = (o + 15) & ~15u;
-    L.off_brow = o; o += L.kb16 * L.ldp * 4;
+    L.off_brow = o; o += screen16 ? 0 : L.kb16 * L.ldp * 4;
+    L.off_b16 = o;  o += screen16 ? L.kb16 * L.rs16 * 2 : 0;
+    o = (o + 15) & ~15u;
     L.off_arow = o; o += kNrChunk * L.ldp * 4;
+    L.off_a16 = o;  o += screen16 ? kNrChunk * L.rs16 * 2 : 0;
+    L.off_arinv = o; o += screen16 ? kNrChunk * 4 : 0;
+    // 16-bit screens stage S(q)'s FP32 rows from off_b16 on, over the 16-bit and chunk regions
+    if (screen16) o = o > L.off_b16 + L.kb16 * L.ldp * 4 ? o : L.off_b16 + L.kb16 * L.ldp * 4;
     L.total = (o + 15) & ~15u;
     return L;
 }
 
-size_t non_reverse_lds(uint32_t k, uint32_t ld) { return nr_layout(k, ld).total; }
+size_t non_reverse_lds(uint32_t k, uint32_t ld, uint32_t d, uint32_t screen16) {
+    return nr_layout(k, ld, d, screen16).total;
+}
+// For these widths the 16-bit rows (rs16 = 32 ceil(d / 32) + 8 halves) are at most as long in
+// bytes as half an FP32 row (rs16 <= 2 ldp), which the in-place conversion of the staged rows needs.
+bool non_screen16_supported(uint32_t d) {
+    const uint32_t ks = (d + 3) / 4;
+    return ks == 16 || ks == 25 || ks == 32;
+}
 
 // Membership hash of an id: h = id * odd constant is a bijection of 32-bit words, so the bucket
 // (h's top 7 bits) and h's low 25 bits identify the id. An entry holds (h << 7) | column (column
@@ -408,9 +430,91 @@ __device__ inline float4_t gram_tile_f32(const float* __restrict__ arow,
     return c0;
 }
 
-template <uint32_t KS>
+// ---- 16-bit screens centred on q -----------------------------------------------------------------
+// A row's 16 threads (one DPP row of a wave) each hold 8 of its dims: 4 sc .. 4 sc + 3 and
+// 64 + 4 sc .. 64 + 4 sc + 3. They centre their dims on q's row in FP32, reduce the row's largest
+// |value| and its squared norm over the 16 lanes (DPP row rotations: all lanes end with the
+// result), scale the row by a power of two so that its largest |value| is below 2^14 (as the leaf
+// stage's rows), and store it as 16-bit values. Relative rounding per element therefore holds for
+// every element of the row, which is what the screen's margin assumes.
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+constexpr uint32_t kNrSPass = 4;  // S(q) rows converted 32 at a time: k <= 128
+
+template <int CTRL>
+__device__ inline float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// row_ror:8, 4, 2, 1 (DPP controls 0x128, 0x124, 0x122, 0x121)
+__device__ inline float row16_max(float v) {
+    v = fmaxf(v, dpp_f<0x128>(v));
+    v = fmaxf(v, dpp_f<0x124>(v));
+    v = fmaxf(v, dpp_f<0x122>(v));
+    return fmaxf(v, dpp_f<0x121>(v));
+}
+__device__ inline float row16_sum(float v) {
+    v = v + dpp_f<0x128>(v);
+    v = v + dpp_f<0x124>(v);
+    v = v + dpp_f<0x122>(v);
+    return v + dpp_f<0x121>(v);
+}
+
+template <uint32_t SCR>
+__device__ inline uint16_t to16(float f) {
+    if constexpr (SCR == 1) return __builtin_bit_cast(uint16_t, (_Float16)f);
+    else return to_bf16_rne(f);
+}
+
+// Centres (v0, v1) on (x0, x1), the thread's dims of a row and of q's row; `valid` false gives a zero
+// row. Writes the 16-bit dims at row + 4 sc and row + 64 + 4 sc (below rs16 - 8 only), returns the
+// FP32 squared norm of the centred row and the inverse of its scale. All 16 threads of the row
+// must call it together.
+template <uint32_t SCR>
+__device__ inline void centre_row16(float4 v0, float4 v1, float4 x0, float4 x1, bool valid,
+                                    uint32_t sc, uint32_t d, uint32_t rs16,
+                                    uint16_t* __restrict__ row, float& nrm, float& rinv) {
+    float c[8] = {v0.x - x0.x, v0.y - x0.y, v0.z - x0.z, v0.w - x0.w,
+                  v1.x - x1.x, v1.y - x1.y, v1.z - x1.z, v1.w - x1.w};
+    float m = 0.f, s2 = 0.f;
+#pragma unroll
+    for (uint32_t e = 0; e < 8; ++e) {
+        const uint32_t dim = (e < 4 ? 4 * sc : 64 + 4 * sc) + (e & 3);
+        if (!valid || dim >= d) c[e] = 0.f;
+        m = fmaxf(m, fabsf(c[e]));
+        s2 = __builtin_fmaf(c[e], c[e], s2);
+    }
+    m = row16_max(m);
+    nrm = row16_sum(s2);
+    int ex = 0;
+    if (m > 0.f) frexpf(m, &ex);
+    const int sh = 14 - ex;
+    const float scl = ldexpf(1.0f, sh);
+    rinv = ldexpf(1.0f, -sh);
+    uint32_t w[4];
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e)
+        w[e] = (uint32_t)to16<SCR>(c[2 * e] * scl) | ((uint32_t)to16<SCR>(c[2 * e + 1] * scl) << 16);
+    const uint32_t lim = rs16 - 8;
+    if (4 * sc < lim) *reinterpret_cast<uint2*>(row + 4 * sc) = make_uint2(w[0], w[1]);
+    if (64 + 4 * sc < lim) *reinterpret_cast<uint2*>(row + 64 + 4 * sc) = make_uint2(w[2], w[3]);
+}
+
+template <uint32_t SCR>
+__device__ inline float4_t mfma32(const uint16_t* a, const uint16_t* b, float4_t c) {
+    const uint4 av = *reinterpret_cast<const uint4*>(a), bv = *reinterpret_cast<const uint4*>(b);
+    if constexpr (SCR == 1)
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8_t, av),
+                                                      __builtin_bit_cast(half8_t, bv), c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, av),
+                                                       __builtin_bit_cast(bf16x8_t, bv), c, 0, 0, 0);
+}
+
+template <uint32_t KS, uint32_t SCR>
 __global__ __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void k_non_reverse(NonRevArgs a) {
+    static_assert(SCR == 0 || KS > 0, "the 16-bit screens take a compile-time width");
+    constexpr uint32_t K32 = SCR ? (4 * KS + 31) / 32 : 1;
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t k = a.k, d = a.d, ld = a.ld;
     // Persistent, XCD-aware schedule: the workgroups of one XCD (blockIdx % 8) share one contiguous
@@ -421,7 +525,12 @@ void k_non_reverse(NonRevArgs a) {
     const uint32_t i_end = min(a.nq, (xcd + 1) * per);
     uint32_t idx = xcd * per + (blockIdx.x >> 3);
     if (idx >= i_end) return;
-    const NrLds L = nr_layout(k, ld);
+    const NrLds L = nr_layout(k, ld, d, SCR);
+    const uint32_t rs16 = L.rs16;
+    uint16_t* s_b16 = reinterpret_cast<uint16_t*>(lds + L.off_b16);
+    uint16_t* s_a16 = reinterpret_cast<uint16_t*>(lds + L.off_a16);
+    float* s_brinv = reinterpret_cast<float*>(lds + L.off_brinv);
+    float* s_arinv = reinterpret_cast<float*>(lds + L.off_arinv);
     const uint32_t ldp = L.ldp, d4 = (d + 3) / 4, ldp4 = ldp / 4;
     // t / ldp4 = umulhi(t, ceil(2^32 / ldp4)) exactly for t < 2^32 / ldp4 (ldp4 > 1)
     const uint32_t ldp4_inv = ldp4 > 1 ? (uint32_t)((0x100000000ull + ldp4 - 1) / ldp4) : 0u;
@@ -483,6 +592,45 @@ void k_non_reverse(NonRevArgs a) {
             pf_snap[u] = (sr < na && j < k) ? sp[j] : 0u;
         }
     };
+    // 16-bit screens: S(q)'s FP32 rows are staged in LDS (the 16-bit rows' and the chunk rows'
+    // regions, free between q), with q's row in s_xq, then centred on q and stored as 16-bit rows
+    // before the q's first chunk is committed. Pass u moves rows 32 u .. 32 u + 31, 16 threads per
+    // row (dims 4 sc .. and 64 + 4 sc ..); its 16-bit rows end at 2 rs16 32 (u + 1) bytes, below
+    // where pass u + 1's FP32 rows start (4 ldp 32 (u + 1), as rs16 <= 2 ldp), so with the reads of
+    // all passes done before one barrier, the writes clobber nothing still to be read.
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float* s_stage = reinterpret_cast<float*>(lds + L.off_b16);
+    float* s_xq = reinterpret_cast<float*>(lds + kNrOffXq);
+    auto convert_s16 = [&](uint32_t tid) {
+        if constexpr (SCR > 0) {
+            const uint32_t sr = tid >> 4, sc = tid & 15;
+            const float4 x0 = sc < d4 ? reinterpret_cast<const float4*>(s_xq)[sc] : zero4;
+            const float4 x1 = 16 + sc < d4 ? reinterpret_cast<const float4*>(s_xq)[16 + sc] : zero4;
+            float4 v[kNrSPass][2];
+#pragma unroll
+            for (uint32_t u = 0; u < kNrSPass; ++u) {
+                const uint32_t r = 32 * u + sr;
+                const bool ok = r < k;
+                const float4* src = reinterpret_cast<const float4*>(s_stage + (ok ? r : 0u) * ldp);
+                v[u][0] = ok && sc < d4 ? src[sc] : zero4;
+                v[u][1] = ok && 16 + sc < d4 ? src[16 + sc] : zero4;
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t u = 0; u < kNrSPass; ++u) {
+                const uint32_t r = 32 * u + sr;
+                if (r < L.kb16) {
+                    float nrm, rinv;
+                    centre_row16<SCR>(v[u][0], v[u][1], x0, x1, r < k, sc, d, rs16, s_b16 + r * rs16,
+                                      nrm, rinv);
+                    if (sc == 0) {
+                        s_bnrm[r] = r < k ? nrm : 0.f;
+                        s_brinv[r] = r < k ? rinv : 0.f;
+                    }
+                }
+            }
+        }
+    };
     // S from its snapshot slots and norms (s_nsq, s_nbn): ids and norms with the new entries first
     // (wave 0; slots lane, lane + 64; k <= 128), s_cnt[2] = its new-entry count; the hash cleared
     auto prep_order = [&](uint32_t tid) {
@@ -514,8 +662,9 @@ void k_non_reverse(NonRevArgs a) {
             s_bnrm[t] = 0.f;
         }
     };
-    // after prep_order and a barrier: S's ids hashed, S's rows in flight to LDS
-    auto prep_rows = [&](uint32_t tid) {
+    // after prep_order and a barrier: S's ids hashed, S's rows in flight (to LDS for the FP32
+    // Gram, to registers for the 16-bit screens, with q's row qq)
+    auto prep_rows = [&](uint32_t tid, uint32_t qq) {
         const uint32_t wave = tid >> 6;
         // Id 0 may be the padding of an unfilled list rather than a member, so it is never hashed:
         // offers of id 0 always go through (the list's id check settles them).
@@ -534,6 +683,13 @@ void k_non_reverse(NonRevArgs a) {
         // are: the Gram reads dims below 4 ceil(d/4) only, a Gram column depends on its own row
         // only and columns past k are never offered, and the exact distances read dims below d of
         // rows below k.
+        if constexpr (SCR > 0) {  // q's row, for centring
+            if (wave == 0 && (tid & 63) < d4)
+                __builtin_amdgcn_global_load_lds(a.X + (size_t)qq * ld + 4 * (tid & 63),
+                                                 (__attribute__((address_space(3))) void*)s_xq, 16, 0, 0);
+        }
+        float* s_dst = SCR ? s_stage : s_brow;
+        __attribute__((address_space(3))) float* s_dst_lds = (__attribute__((address_space(3))) float*)(s_dst);
         const uint32_t tot = L.kb16 * ldp4;
 #pragma unroll
         for (uint32_t u = 0; u < kNrBRows; ++u) {
@@ -542,7 +698,7 @@ void k_non_reverse(NonRevArgs a) {
             if (t < tot && r < k && q4 < d4) {
                 __builtin_amdgcn_global_load_lds(
                     a.X + (size_t)s_bid[r] * ld + 4 * q4,
-                    (__attribute__((address_space(3))) void*)(s_brow_lds + 4 * (u * kNrThreads + wave * 64)),
+                    (__attribute__((address_space(3))) void*)(s_dst_lds + 4 * (u * kNrThreads + wave * 64)),
                     16, 0, 0);
             }
         }
@@ -563,7 +719,7 @@ void k_non_reverse(NonRevArgs a) {
     prep_order(tid0);
     __syncthreads();
     if (a1 > a0) {
-        prep_rows(tid0);
+        prep_rows(tid0, q);
         if (early_fetch) fetch_rows(tid0, s_rsbuf0, (uint32_t)min(a1 - a0, (uint64_t)kNrChunk));
     }
     __builtin_amdgcn_s_waitcnt(0);
@@ -577,6 +733,12 @@ void k_non_reverse(NonRevArgs a) {
     uint32_t tid = tid0;
     asm volatile("" : "+v"(tid));
     const uint32_t lane = tid & 63, wave = tid >> 6;
+    if constexpr (SCR > 0) {  // S(q)'s staged rows -> 16-bit rows centred on q
+        if (a1 > a0) {
+            convert_s16(tid);
+            __syncthreads();
+        }
+    }
     uint16_t* s_queue = reinterpret_cast<uint16_t*>(lds + kNrOffQueue) + wave * kNrQueue;
     // commit map: 16 threads per chunk row
     const uint32_t sr = tid >> 4, sc = tid & 15;
@@ -647,6 +809,19 @@ void k_non_reverse(NonRevArgs a) {
             const uint32_t q4 = sc + 16 * u;
             if (q4 < ldp4) reinterpret_cast<float4*>(s_arow)[sr * ldp4 + q4] = pf_row[u];
         }
+        if constexpr (SCR > 0) {  // the row centred on q, 16-bit, with its centred norm
+            static_assert(kNrPfRows == 2, "a thread holds dims 4 sc .. and 64 + 4 sc ..");
+            float nrm, rinv;
+            const bool v = sr < pf_na;
+            const float4 x0 = sc < d4 ? reinterpret_cast<const float4*>(s_xq)[sc] : zero4;
+            const float4 x1 = 16 + sc < d4 ? reinterpret_cast<const float4*>(s_xq)[16 + sc] : zero4;
+            centre_row16<SCR>(pf_row[0], pf_row[1], x0, x1, v, sc, d, rs16, s_a16 + sr * rs16,
+                              nrm, rinv);
+            if (sc == 0) {
+                s_anrm[sr] = v ? nrm : 0.f;
+                s_arinv[sr] = v ? rinv : 0.f;
+            }
+        }
         // member[sr][j]: S(q)[j] is in S(p): the list's id check would reject it. Every bucket read
         // is issued before the first is used.
         if (sr < pf_na) {
@@ -669,8 +844,12 @@ void k_non_reverse(NonRevArgs a) {
     commit();  // S(q)'s rows and hash were made ready before the loop head's barrier
     __syncthreads();
 
-    // |approx - exact| <= (K + 8) 2^-23 (n_p + n_r) for the FP32 Gram over K = 4 ceil(d/4) terms
-    const float margin_c = (float)(4 * d4 + 8) * (1.0f / 8388608.0f);
+    // FP32 Gram: |approx - exact| <= (K + 8) 2^-23 (n_p + n_r) over K = 4 ceil(d/4) terms.
+    // 16-bit Gram of rows centred on q (n_p, n_r their centred norms): the leaf stage's bound for
+    // fp16 (bf16 screens are only used without a margin).
+    const bool exact16 = a.exact != 0;
+    const float margin_c = SCR == 0 ? (float)(4 * d4 + 8) * (1.0f / 8388608.0f)
+                                    : HalfTraits<SCR == 1 ? 0 : 1>::kMargin;
     const uint32_t ksteps = d4;
     const uint32_t ntile = L.kb16 / 16;
     const uint32_t lrow = lane & 15, lk = lane >> 4;
@@ -696,7 +875,8 @@ void k_non_reverse(NonRevArgs a) {
                 const bool act = e < qn;
                 const uint32_t ent = act ? s_queue[e] : 0u;
                 const uint32_t i = ent >> 7, cc = ent & 127;
-                const float* rb = s_brow + cc * ldp;
+                // S(q)'s FP32 rows: in LDS for the FP32 Gram, in global memory for the 16-bit screens
+                const float* rb = SCR == 0 ? s_brow + cc * ldp : a.X + (size_t)s_bid[cc] * ld;
                 const float* ra = s_arow + i * ldp;
                 float ex = 0.f;
                 if (wide) {
@@ -757,31 +937,49 @@ void k_non_reverse(NonRevArgs a) {
                 rok[i] = rr[i] < na;
                 rn[i] = s_anrm[rr[i]];
                 pid[i] = s_aid[rr[i]];
-                thr0[i] = s_abnd[rr[i]] * (1.0f + 1.0f / 65536.0f);
+                thr0[i] = SCR == 0 || exact16 ? s_abnd[rr[i]] * (1.0f + 1.0f / 65536.0f) : s_abnd[rr[i]];
             }
             const float* arow = s_arow + (16 * strip + lrow) * ldp;
+            // 16-bit screens: the strip's rows centred on q (lane group lk >> 2 takes halves
+            // 8 (lk >> 2) .. + 7 of each 32-dim k-step, for the rows and the columns alike), the
+            // inverse scales, and -2 folded into the rows' inverse scales (powers of two: exact)
+            const uint16_t* a16 = s_a16 + (16 * strip + lrow) * rs16 + 8 * (lane >> 4);
+            float m2r[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) m2r[i] = SCR ? -2.0f * s_arinv[rr[i]] : 0.f;
             for (uint32_t j = wv; j < nstrip * tt; j += kNrThreads / 64) {
                 const uint32_t t = nstrip == 2 ? j >> 1 : j;
                 if (qn + 256 > kNrQueue) flush();
-                const float* brow = s_brow + (16 * t + lrow) * ldp;
                 // membership words of this tile's columns (c >> 5 = t >> 1 for the whole tile)
                 uint32_t mw[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) mw[i] = s_mask[rr[i] * 4 + (t >> 1)];
-                const float4_t acc = gram_tile_f32<KS>(arow, brow, lk, ksteps);
+                float4_t acc;
+                if constexpr (SCR == 0) {
+                    acc = gram_tile_f32<KS>(arow, s_brow + (16 * t + lrow) * ldp, lk, ksteps);
+                } else {
+                    const uint16_t* b16 = s_b16 + (16 * t + lrow) * rs16 + 8 * (lane >> 4);
+                    acc = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (uint32_t m = 0; m < K32; ++m) acc = mfma32<SCR>(a16 + 32 * m, b16 + 32 * m, acc);
+                }
                 const uint32_t c = 16 * t + lrow, cb = c & 31;
                 const bool cok = c < k;
                 const float cn = s_bnrm[c];  // zero past k
+                const float cinv = SCR ? s_brinv[c] : 0.f;
                 const uint32_t rid = s_bid[c];
                 // compact the survivors of this tile into the wave's queue as (row << 7 | col)
                 const uint32_t qn_before = qn;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float nsum = rn[i] + cn;
-                    const float approx = __builtin_fmaf(-2.0f, acc[i], nsum);  // 2 acc is exact
+                    // FP32 Gram: 2 acc is exact. 16-bit: acc * cinv * (-2 rinv) is exact scaling.
+                    const float approx = SCR == 0 ? __builtin_fmaf(-2.0f, acc[i], nsum)
+                                                  : __builtin_fmaf(acc[i] * cinv, m2r[i], nsum);
+                    const float thr = SCR == 0 || exact16 ? __builtin_fmaf(margin_c, nsum, thr0[i]) : thr0[i];
                     // <=: a distance equal to a zero bound can still win on its id
                     const bool ok = cok && rok[i] && rid != pid[i] && !((mw[i] >> cb) & 1u) &&
-                                    approx <= __builtin_fmaf(margin_c, nsum, thr0[i]);
+                                    approx <= thr;
                     const uint64_t bal = __ballot(ok);
                     if (ok) {
                         const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
@@ -852,7 +1050,7 @@ void k_non_reverse(NonRevArgs a) {
     prep_order(tid);
     __syncthreads();
     const uint32_t nrs1 = (uint32_t)(s_meta[2] - s_meta[1]);
-    if (nrs1 > 0) prep_rows(tid);
+    if (nrs1 > 0) prep_rows(tid, (uint32_t)s_meta[0]);
     fetch_rows(tid, s_nrs, early_fetch ? min(nrs1, kNrChunk) : 0u);
     if (lane == 0 && nidx + nslot < i_end) nq_l = a.order[(size_t)(nidx + nslot) * a.order_stride];
     append();  // the atomics' round trips overlap the loads above
@@ -1022,7 +1220,8 @@ void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t
 
 void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {
     if (!a.nq) return;
-    const size_t lds = non_reverse_lds(a.k, a.ld);
+    const uint32_t scr = non_screen16_supported(a.d) ? a.screen16 : 0u;
+    const size_t lds = non_reverse_lds(a.k, a.ld, a.d, scr);
     auto go = [&](auto kernel) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1035,11 +1234,17 @@ void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {
         const uint32_t grid = 8 * std::min((a.nq + 7) / 8, slots);
         kernel<<<grid, kNrThreads, lds, st>>>(a);
     };
-    switch ((a.d + 3) / 4) {  // the Gram's k-steps
-        case 16: go(&k_non_reverse<16>); break;
-        case 25: go(&k_non_reverse<25>); break;
-        case 32: go(&k_non_reverse<32>); break;
-        default: go(&k_non_reverse<0>); break;
+    switch ((a.d + 3) / 4 + 100 * scr) {  // the FP32 Gram's k-steps, and the screen
+        case 16: go(&k_non_reverse<16, 0>); break;
+        case 25: go(&k_non_reverse<25, 0>); break;
+        case 32: go(&k_non_reverse<32, 0>); break;
+        case 116: go(&k_non_reverse<16, 1>); break;
+        case 125: go(&k_non_reverse<25, 1>); break;
+        case 132: go(&k_non_reverse<32, 1>); break;
+        case 216: go(&k_non_reverse<16, 2>); break;
+        case 225: go(&k_non_reverse<25, 2>); break;
+        case 232: go(&k_non_reverse<32, 2>); break;
+        default: go(&k_non_reverse<0, 0>); break;
     }
 }
 

@@ -9,7 +9,8 @@
 //   lloyd_finish one block per node: chunk partials added in chunk order, centres = sum / count
 //                (:182-185), next hyperplane.
 //   side_count   one block per tile of <= 1024 positions: final side test over the full range
-//                (:189-197, :262-289), one flag byte per position, lo count per tile.
+//                (:189-197, :262-289) on rows staged through LDS by coalesced loads, one flag
+//                byte per position, lo count per tile.
 //   scan / node_lo / scatter / copy_back: stable partition of each node's positions (lo first).
 //
 // The side test reads one 400-byte row per position and is HBM bound; everything else is small.
@@ -180,35 +181,69 @@ __global__ __launch_bounds__(kSplitThreads) void k_lloyd_finish(
 }
 
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kSplitThreads) void k_side_count(
+// The final side test reads every position's row once. A tile's rows are staged into LDS in passes
+// of kSideRows with global_load_lds (16 bytes per lane, consecutive lanes on consecutive 16-byte
+// words of a row, so a wave's load instruction reads 1 KB of whole rows), then each thread takes one
+// staged row through the reference-order dot. LDS rows of ld floats (ld + 4 when ld is a multiple
+// of 8) put the 16-byte reads of 16 consecutive lanes in distinct banks.
+constexpr uint32_t kSideThreads = 128;
+constexpr uint32_t kSideRows = kSideThreads;  // one row per thread and pass
+
+__host__ __device__ inline uint32_t side_row_stride(uint32_t ld) { return ld % 8 == 0 ? ld + 4 : ld; }
+
+__global__ __launch_bounds__(kSideThreads) void k_side_count(
     const float* __restrict__ X, uint32_t ld, uint32_t d, const uint32_t* __restrict__ perm,
     const NodeDesc* __restrict__ nodes, const TileDesc* __restrict__ tiles,
     const float* __restrict__ coef, const float* __restrict__ off, uint8_t* __restrict__ flags,
     uint32_t* __restrict__ tile_lo) {
+    extern __shared__ __align__(16) float s_rows[];  // kSideRows rows of side_row_stride(ld)
+    __shared__ __align__(16) float cf[kMaxDim];
+    __shared__ uint32_t s_id[kSideRows];
+    __shared__ uint32_t nlo;
     const TileDesc tl = tiles[blockIdx.x];
     const NodeDesc nd = nodes[tl.node];
-    __shared__ __align__(16) float cf[kMaxDim];
-    __shared__ uint32_t nlo;
     if (nd.status == kNodeFailed) {
         if (threadIdx.x == 0) tile_lo[blockIdx.x] = 0;
         return;
     }
-    for (uint32_t j = threadIdx.x; j < ld; j += blockDim.x) cf[j] = coef[(size_t)tl.node * ld + j];
-    if (threadIdx.x == 0) nlo = 0;
-    __syncthreads();
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (uint32_t j = tid; j < ld; j += kSideThreads) cf[j] = coef[(size_t)tl.node * ld + j];
+    if (tid == 0) nlo = 0;
     const float o = off[tl.node];
+    const uint32_t ldp = side_row_stride(ld), ldp4 = ldp / 4, d4 = (d + 3) / 4;
+    // t / ldp4 = umulhi(t, ceil(2^32 / ldp4)) exactly for t < 2^32 / ldp4 (ldp4 > 1)
+    const uint32_t inv = ldp4 > 1 ? (uint32_t)((0x100000000ull + ldp4 - 1) / ldp4) : 0u;
+    __attribute__((address_space(3))) float* s_rows_lds = (__attribute__((address_space(3))) float*)(s_rows);
     uint32_t mylo = 0;
     const uint32_t base = nd.start + tl.begin;
-    for (uint32_t e = threadIdx.x; e < tl.count; e += blockDim.x) {
-        const uint32_t r = perm[base + e];
-        const bool lo = dot_ref(cf, X + (size_t)r * ld, d) >= o;
-        flags[base + e] = lo;
-        mylo += lo;
+    for (uint32_t s0 = 0; s0 < tl.count; s0 += kSideRows) {
+        const uint32_t ns = min(kSideRows, tl.count - s0);
+        s_id[tid] = tid < ns ? perm[base + s0 + tid] : 0u;
+        __syncthreads();  // ids in; the previous pass's rows all read
+        const uint32_t tot = ns * ldp4;
+        for (uint32_t t0 = 0; t0 < tot; t0 += kSideThreads) {
+            const uint32_t t = t0 + tid;
+            const uint32_t r = ldp4 == 1 ? t : __umulhi(t, inv), q4 = t - r * ldp4;
+            if (t < tot && q4 < d4)
+                __builtin_amdgcn_global_load_lds(
+                    X + (size_t)s_id[r] * ld + 4 * q4,
+                    (__attribute__((address_space(3))) void*)(s_rows_lds + 4 * (t0 + wave * 64)), 16, 0, 0);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid < ns) {
+            const bool lo = dot_ref(cf, s_rows + tid * ldp, d) >= o;
+            flags[base + s0 + tid] = lo;
+            mylo += lo;
+        }
     }
+    (void)lane;
     atomicAdd(&nlo, mylo);
     __syncthreads();
-    if (threadIdx.x == 0) tile_lo[blockIdx.x] = nlo;
+    if (tid == 0) tile_lo[blockIdx.x] = nlo;
 }
+
+size_t side_count_lds(uint32_t ld) { return (size_t)kSideRows * side_row_stride(ld) * sizeof(float); }
 
 // Exclusive scan of tile lo counts over all tiles of the level (one block; tiles <= ~1e6).
 __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict__ tile_lo,
@@ -326,8 +361,15 @@ void launch_split_prepare(const SplitLevelArgs& a, hipStream_t st) {
 }
 
 void launch_split_side(const SplitLevelArgs& a, hipStream_t st) {
-    k_side_count<<<a.ntiles, kSplitThreads, 0, st>>>(a.X, a.ld, a.d, a.perm, a.nodes, a.tiles,
-                                                     a.coef, a.off, a.flags, a.tile_lo);
+    const size_t lds = side_count_lds(a.ld);
+    static const bool attr_set = [] {
+        return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_side_count),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)side_count_lds(kMaxDim)) == hipSuccess;
+    }();
+    (void)attr_set;
+    k_side_count<<<a.ntiles, kSideThreads, lds, st>>>(a.X, a.ld, a.d, a.perm, a.nodes, a.tiles,
+                                                      a.coef, a.off, a.flags, a.tile_lo);
 }
 
 void launch_split_partition(const SplitLevelArgs& a, hipStream_t st) {

@@ -256,6 +256,24 @@ def test_non_matches_oracle(flags):
     assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
 
 
+@pytest.mark.parametrize("gen,d", [("lowrank", 64), ("mixture", 100), ("lowrank", 100), ("lowrank", 128),
+                                   ("mixture", 37)])
+@pytest.mark.parametrize("flags", [0, K.FLAG_NON_SCREEN16])
+def test_non_screens_match_oracle(gen, d, flags):
+    # the reverse kernel's screens: the FP32 Gram (every width) or the 16-bit Gram of rows centred
+    # on each q with the exact margin (widths 64, 100, 128; others fall back to the FP32 Gram); the
+    # lists must be the oracle's
+    X = lowrank(8000, d, 70 + d) if gen == "lowrank" else mixture(8000, d, 40, 70 + d)
+    k = 50
+    kw = dict(k=k, leaf_max=128, small_max=3000, iterations=1, seed=12)
+    o_prm = oracle.make_params(**kw)
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    st = oracle.leaf_stage(X, k, perm, ls, lz)
+    ids, ds = K.Builder(K.params(flags=flags, **kw)).non(X, (st[0], st[1]))
+    o_ids, o_ds, _ = oracle.non(X, k, st)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
+
+
 def test_non_overflow_path_matches_oracle():
     # 16 candidate slots per point: most buffers overflow and those points are redone by the
     # direct kernel; the lists must still equal the oracle's
@@ -386,10 +404,11 @@ def test_build_device_rows_ignore_padding(d, stride):
 # Precision forms, exact FP64 tool, generators
 # ---------------------------------------------------------------------------------------------
 @pytest.mark.parametrize("prec", [K.PREC_FP16, K.PREC_BF16])
-def test_precision_forms_valid_and_close(prec):
+@pytest.mark.parametrize("flags", [0, K.FLAG_NON_SCREEN16])
+def test_precision_forms_valid_and_close(prec, flags):
     X = lowrank(20000, 100, 12)
     kw = dict(k=50, leaf_max=256, small_max=3000, iterations=1, rounds=4, non_passes=1, seed=3)
-    ids, ds = K.Builder(K.params(precision=prec, **kw)).build(X, return_dists=True)
+    ids, ds = K.Builder(K.params(precision=prec, flags=flags, **kw)).build(X, return_dists=True)
     f_ids, f_ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
     assert_valid_lists(X, ids, ds)
     q = np.arange(0, 20000, 40, dtype=np.uint32)
