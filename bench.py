@@ -58,6 +58,9 @@ def parse():
                     help="kmeans: SolutionKmeans' 2-means split; kd: SolutionRandomKD's "
                          "random-rotation split (the reference runs it with 5 rounds, leaves < 400, "
                          "no NoN: --splitter kd --rounds 5 --leaf 399 --non 0)")
+    ap.add_argument("--leaf-mode", default="binned", choices=["binned", "batched", "graph"],
+                    help="leaf stage: one launch per bucket-size bin; one persistent kernel over the "
+                         "round's work list; the same replayed from a captured hipGraph")
     ap.add_argument("--non-screen16", action="store_true",
                     help="neighbours-of-neighbours screen on a 16-bit Gram of rows centred on each "
                          "q instead of the FP32 Gram of the rows")
@@ -110,6 +113,10 @@ def copy_calibration(mib):
     log(f"[bench] copy calibration: {mib} MiB read, {mib} MiB written")
 
 
+def leaf_flags(a, K):
+    return {"binned": 0, "batched": K.FLAG_LEAF_BATCHED, "graph": K.FLAG_LEAF_GRAPH}[a.leaf_mode]
+
+
 def zipf_round(a, K):
     """One round of the batching form: 2-means split + every bucket's all-pairs stage."""
     n, d, k = a.n, (a.d if a.d != 100 else 128), a.k
@@ -117,7 +124,7 @@ def zipf_round(a, K):
     ld = (d + 3) & ~3
     prm = K.params(k=k, leaf_max=leaf, small_max=3000, iterations=a.iterations, rounds=1,
                    non_passes=0, precision=K.PREC_FP32, seed=a.seed,
-                   flags=K.FLAG_AUTOTUNE if a.autotune else 0)
+                   flags=(K.FLAG_AUTOTUNE if a.autotune else 0) | leaf_flags(a, K))
     b = K.Builder(prm)
     log(f"[bench] generating {n}x{d} Zipf-mixture points (20,000 clusters) on the device")
     dX = K.DeviceArray(n * ld * 4)
@@ -146,7 +153,8 @@ def zipf_round(a, K):
         "value": round(mean_t * 1e3, 2), "unit": "ms per round", "higher_is_better": False,
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"Zipf(1.1) mixture {n}x{d}, 20000 clusters, sigma U[0.3,2], "
-                               f"leaf<={leaf}, k={k}", "n": n, "d": d, "k": k, "leaf_max": leaf},
+                               f"leaf<={leaf}, k={k}", "n": n, "d": d, "k": k, "leaf_max": leaf,
+                   "leaf_mode": a.leaf_mode},
         "stages_ms": {"split": round(st["split_ms"], 2), "side": round(st["side_ms"], 2),
                       "leaf": round(st["leaf_ms"], 2), "total": round(st["total_ms"], 2)},
         "buckets": int(st["leaves"]), "leaf_pairs": int(st["leaf_pairs"]),
@@ -181,6 +189,7 @@ def serving(a, K):
     def finish(i, t, slot, n, d, k, seed, t_arr):
         ids, _, res = srv.wait(t)
         t_done = time.perf_counter()
+        log(f"[bench] job {i} done: n={n} d={d} k={k} latency {(t_done - t_arr) * 1e3:.0f} ms")
         ns = min(n, 10_000)
         q = np.sort(np.random.default_rng([seed, 7]).choice(n, size=ns, replace=False)).astype(np.uint32)
         with cv:
@@ -292,7 +301,7 @@ def main():
 
     prec = {"fp32": K.PREC_FP32, "fp16": K.PREC_FP16, "bf16": K.PREC_BF16}[a.precision]
     flags = ((K.FLAG_KD_SPLIT if a.splitter == "kd" else 0) | (K.FLAG_AUTOTUNE if a.autotune else 0)
-             | (K.FLAG_NON_SCREEN16 if a.non_screen16 else 0))
+             | (K.FLAG_NON_SCREEN16 if a.non_screen16 else 0) | leaf_flags(a, K))
     prm = K.params(k=a.k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
                    rounds=a.rounds, non_passes=a.non, precision=prec, seed=a.seed, flags=flags)
     b = K.Builder(prm)
@@ -511,6 +520,7 @@ def main():
         "config": {"workload": f"low-rank {n}x{d} (24 latent, unit norm), k={k}, leaf<={a.leaf}",
                    "n": n, "d": d, "k": k, "rounds": a.rounds, "iterations": a.iterations,
                    "non_passes": a.non, "precision": a.precision, "splitter": a.splitter, "autotune": a.autotune,
+                   "leaf_mode": a.leaf_mode,
                    "recall_at_k": None if rec is None else round(rec, 4),
                    "recall_queries": nq, "parallelism": "single GPU"},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
