@@ -49,6 +49,11 @@ extern "C" {
                                         in the FP32 form, so the same lists; bf16/fp16 without a
                                         margin in those forms). Measured slower at 10M x 100: the
                                         exact re-rank then reads S(q)'s rows from global memory */
+#define KNNG_FLAG_LEAF_BATCHED 128u  /* leaf stage as one persistent kernel over one work list per
+                                        round (big-bucket blocks first, then buckets by decreasing
+                                        size) instead of one launch per size bin; same lists */
+#define KNNG_FLAG_LEAF_GRAPH 256u    /* as KNNG_FLAG_LEAF_BATCHED, its launches captured once in a
+                                        hipGraph and replayed every round */
 #define KNNG_FLAG_AUTOTUNE 32u       /* leaf stage: the first launch of each size bin above 128
                                         points (per device, precision and width, once per process)
                                         also times both workgroup shapes, 8 and 16 waves, and the

@@ -39,6 +39,8 @@ FLAG_BIG_LEAF = 2
 FLAG_INCREMENTAL_NON = 4
 FLAG_SMALL_NON_CAP = 8  # tests: tiny candidate buffers, so the overflow path runs
 FLAG_NON_SCREEN16 = 64  # neighbours-of-neighbours screen on a 16-bit Gram of rows centred on q
+FLAG_LEAF_BATCHED = 128  # leaf stage: one persistent kernel over the round's work list
+FLAG_LEAF_GRAPH = 256    # ... replayed from a captured hipGraph
 FLAG_KD_SPLIT = 16  # random-rotation KD splitter (SolutionRandomKD) instead of 2-means
 FLAG_AUTOTUNE = 32  # leaf workgroup shape picked by timing both on the first launch of a size bin
 

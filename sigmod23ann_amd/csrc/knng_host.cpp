@@ -273,6 +273,19 @@ struct Builder {
     DevBuf<float> knn_d, bound;
     DevBuf<LeafDesc> leaves[2];
     DevBuf<BigBlock> bigblocks[2];
+    // batched leaf stage: work-queue headers per round parity, and the captured launches
+    DevBuf<LeafQueue> lq[2];
+    PinBuf<LeafQueue> h_lq[2];
+    hipGraphExec_t leaf_graph[2] = {};
+    LeafArgs leaf_graph_args[2] = {};
+    const BigBlock* leaf_graph_big[2] = {};
+    uint64_t leaf_graph_replays = 0;
+    // Leaf-stage configuration: the launch form and, in the exact form, the screen's 16-bit type
+    // (fp16 or bf16 Gram, each with its own rigorous margin: the same lists). Set by the flags, or
+    // by the autotuner on a build's first leaf stage (KNNG_FLAG_AUTOTUNE).
+    bool leaf_batched = false;
+    uint32_t leaf_half = 0;
+    bool leaf_timing = false;  // a tuner pass: no statistics, no per-bin shape tuning
     DevBuf<uint64_t> counters;
 
     // exact k-NN scratch
@@ -318,6 +331,8 @@ struct Builder {
             if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
         if (sst) (void)hipStreamDestroy(sst);
+        for (auto& g : leaf_graph)
+            if (g) (void)hipGraphExecDestroy(g);
     }
 
     bool verbose = false;
@@ -792,9 +807,9 @@ struct Builder {
         a.bound = bnd;
         a.screen = prm.precision == KNNG_PREC_FP32 ? kScreenExact : kScreenApprox;
         a.rerank_cap = prm.rerank;
-        a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
-        a.counters = cnt;
-        a.tune = tune ? 1u : 0u;
+        a.half_kind = screen_half();
+        a.counters = leaf_timing ? nullptr : cnt;
+        a.tune = tune && !leaf_timing ? 1u : 0u;
         KNNG_HIP(hipEventRecord(e0, st));
         size_t off = 0;
         for (auto& v : pl.bin) {
@@ -828,6 +843,110 @@ struct Builder {
         return KNNG_OK;
     }
 
+    // Batched leaf stage: one persistent kernel over one work list (big-bucket blocks, then the
+    // LDS-resident buckets by decreasing size), optionally replayed from a captured graph.
+    int leaf_stage_batched(const std::vector<LeafDesc>& lv, const uint32_t* pm, uint32_t slot,
+                           bool sync, hipEvent_t e0, hipEvent_t e1) {
+        if (d > 128) return fail(KNNG_ERR_UNSUPPORTED, "leaf stage supports d <= 128");
+        const uint32_t lim = leaf_batched_limit(d, ld);
+        if (leaf_big_lds_bytes(d, ld) > 160 * 1024 || leaf_batched_lds(d, ld) > 160 * 1024)
+            return fail(KNNG_ERR_UNSUPPORTED, "batched leaf kernel does not fit LDS at this d");
+        std::vector<LeafDesc> small;
+        std::vector<BigBlock> big;
+        uint64_t pairs = 0;
+        small.reserve(lv.size());
+        for (auto& l : lv) {
+            pairs += (uint64_t)l.size * (l.size - 1) / 2;
+            if (l.size < 2) continue;
+            if (l.size <= lim && !(prm.flags & KNNG_FLAG_BIG_LEAF)) small.push_back(l);
+            else for (uint32_t r0 = 0; r0 < l.size; r0 += kLeafBigRows) big.push_back({l.start, l.size, r0});
+        }
+        std::stable_sort(small.begin(), small.end(),
+                         [](const LeafDesc& x, const LeafDesc& y) { return x.size > y.size; });
+        KNNG_HIP(hipEventSynchronize(up_ev[slot]));
+        KNNG_TRY(h_leaves[slot].ensure(std::max<size_t>(small.size(), 1)));
+        KNNG_TRY(leaves[slot].ensure(std::max<size_t>(small.size(), 1)));
+        KNNG_TRY(h_big[slot].ensure(std::max<size_t>(big.size(), 1)));
+        KNNG_TRY(bigblocks[slot].ensure(std::max<size_t>(big.size(), 1)));
+        KNNG_TRY(h_lq[slot].ensure(1));
+        KNNG_TRY(lq[slot].ensure(1));
+        if (!small.empty()) {
+            std::memcpy(h_leaves[slot].p, small.data(), small.size() * sizeof(LeafDesc));
+            KNNG_HIP(hipMemcpyAsync(leaves[slot].p, h_leaves[slot].p, small.size() * sizeof(LeafDesc),
+                                    hipMemcpyHostToDevice, st));
+        }
+        if (!big.empty()) {
+            std::memcpy(h_big[slot].p, big.data(), big.size() * sizeof(BigBlock));
+            KNNG_HIP(hipMemcpyAsync(bigblocks[slot].p, h_big[slot].p, big.size() * sizeof(BigBlock),
+                                    hipMemcpyHostToDevice, st));
+        }
+        *h_lq[slot].p = LeafQueue{0u, (uint32_t)big.size(), (uint32_t)small.size(), 0u};
+        KNNG_HIP(hipMemcpyAsync(lq[slot].p, h_lq[slot].p, sizeof(LeafQueue), hipMemcpyHostToDevice, st));
+        KNNG_HIP(hipEventRecord(up_ev[slot], st));
+        LeafArgs a{};
+        a.X = X;
+        a.ld = ld;
+        a.d = d;
+        a.k = kl();
+        a.perm = pm;
+        a.knn_ids = knn_ids.p;
+        a.knn_d = knn_d.p;
+        a.bound = bound.p;
+        a.screen = prm.precision == KNNG_PREC_FP32 ? kScreenExact : kScreenApprox;
+        a.rerank_cap = prm.rerank;
+        a.half_kind = screen_half();
+        a.counters = leaf_timing ? nullptr : counters.p;
+        a.tune = 0;
+        a.leaves = leaves[slot].p;
+        a.nleaves = (uint32_t)small.size();
+        a.max_leaf = lim;
+        KNNG_HIP(hipEventRecord(e0, st));
+        if (prm.flags & KNNG_FLAG_LEAF_GRAPH) {
+            // the launches depend on the round only through the uploaded lists and counts, so a
+            // graph captured once per round parity is replayed while its arguments are unchanged
+            a.nleaves = 0;  // the kernel reads the counts from the queue header
+            const bool same = leaf_graph[slot] && leaf_graph_big[slot] == bigblocks[slot].p &&
+                              std::memcmp(&leaf_graph_args[slot], &a, sizeof(LeafArgs)) == 0;
+            if (!same) {
+                if (leaf_graph[slot]) KNNG_HIP(hipGraphExecDestroy(leaf_graph[slot]));
+                leaf_graph[slot] = nullptr;
+                hipGraph_t g = nullptr;
+                KNNG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                launch_leaf_batched(a, bigblocks[slot].p, lq[slot].p, st);
+                KNNG_HIP(hipStreamEndCapture(st, &g));
+                const hipError_t ie = hipGraphInstantiate(&leaf_graph[slot], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                KNNG_HIP(ie);
+                leaf_graph_args[slot] = a;
+                leaf_graph_big[slot] = bigblocks[slot].p;
+            }
+            KNNG_HIP(hipGraphLaunch(leaf_graph[slot], st));
+            ++leaf_graph_replays;
+        } else {
+            launch_leaf_batched(a, bigblocks[slot].p, lq[slot].p, st);
+        }
+        KNNG_HIP(hipGetLastError());
+        KNNG_HIP(hipEventRecord(e1, st));
+        stats.leaf_launches += 2;
+        if (sync) {
+            KNNG_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            KNNG_HIP(hipEventElapsedTime(&ms, e0, e1));
+            stats.leaf_ms += ms;
+        }
+        stats.leaves += lv.size();
+        stats.leaf_pairs += pairs;
+        return KNNG_OK;
+    }
+
+    // The 16-bit type of the leaf screen: the precision's in the approximate forms; in the exact
+    // form fp16 unless the tuner kept bf16.
+    uint32_t screen_half() const {
+        if (prm.precision == KNNG_PREC_BF16) return 1;
+        if (prm.precision == KNNG_PREC_FP16) return 0;
+        return leaf_half;
+    }
+
     int leaf_stage(const std::vector<LeafDesc>& lv, const uint32_t* pm, uint32_t slot, bool sync,
                    hipEvent_t e0, hipEvent_t e1) {
         if (lv.empty()) {
@@ -835,6 +954,97 @@ struct Builder {
             KNNG_HIP(hipEventRecord(e1, st));
             return KNNG_OK;
         }
+        if (prm.flags & KNNG_FLAG_AUTOTUNE) {
+            bool ran = false;
+            KNNG_TRY(tune_leaf_stage(lv, pm, slot, sync, e0, e1, ran));
+            if (ran) return KNNG_OK;
+        }
+        if (leaf_batched || (prm.flags & (KNNG_FLAG_LEAF_BATCHED | KNNG_FLAG_LEAF_GRAPH)))
+            return leaf_stage_batched(lv, pm, slot, sync, e0, e1);
+        return leaf_stage_binned(lv, pm, slot, sync, e0, e1);
+    }
+
+    // Autotuner over the leaf stage's forms: {one launch per size bin, one persistent kernel over
+    // the work list} x {fp16, bf16 screen} (the screen type only in the exact form, where both
+    // give the same lists). On the first leaf stage of a (device, d, k, precision, leaf sizes) key
+    // in the process, the stage runs once for real in the default form, then once more in each
+    // candidate form, timed: a repeated pass offers pairs that were already offered, so the lists
+    // stay as they are, and every candidate sees the same lists. The fastest form is kept for the
+    // process and reported on stderr.
+    // ran: the stage itself was run here (the real pass, then the timed ones).
+    int tune_leaf_stage(const std::vector<LeafDesc>& lv, const uint32_t* pm, uint32_t slot,
+                        bool sync, hipEvent_t e0, hipEvent_t e1, bool& ran) {
+        ran = false;
+        int dev = 0;
+        KNNG_HIP(hipGetDevice(&dev));
+        const uint64_t key = ((uint64_t)(uint32_t)dev << 56) ^ ((uint64_t)d << 44) ^ ((uint64_t)kl() << 36) ^
+                             ((uint64_t)prm.precision << 32) ^ ((uint64_t)prm.leaf_max << 16) ^
+                             (uint64_t)prm.leaf_max_odd;
+        {
+            std::lock_guard<std::mutex> g(leaf_tune_mu());
+            const auto it = leaf_tune_map().find(key);
+            if (it != leaf_tune_map().end()) {
+                leaf_batched = it->second.first;
+                leaf_half = it->second.second;
+                return KNNG_OK;
+            }
+        }
+        // the real pass, in the default form
+        leaf_batched = false;
+        leaf_half = 0;
+        KNNG_TRY(leaf_stage_binned(lv, pm, slot, sync, e0, e1));
+        ran = true;
+        const bool exact = prm.precision == KNNG_PREC_FP32;
+        struct Cand { bool batched; uint32_t half; float ms; };
+        std::vector<Cand> cands = {{false, 0, 0.f}, {true, 0, 0.f}};
+        if (exact) {
+            cands.push_back({false, 1, 0.f});
+            cands.push_back({true, 1, 0.f});
+        }
+        const knng_stats saved = stats;
+        leaf_timing = true;
+        int rc = KNNG_OK;
+        for (auto& c : cands) {
+            leaf_batched = c.batched;
+            leaf_half = c.half;
+            rc = c.batched ? leaf_stage_batched(lv, pm, slot, true, ev[6], ev[7])
+                           : leaf_stage_binned(lv, pm, slot, true, ev[6], ev[7]);
+            if (rc != KNNG_OK) break;
+            KNNG_HIP(hipEventElapsedTime(&c.ms, ev[6], ev[7]));
+        }
+        leaf_timing = false;
+        stats = saved;
+        if (rc != KNNG_OK) return rc;
+        const Cand* best = &cands[0];
+        for (auto& c : cands)
+            if (c.ms < best->ms) best = &c;
+        leaf_batched = best->batched;
+        leaf_half = best->half;
+        std::string line;
+        for (auto& c : cands) {
+            char buf[96];
+            std::snprintf(buf, sizeof buf, "%s%s %s %.2f ms", line.empty() ? "" : ", ",
+                          c.batched ? "batched" : "binned", c.half ? "bf16" : "fp16", c.ms);
+            line += buf;
+        }
+        std::fprintf(stderr, "[knng] leaf stage tuned (d %u, k %u, leaves <= %u/%u): %s; kept %s %s\n",
+                     d, kl(), prm.leaf_max, prm.leaf_max_odd ? prm.leaf_max_odd : prm.leaf_max,
+                     line.c_str(), leaf_batched ? "batched" : "binned", leaf_half ? "bf16" : "fp16");
+        std::lock_guard<std::mutex> g(leaf_tune_mu());
+        leaf_tune_map().emplace(key, std::make_pair(leaf_batched, leaf_half));
+        return KNNG_OK;
+    }
+    static std::mutex& leaf_tune_mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::unordered_map<uint64_t, std::pair<bool, uint32_t>>& leaf_tune_map() {
+        static std::unordered_map<uint64_t, std::pair<bool, uint32_t>> m;
+        return m;
+    }
+
+    int leaf_stage_binned(const std::vector<LeafDesc>& lv, const uint32_t* pm, uint32_t slot,
+                          bool sync, hipEvent_t e0, hipEvent_t e1) {
         LeafPlan pl;
         KNNG_TRY(plan_leaves(lv, pl));
         // the pinned set's previous upload (two rounds ago) must have been read before it is refilled

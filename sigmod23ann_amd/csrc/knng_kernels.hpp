@@ -108,6 +108,20 @@ struct BigBlock {
 size_t leaf_big_lds_bytes(uint32_t d, uint32_t ld);
 void launch_leaf_big(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks, hipStream_t st);
 
+// Batched form: one persistent kernel takes the round's work from one list, the big-bucket blocks
+// first, then the LDS-resident buckets (at most leaf_batched_limit points, LeafArgs::max_leaf set
+// to it) by decreasing size.
+struct LeafQueue {
+    uint32_t next;   // next item to take (reset by the launch)
+    uint32_t nbig;   // items [0, nbig): BigBlock entries
+    uint32_t nleaf;  // items [nbig, nbig + nleaf): LeafDesc entries (LeafArgs::leaves)
+    uint32_t pad_;
+};
+uint32_t leaf_batched_limit(uint32_t d, uint32_t ld);
+size_t leaf_batched_lds(uint32_t d, uint32_t ld);
+// The same two launches every round (queue reset, persistent kernel on the chip's resident slots).
+void launch_leaf_batched(const LeafArgs& a, const BigBlock* blocks, LeafQueue* q, hipStream_t st);
+
 void launch_init_knn(uint32_t* ids, float* dists, float* bound, uint32_t n, uint32_t k,
                      hipStream_t st);
 // bound[p] = dists[p * k + k - 1] for p < n, and counters[0, ncounters) = 0.

@@ -62,11 +62,11 @@ __host__ __device__ inline LeafLds leaf_lds_layout(uint32_t max_leaf, uint32_t d
 
 // KS = (d rounded up to 16) / 16 MFMA k-steps, a compile-time constant so the k-loop unrolls
 // without guards and its LDS loads pipeline.
+// One bucket, by the whole workgroup (W waves); a.max_leaf sets the LDS layout.
 template <uint32_t HALF, uint32_t KS, uint32_t W>
-__global__ __launch_bounds__(W * 64) void k_leaf(LeafArgs a) {
+__device__ inline void leaf_bucket(const LeafArgs& a, const LeafDesc lf) {
     constexpr uint32_t kLeafWaves = W, kLeafThreads = W * 64;
     extern __shared__ __align__(16) unsigned char lds[];
-    const LeafDesc lf = a.leaves[blockIdx.x];
     const uint32_t b = lf.size;
     if (b < 2) return;
     const uint32_t d = a.d, ld = a.ld, k = a.k;
@@ -281,6 +281,11 @@ __global__ __launch_bounds__(W * 64) void k_leaf(LeafArgs a) {
     }
 }
 
+template <uint32_t HALF, uint32_t KS, uint32_t W>
+__global__ __launch_bounds__(W * 64) void k_leaf(LeafArgs a) {
+    leaf_bucket<HALF, KS, W>(a, a.leaves[blockIdx.x]);
+}
+
 // ---- buckets too large for LDS ------------------------------------------------------------------
 // One workgroup (16 waves) per 64-row block of a bucket. Each wave owns 4 of the rows and keeps
 // their sorted lists in registers for the whole kernel; the block's FP32 and 16-bit rows stay in
@@ -330,10 +335,10 @@ __host__ __device__ inline BigLds big_lds_layout(uint32_t d, uint32_t ld) {
 
 size_t leaf_big_lds_bytes(uint32_t d, uint32_t ld) { return big_lds_layout(d, ld).total; }
 
+// One 64-row block of a bucket, by the whole workgroup (kBigThreads).
 template <uint32_t HALF, uint32_t KS>
-__global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigBlock* __restrict__ blocks) {
+__device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const BigBlock bb = blocks[blockIdx.x];
     const uint32_t b = bb.size, r0 = bb.row0;
     const uint32_t nrows = min(kBigRows, b - r0);
     const uint32_t d = a.d, ld = a.ld, k = a.k;
@@ -583,6 +588,52 @@ __global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigB
 }
 
 template <uint32_t HALF, uint32_t KS>
+__global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigBlock* __restrict__ blocks) {
+    big_block<HALF, KS>(a, blocks[blockIdx.x]);
+}
+
+struct LeafQueue;  // knng_kernels.hpp
+
+// ---- batched form: one persistent kernel over size-binned work queues -----------------------
+// The round's work is one list: the 64-row blocks of the buckets above the LDS-resident limit
+// first, then the LDS-resident buckets by decreasing size. A fixed grid of workgroups (the chip's
+// resident slots, 16 waves each) takes items in list order from one device counter, so the
+// longest items start first and no launch tail waits for a short one. LDS is sized once, for the
+// limit bucket or a big-bucket block, whichever needs more, so the launch is the same every round
+// and can be replayed from a graph.
+__global__ void k_leaf_queue_reset(LeafQueue* q) {
+    if (threadIdx.x == 0) q->next = 0;
+}
+
+// The two item kinds as separate functions: each gets its own register allocation, as in its
+// stand-alone kernel, instead of one allocation for both.
+template <uint32_t HALF, uint32_t KS>
+__device__ __attribute__((noinline)) void big_block_item(const LeafArgs& a, const BigBlock bb) {
+    big_block<HALF, KS>(a, bb);
+}
+template <uint32_t HALF, uint32_t KS>
+__device__ __attribute__((noinline)) void leaf_bucket_item(const LeafArgs& a, const LeafDesc lf) {
+    leaf_bucket<HALF, KS, kLeafWavesLarge>(a, lf);
+}
+
+template <uint32_t HALF, uint32_t KS>
+__global__ __launch_bounds__(kBigThreads) void k_leaf_batched(LeafArgs a, const BigBlock* __restrict__ blocks,
+                                                              LeafQueue* __restrict__ q) {
+    static_assert(kBigThreads == kLeafWavesLarge * 64, "both item kinds take 16 waves");
+    __shared__ uint32_t s_item;
+    const uint32_t nbig = q->nbig, nleaf = q->nleaf;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&q->next, 1u);
+        __syncthreads();
+        const uint32_t it = s_item;
+        if (it < nbig) big_block_item<HALF, KS>(a, blocks[it]);
+        else if (it - nbig < nleaf) leaf_bucket_item<HALF, KS>(a, a.leaves[it - nbig]);
+        __syncthreads();  // the item's LDS use and s_item read are over before the next item
+        if (it >= nbig + nleaf) break;
+    }
+}
+
+template <uint32_t HALF, uint32_t KS>
 static void launch_leaf_big_t(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks, size_t lds,
                               hipStream_t st) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf_big<HALF, KS>),
@@ -614,6 +665,53 @@ void launch_leaf_big(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks
 
 size_t leaf_lds_bytes(uint32_t max_leaf, uint32_t d, uint32_t ld) {
     return leaf_lds_layout(max_leaf, d, ld).total;
+}
+
+uint32_t leaf_batched_limit(uint32_t d, uint32_t ld) {
+    uint32_t lim = 0;
+    for (uint32_t b = 16; b <= kMaxLeaf; b += 16)
+        if (leaf_lds_layout(b, d, ld, kLeafWavesLarge).total <= 160 * 1024) lim = b;
+    return lim;
+}
+
+size_t leaf_batched_lds(uint32_t d, uint32_t ld) {
+    const uint32_t lim = leaf_batched_limit(d, ld);
+    const size_t a = lim ? leaf_lds_layout(lim, d, ld, kLeafWavesLarge).total : 0;
+    return std::max(a, leaf_big_lds_bytes(d, ld));
+}
+
+template <uint32_t HALF, uint32_t KS>
+static void launch_leaf_batched_t(const LeafArgs& a, const BigBlock* blocks, LeafQueue* q, hipStream_t st) {
+    const size_t lds = leaf_batched_lds(a.d, a.ld);
+    auto kern = &k_leaf_batched<HALF, KS>;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBigThreads, lds);
+    const uint32_t grid = (uint32_t)std::max(1, per_cu * cus);
+    k_leaf_queue_reset<<<1, 64, 0, st>>>(q);
+    kern<<<grid, kBigThreads, lds, st>>>(a, blocks, q);
+}
+
+template <uint32_t HALF>
+static void launch_leaf_batched_h(const LeafArgs& a, const BigBlock* blocks, LeafQueue* q, hipStream_t st) {
+    switch ((a.d + 15) / 16) {
+        case 1: launch_leaf_batched_t<HALF, 1>(a, blocks, q, st); break;
+        case 2: launch_leaf_batched_t<HALF, 2>(a, blocks, q, st); break;
+        case 3: launch_leaf_batched_t<HALF, 3>(a, blocks, q, st); break;
+        case 4: launch_leaf_batched_t<HALF, 4>(a, blocks, q, st); break;
+        case 5: launch_leaf_batched_t<HALF, 5>(a, blocks, q, st); break;
+        case 6: launch_leaf_batched_t<HALF, 6>(a, blocks, q, st); break;
+        case 7: launch_leaf_batched_t<HALF, 7>(a, blocks, q, st); break;
+        default: launch_leaf_batched_t<HALF, 8>(a, blocks, q, st); break;
+    }
+}
+
+void launch_leaf_batched(const LeafArgs& a, const BigBlock* blocks, LeafQueue* q, hipStream_t st) {
+    if (a.half_kind == 0) launch_leaf_batched_h<0>(a, blocks, q, st);
+    else launch_leaf_batched_h<1>(a, blocks, q, st);
 }
 
 template <uint32_t HALF, uint32_t KS, uint32_t W>

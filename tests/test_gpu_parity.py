@@ -243,6 +243,36 @@ def test_leaf_stage_autotune_matches_default_shape():
     np.testing.assert_array_equal(ds1, ds0)
 
 
+@pytest.mark.parametrize("flags", [K.FLAG_LEAF_BATCHED, K.FLAG_LEAF_GRAPH])
+def test_batched_leaf_stage_equals_binned_launches(flags):
+    # One persistent kernel over the round's work list (and its graph replays) against one launch
+    # per size bin: buckets up to 1500 points (blocks of the big-bucket path) and small ones, three
+    # rounds in a row, each call's lists compared (a graph is replayed on every call after the
+    # first, with host work and synchronisations between the replays).
+    X = mixture(30000, 100, 60, 81)
+    k = 50
+    kw = dict(k=k, leaf_max=1500, small_max=3000, iterations=1, seed=8)
+    o_prm = oracle.make_params(**kw)
+    ref = K.Builder(K.params(**kw))
+    bat = K.Builder(K.params(flags=flags, **kw))
+    st_r = st_b = None
+    for rnd in range(3):
+        perm, ls, lz = oracle.split_round(X, o_prm, rnd)
+        if rnd == 0:
+            assert lz.max() > 512 and lz.min() < 256
+        st_r = ref.leaf_stage(X, perm, ls, lz, state=st_r)
+        st_b = bat.leaf_stage(X, perm, ls, lz, state=st_b)
+        np.testing.assert_array_equal(st_b[1], st_r[1])
+        np.testing.assert_array_equal(st_b[0], st_r[0])
+    # whole builds: rounds alternate between two graphs (one per round parity)
+    bkw = dict(k=50, leaf_max=1200, leaf_max_odd=300, small_max=3000, iterations=1, rounds=5,
+               non_passes=1, seed=3)
+    ids0, ds0 = K.Builder(K.params(**bkw)).build(X, return_dists=True)
+    ids1, ds1 = K.Builder(K.params(flags=flags, **bkw)).build(X, return_dists=True)
+    np.testing.assert_array_equal(ds1, ds0)
+    np.testing.assert_array_equal(ids1, ids0)
+
+
 @pytest.mark.parametrize("flags", [0, K.FLAG_DIRECT_NON])
 def test_non_matches_oracle(flags):
     X = mixture(6000, 100, 30, 31)
