@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--leaf-mode", default="binned", choices=["binned", "batched", "graph"],
                     help="leaf stage: one launch per bucket-size bin; one persistent kernel over the "
                          "round's work list; the same replayed from a captured hipGraph")
+    ap.add_argument("--non-fp32-gram", action="store_true",
+                    help="neighbours-of-neighbours screen on the FP32 Gram of the rows instead of the "
+                         "fp16 Gram of the rows (each scaled by its power of two, exact margin)")
     ap.add_argument("--non-screen16", action="store_true",
                     help="neighbours-of-neighbours screen on a 16-bit Gram of rows centred on each "
                          "q instead of the FP32 Gram of the rows")
@@ -301,7 +304,8 @@ def main():
 
     prec = {"fp32": K.PREC_FP32, "fp16": K.PREC_FP16, "bf16": K.PREC_BF16}[a.precision]
     flags = ((K.FLAG_KD_SPLIT if a.splitter == "kd" else 0) | (K.FLAG_AUTOTUNE if a.autotune else 0)
-             | (K.FLAG_NON_SCREEN16 if a.non_screen16 else 0) | leaf_flags(a, K))
+             | (K.FLAG_NON_SCREEN16 if a.non_screen16 else 0) | (K.FLAG_NON_FP32_GRAM if a.non_fp32_gram else 0)
+             | leaf_flags(a, K))
     prm = K.params(k=a.k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
                    rounds=a.rounds, non_passes=a.non, precision=prec, seed=a.seed, flags=flags)
     b = K.Builder(prm)
@@ -425,11 +429,14 @@ def main():
             err = np.concatenate(errs)
             relerr = np.concatenate(rels)
             del Xh, ds_all
+            non_screen = ("fp32 Gram with exact error margin" if a.non_fp32_gram or d not in (61, 62, 63, 64, 97, 98, 99, 100, 125, 126, 127, 128)
+                          else "16-bit Gram of rows centred on each q" + (" with exact error margin" if a.precision == "fp32" else ", no margin")
+                          if a.non_screen16 else "fp16 Gram of power-of-two-scaled rows with exact error margin")
             prec_block = {"precision": a.precision,
                           "screen": {"fp32": "leaf: fp16 Gram with exact error margin",
                                      "fp16": "leaf: fp16 Gram, no margin",
                                      "bf16": "leaf: bf16 Gram, no margin"}[a.precision]
-                                    + "; neighbours of neighbours: fp32 Gram with exact error margin",
+                                    + "; neighbours of neighbours: " + non_screen,
                           "rerank": "exact reference-order FP32 for every screen survivor",
                           "dist_abs_err_vs_fp64": {"max": float(err.max()), "mean": float(err.mean())},
                           "dist_rel_err_vs_fp64": {"max": float(relerr.max()),

@@ -49,6 +49,12 @@ extern "C" {
                                         in the FP32 form, so the same lists; bf16/fp16 without a
                                         margin in those forms). Measured slower at 10M x 100: the
                                         exact re-rank then reads S(q)'s rows from global memory */
+#define KNNG_FLAG_NON_FP32_GRAM 512u /* neighbours-of-neighbours screen on the FP32 Gram (16x16x4
+                                        MFMA) of the rows. Without it, for d of 61-64, 97-100 or
+                                        125-128, the screen is an fp16 Gram (16x16x32 MFMA) of each
+                                        row scaled by its power of two, read from the FP32 rows the
+                                        exact re-rank uses, with the rigorous fp16 margin: the same
+                                        lists, fewer MFMA cycles */
 #define KNNG_FLAG_LEAF_BATCHED 128u  /* leaf stage as one persistent kernel over one work list per
                                         round (big-bucket blocks first, then buckets by decreasing
                                         size) instead of one launch per size bin; same lists */

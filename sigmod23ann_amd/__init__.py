@@ -39,6 +39,7 @@ FLAG_BIG_LEAF = 2
 FLAG_INCREMENTAL_NON = 4
 FLAG_SMALL_NON_CAP = 8  # tests: tiny candidate buffers, so the overflow path runs
 FLAG_NON_SCREEN16 = 64  # neighbours-of-neighbours screen on a 16-bit Gram of rows centred on q
+FLAG_NON_FP32_GRAM = 512  # neighbours-of-neighbours screen on the FP32 Gram (not the fp16 one)
 FLAG_LEAF_BATCHED = 128  # leaf stage: one persistent kernel over the round's work list
 FLAG_LEAF_GRAPH = 256    # ... replayed from a captured hipGraph
 FLAG_KD_SPLIT = 16  # random-rotation KD splitter (SolutionRandomKD) instead of 2-means

@@ -307,7 +307,7 @@ struct Builder {
     bool have_prev = false;  // snap_prev holds the previous pass's snapshot of the current lists
     uint32_t pass_no = 0;    // neighbours-of-neighbours passes so far in this build
     DevBuf<uint64_t> roff, scan_tmp;
-    DevBuf<float> nbuf_d, norms32;
+    DevBuf<float> nbuf_d, norms32, scales32;
     // the second candidate-buffer set: sub-passes alternate, so a sub-pass's merge (on sst) runs
     // beside the next sub-pass's screen
     DevBuf<uint32_t> ncnt2, nbuf_id2;
@@ -1083,9 +1083,11 @@ struct Builder {
         // later passes of a build screen only offers with a new hop (not in the ordered form: an
         // old entry can cross the rank threshold and be expanded for the first time)
         const bool incremental = have_prev && (prm.flags & KNNG_FLAG_INCREMENTAL_NON) && prm.non_rank_lo == 0;
-        const uint32_t screen16 = non_screen16_supported(d) && (prm.flags & KNNG_FLAG_NON_SCREEN16)
-                                      ? (prm.precision == KNNG_PREC_BF16 ? 2u : 1u)
-                                      : 0u;
+        // 3 (default): fp16 Gram of the scaled rows with the exact margin, in every precision
+        // form; 0: FP32 Gram; 1/2: 16-bit Gram of rows centred on q (opt-in, slower)
+        const uint32_t screen16 = !non_screen16_supported(d) ? 0u
+                                  : (prm.flags & KNNG_FLAG_NON_SCREEN16) ? (prm.precision == KNNG_PREC_BF16 ? 2u : 1u)
+                                  : (prm.flags & KNNG_FLAG_NON_FP32_GRAM) ? 0u : 3u;
         const bool reverse = ld <= 128 && non_reverse_lds(k, ld, d, screen16) <= 160 * 1024 &&
                              !(prm.flags & KNNG_FLAG_DIRECT_NON);
         if (reverse) {  // the snapshot kernel also counts the reverse lists (and, in full
@@ -1166,11 +1168,13 @@ struct Builder {
         a.cap = non_cap();
         a.half_kind = prm.precision == KNNG_PREC_BF16 ? 1 : 0;
         a.screen16 = screen16;
-        a.exact = prm.precision == KNNG_PREC_FP32 ? 1u : 0u;
+        a.exact = prm.precision == KNNG_PREC_FP32 || screen16 == 3 ? 1u : 0u;
         a.counters = verbose ? counters.p : nullptr;  // same-address atomics: diagnostics only
         KNNG_TRY(norms32.ensure(n));
-        launch_row_norms(X, ld, d, n, norms32.p, st);
+        if (screen16 == 3) KNNG_TRY(scales32.ensure(n));
+        launch_row_norms(X, ld, d, n, norms32.p, screen16 == 3 ? scales32.p : nullptr, st);
         a.norms = norms32.p;
+        a.scales = screen16 == 3 ? scales32.p : nullptr;
         // Sub-passes over interleaved parts of the tree order (every subs-th q, so a point's q are
         // spread over all of them), each merged before the next: the bounds tighten as the pass
         // goes, which keeps the candidate buffers small.
@@ -2019,11 +2023,11 @@ uint64_t knng_job_footprint(uint32_t n, uint32_t d, const knng_params* params) {
     const bool kd = (params->flags & KNNG_FLAG_KD_SPLIT) != 0;
     // Per point: the points copy; top-k ids/dists; outputs; the snapshot, its reverse-list slots
     // and the reverse lists (plus the previous snapshot in incremental passes); two sets of NoN
-    // candidate buffers; three permutations, side flags, bounds, norms, seven per-point counters,
+    // candidate buffers; three permutations, side flags, bounds, norms, fp16 scales, seven per-point counters,
     // overflow list, reverse offsets; two sets of leaf descriptors (at most one leaf per point);
     // projected rows for the KD splitter.
     uint64_t per_point = 4 * ld + 8 * k + 8 * k + (incremental ? 16 : 12) * k +
-                         16 * (uint64_t)Builder::kNonCap + 12 + 1 + 4 + 4 + 28 + 4 + 8 + 16;
+                         16 * (uint64_t)Builder::kNonCap + 12 + 1 + 4 + 4 + 4 + 28 + 4 + 8 + 16;
     if (kd) per_point += 4 * (uint64_t)knng::kKdStride;
     // Split tables of one level: at most one node per leaf_max + 1 points, each with its
     // descriptor, three centre/hyperplane rows, an offset and two FP64 partial rows per chunk

@@ -277,7 +277,10 @@ constexpr uint32_t kNrOffSrid = kNrOffSpid + kNrSurv * 4;            // accepted
 constexpr uint32_t kNrOffSd = kNrOffSrid + kNrSurv * 4;
 constexpr uint32_t kNrOffCnt = kNrOffSd + kNrSurv * 4;               // [0] accepted, [1] screened, [2] new
 constexpr uint32_t kNrOffXq = (kNrOffCnt + 16 + 15) & ~15u;         // q's FP32 row (16-bit screens)
-constexpr uint32_t kNrFixed = kNrOffXq + 128 * 4;
+constexpr uint32_t kNrOffNbs = kNrOffXq + 128 * 4;                   // next S(q): row scales (fp16 Gram)
+constexpr uint32_t kNrOffBscl = kNrOffNbs + 128 * 4;                 // S(q)'s row scales
+constexpr uint32_t kNrOffAscl = kNrOffBscl + 128 * 4;                // chunk rows' scales
+constexpr uint32_t kNrFixed = kNrOffAscl + kNrChunk * 4;
 static_assert(kNrOffMeta % 8 == 0, "s_meta holds 64-bit words");
 
 struct NrLds {
@@ -290,6 +293,7 @@ struct NrLds {
 // their FP32 rows from global memory), beside the chunk's FP32 and 16-bit rows, so two workgroups
 // still share a CU.
 __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld, uint32_t d, uint32_t screen16) {
+    if (screen16 == 3) screen16 = 0;  // fp16 Gram of the FP32 rows in LDS: the FP32 layout
     NrLds L;
     L.kb16 = (k + 15) & ~15u;
     L.ldp = nr_row_stride(ld);
@@ -499,6 +503,33 @@ __device__ inline void centre_row16(float4 v0, float4 v1, float4 x0, float4 x1, 
     if (64 + 4 * sc < lim) *reinterpret_cast<uint2*>(row + 64 + 4 * sc) = make_uint2(w[2], w[3]);
 }
 
+// fp16 Gram of a 16x16 tile straight from FP32 rows in LDS: lane group g = lane >> 4 takes dims
+// 32 m + 8 g .. + 7 of k-step m for the row (arow) and the column (brow) alike, each row scaled by
+// its power of two (sa, sb: exact) into fp16 range; dims at or past 4 KS read as zero. Both operands
+// are converted per tile: holding the strip's converted rows across tiles costs registers the
+// kernel does not have (measured: spills, 313 ms per launch against 217).
+template <uint32_t KS>
+__device__ inline float4_t gram_tile_f16s(const float* __restrict__ arow, const float* __restrict__ brow,
+                                          float sa, float sb, uint32_t g) {
+    constexpr uint32_t K32 = (4 * KS + 31) / 32;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4_t c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (uint32_t m = 0; m < K32; ++m) {
+        const uint32_t q0 = 8 * m + 2 * g;  // float4 index of the lane's first dims
+        const float4 a0 = q0 < KS ? reinterpret_cast<const float4*>(arow)[q0] : z;
+        const float4 a1 = q0 + 1 < KS ? reinterpret_cast<const float4*>(arow)[q0 + 1] : z;
+        const float4 b0 = q0 < KS ? reinterpret_cast<const float4*>(brow)[q0] : z;
+        const float4 b1 = q0 + 1 < KS ? reinterpret_cast<const float4*>(brow)[q0 + 1] : z;
+        const half8_t ha = {(_Float16)(a0.x * sa), (_Float16)(a0.y * sa), (_Float16)(a0.z * sa), (_Float16)(a0.w * sa),
+                            (_Float16)(a1.x * sa), (_Float16)(a1.y * sa), (_Float16)(a1.z * sa), (_Float16)(a1.w * sa)};
+        const half8_t hb = {(_Float16)(b0.x * sb), (_Float16)(b0.y * sb), (_Float16)(b0.z * sb), (_Float16)(b0.w * sb),
+                            (_Float16)(b1.x * sb), (_Float16)(b1.y * sb), (_Float16)(b1.z * sb), (_Float16)(b1.w * sb)};
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ha, hb, c, 0, 0, 0);
+    }
+    return c;
+}
+
 template <uint32_t SCR>
 __device__ inline float4_t mfma32(const uint16_t* a, const uint16_t* b, float4_t c) {
     const uint4 av = *reinterpret_cast<const uint4*>(a), bv = *reinterpret_cast<const uint4*>(b);
@@ -546,6 +577,10 @@ void k_non_reverse(NonRevArgs a) {
     uint32_t* s_rsbuf1 = reinterpret_cast<uint32_t*>(lds + kNrOffRsid2);
     uint32_t* s_nsq = reinterpret_cast<uint32_t*>(lds + kNrOffNsq);  // next S(q): snapshot slots
     float* s_nbn = reinterpret_cast<float*>(lds + kNrOffNbn);        // ... and their norms
+    float* s_nbs = reinterpret_cast<float*>(lds + kNrOffNbs);        // ... and their row scales
+    float* s_bscl = reinterpret_cast<float*>(lds + kNrOffBscl);
+    float* s_ascl = reinterpret_cast<float*>(lds + kNrOffAscl);
+    constexpr bool kScaled = SCR == 3;  // fp16 Gram of the FP32 LDS rows, per-row scales
     uint64_t* s_meta = reinterpret_cast<uint64_t*>(lds + kNrOffMeta);  // next q: q, roff[q], roff[q+1]
     uint32_t* s_aid = reinterpret_cast<uint32_t*>(lds + kNrOffAid);
     float* s_abnd = reinterpret_cast<float*>(lds + kNrOffAbnd);
@@ -568,7 +603,7 @@ void k_non_reverse(NonRevArgs a) {
     float4 pf_row[kNrPfRows];
     uint32_t pf_snap[kNrPfSnap];
     uint32_t pf_id = kNoId, pf_na = 0;
-    float pf_bnd = 0.f, pf_nrm = 0.f;
+    float pf_bnd = 0.f, pf_nrm = 0.f, pf_scl = 0.f;
     // ids[sr] for chunk rows sr < na (16 threads per row): bound, norm, row, snapshot -> registers
     auto fetch_rows = [&](uint32_t tid, const uint32_t* ids, uint32_t na) {
         const uint32_t sr = tid >> 4, sc = tid & 15;
@@ -577,6 +612,7 @@ void k_non_reverse(NonRevArgs a) {
         if (sc == 0 && sr < na) {
             pf_bnd = a.bound[pf_id];
             pf_nrm = a.norms[pf_id];
+            if constexpr (kScaled) pf_scl = a.scales[pf_id];
         }
         const float* xr = a.X + (size_t)(sr < na ? pf_id : 0) * ld;
 #pragma unroll
@@ -602,7 +638,7 @@ void k_non_reverse(NonRevArgs a) {
     float* s_stage = reinterpret_cast<float*>(lds + L.off_b16);
     float* s_xq = reinterpret_cast<float*>(lds + kNrOffXq);
     auto convert_s16 = [&](uint32_t tid) {
-        if constexpr (SCR > 0) {
+        if constexpr (SCR == 1 || SCR == 2) {
             const uint32_t sr = tid >> 4, sc = tid & 15;
             const float4 x0 = sc < d4 ? reinterpret_cast<const float4*>(s_xq)[sc] : zero4;
             const float4 x1 = 16 + sc < d4 ? reinterpret_cast<const float4*>(s_xq)[16 + sc] : zero4;
@@ -640,6 +676,8 @@ void k_non_reverse(NonRevArgs a) {
         if (wave == 0) {
             const uint32_t v0 = lane < k ? s_nsq[lane] : 0u, v1 = lane + 64 < k ? s_nsq[lane + 64] : 0u;
             const float bn0 = lane < k ? s_nbn[lane] : 0.f, bn1 = lane + 64 < k ? s_nbn[lane + 64] : 0.f;
+            const float bs0 = kScaled && lane < k ? s_nbs[lane] : 0.f;
+            const float bs1 = kScaled && lane + 64 < k ? s_nbs[lane + 64] : 0.f;
             const bool n0 = lane < k && (v0 & kNewBit), n1 = lane + 64 < k && (v1 & kNewBit);
             const uint64_t m0 = __ballot(n0), m1 = __ballot(n1);
             const uint64_t below = (1ull << lane) - 1ull;
@@ -649,17 +687,20 @@ void k_non_reverse(NonRevArgs a) {
                 const uint32_t pos = n0 ? b0 : nnew + lane - b0, id = v0 & kIdMask;
                 s_bid[pos] = id;
                 s_bnrm[pos] = bn0;
+                if constexpr (kScaled) s_bscl[pos] = bs0;
             }
             if (lane + 64 < k) {
                 const uint32_t pos = n1 ? b1 : nnew + lane + 64 - b1, id = v1 & kIdMask;
                 s_bid[pos] = id;
                 s_bnrm[pos] = bn1;
+                if constexpr (kScaled) s_bscl[pos] = bs1;
             }
             if (lane == 0) s_cnt[2] = nnew;
         }
         for (uint32_t t = k + tid; t < L.kb16; t += kNrThreads) {
             s_bid[t] = kNoId;
             s_bnrm[t] = 0.f;
+            if constexpr (kScaled) s_bscl[t] = 0.f;
         }
     };
     // after prep_order and a barrier: S's ids hashed, S's rows in flight (to LDS for the FP32
@@ -683,12 +724,12 @@ void k_non_reverse(NonRevArgs a) {
         // are: the Gram reads dims below 4 ceil(d/4) only, a Gram column depends on its own row
         // only and columns past k are never offered, and the exact distances read dims below d of
         // rows below k.
-        if constexpr (SCR > 0) {  // q's row, for centring
+        if constexpr (SCR == 1 || SCR == 2) {  // q's row, for centring
             if (wave == 0 && (tid & 63) < d4)
                 __builtin_amdgcn_global_load_lds(a.X + (size_t)qq * ld + 4 * (tid & 63),
                                                  (__attribute__((address_space(3))) void*)s_xq, 16, 0, 0);
         }
-        float* s_dst = SCR ? s_stage : s_brow;
+        float* s_dst = SCR == 1 || SCR == 2 ? s_stage : s_brow;
         __attribute__((address_space(3))) float* s_dst_lds = (__attribute__((address_space(3))) float*)(s_dst);
         const uint32_t tot = L.kb16 * ldp4;
 #pragma unroll
@@ -711,6 +752,7 @@ void k_non_reverse(NonRevArgs a) {
         const uint32_t v = a.snap[(size_t)q * k + t];
         s_nsq[t] = v;
         s_nbn[t] = a.norms[v & kIdMask];
+        if constexpr (kScaled) s_nbs[t] = a.scales[v & kIdMask];
     }
     for (uint32_t t = tid0; t < min(a1 - a0, (uint64_t)kNrIds); t += kNrThreads) s_rsbuf0[t] = a.rs[a0 + t];
     uint32_t nq_l = 0;  // the next q's position (lane 0 of each wave)
@@ -733,7 +775,7 @@ void k_non_reverse(NonRevArgs a) {
     uint32_t tid = tid0;
     asm volatile("" : "+v"(tid));
     const uint32_t lane = tid & 63, wave = tid >> 6;
-    if constexpr (SCR > 0) {  // S(q)'s staged rows -> 16-bit rows centred on q
+    if constexpr (SCR == 1 || SCR == 2) {  // S(q)'s staged rows -> 16-bit rows centred on q
         if (a1 > a0) {
             convert_s16(tid);
             __syncthreads();
@@ -782,6 +824,12 @@ void k_non_reverse(NonRevArgs a) {
                 __builtin_amdgcn_global_load_lds(a.norms + (s_nsq[lane] & kIdMask), (kLdsU32*)s_nbn, 4, 0, 0);
             if (lane + 64 < k)
                 __builtin_amdgcn_global_load_lds(a.norms + (s_nsq[lane + 64] & kIdMask), (kLdsU32*)(s_nbn + 64), 4, 0, 0);
+            if constexpr (kScaled) {
+                if (lane < k)
+                    __builtin_amdgcn_global_load_lds(a.scales + (s_nsq[lane] & kIdMask), (kLdsU32*)s_nbs, 4, 0, 0);
+                if (lane + 64 < k)
+                    __builtin_amdgcn_global_load_lds(a.scales + (s_nsq[lane + 64] & kIdMask), (kLdsU32*)(s_nbs + 64), 4, 0, 0);
+            }
         }
     };
     // old rows pair with S(q)'s new entries only; with none of either there is nothing to offer
@@ -803,13 +851,14 @@ void k_non_reverse(NonRevArgs a) {
             s_aid[sr] = pf_id;
             s_abnd[sr] = sr < pf_na ? pf_bnd : 0.f;
             s_anrm[sr] = sr < pf_na ? pf_nrm : 0.f;
+            if constexpr (kScaled) s_ascl[sr] = sr < pf_na ? pf_scl : 0.f;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kNrPfRows; ++u) {
             const uint32_t q4 = sc + 16 * u;
             if (q4 < ldp4) reinterpret_cast<float4*>(s_arow)[sr * ldp4 + q4] = pf_row[u];
         }
-        if constexpr (SCR > 0) {  // the row centred on q, 16-bit, with its centred norm
+        if constexpr (SCR == 1 || SCR == 2) {  // the row centred on q, 16-bit, with its centred norm
             static_assert(kNrPfRows == 2, "a thread holds dims 4 sc .. and 64 + 4 sc ..");
             float nrm, rinv;
             const bool v = sr < pf_na;
@@ -849,7 +898,7 @@ void k_non_reverse(NonRevArgs a) {
     // fp16 (bf16 screens are only used without a margin).
     const bool exact16 = a.exact != 0;
     const float margin_c = SCR == 0 ? (float)(4 * d4 + 8) * (1.0f / 8388608.0f)
-                                    : HalfTraits<SCR == 1 ? 0 : 1>::kMargin;
+                                    : HalfTraits<SCR == 2 ? 1 : 0>::kMargin;
     const uint32_t ksteps = d4;
     const uint32_t ntile = L.kb16 / 16;
     const uint32_t lrow = lane & 15, lk = lane >> 4;
@@ -876,7 +925,7 @@ void k_non_reverse(NonRevArgs a) {
                 const uint32_t ent = act ? s_queue[e] : 0u;
                 const uint32_t i = ent >> 7, cc = ent & 127;
                 // S(q)'s FP32 rows: in LDS for the FP32 Gram, in global memory for the 16-bit screens
-                const float* rb = SCR == 0 ? s_brow + cc * ldp : a.X + (size_t)s_bid[cc] * ld;
+                const float* rb = SCR == 0 || SCR == 3 ? s_brow + cc * ldp : a.X + (size_t)s_bid[cc] * ld;
                 const float* ra = s_arow + i * ldp;
                 float ex = 0.f;
                 if (wide) {
@@ -946,7 +995,10 @@ void k_non_reverse(NonRevArgs a) {
             const uint16_t* a16 = s_a16 + (16 * strip + lrow) * rs16 + 8 * (lane >> 4);
             float m2r[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) m2r[i] = SCR ? -2.0f * s_arinv[rr[i]] : 0.f;
+            for (int i = 0; i < 4; ++i)
+                m2r[i] = SCR == 3 ? -2.0f * ldexpf(1.0f, -(int)s_ascl[rr[i]])
+                                  : SCR ? -2.0f * s_arinv[rr[i]] : 0.f;
+            const float sa = SCR == 3 ? ldexpf(1.0f, (int)s_ascl[16 * strip + lrow]) : 0.f;
             for (uint32_t j = wv; j < nstrip * tt; j += kNrThreads / 64) {
                 const uint32_t t = nstrip == 2 ? j >> 1 : j;
                 if (qn + 256 > kNrQueue) flush();
@@ -957,6 +1009,9 @@ void k_non_reverse(NonRevArgs a) {
                 float4_t acc;
                 if constexpr (SCR == 0) {
                     acc = gram_tile_f32<KS>(arow, s_brow + (16 * t + lrow) * ldp, lk, ksteps);
+                } else if constexpr (SCR == 3) {
+                    const float sb = ldexpf(1.0f, (int)s_bscl[16 * t + lrow]);
+                    acc = gram_tile_f16s<KS>(arow, s_brow + (16 * t + lrow) * ldp, sa, sb, lane >> 4);
                 } else {
                     const uint16_t* b16 = s_b16 + (16 * t + lrow) * rs16 + 8 * (lane >> 4);
                     acc = float4_t{0.f, 0.f, 0.f, 0.f};
@@ -966,7 +1021,7 @@ void k_non_reverse(NonRevArgs a) {
                 const uint32_t c = 16 * t + lrow, cb = c & 31;
                 const bool cok = c < k;
                 const float cn = s_bnrm[c];  // zero past k
-                const float cinv = SCR ? s_brinv[c] : 0.f;
+                const float cinv = SCR == 3 ? ldexpf(1.0f, -(int)s_bscl[c]) : SCR ? s_brinv[c] : 0.f;
                 const uint32_t rid = s_bid[c];
                 // compact the survivors of this tile into the wave's queue as (row << 7 | col)
                 const uint32_t qn_before = qn;
@@ -1038,7 +1093,7 @@ void k_non_reverse(NonRevArgs a) {
     for (uint32_t u = 0; u < kNrPfSnap; ++u) pf_snap[u] = 0u;
     pf_id = kNoId;
     pf_na = 0;
-    pf_bnd = pf_nrm = 0.f;
+    pf_bnd = pf_nrm = pf_scl = 0.f;
     if (!has_next) {
         append();
         break;
@@ -1150,28 +1205,40 @@ __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a, const 
 // FP32 squared norms for the screen (any summation order: the screen's margin covers it). Sixteen
 // lanes per row, float4 loads, so a wave reads four whole rows at a time.
 __global__ __launch_bounds__(256) void k_row_norms(const float* __restrict__ X, uint32_t ld,
-                                                   uint32_t d, uint32_t n, float* __restrict__ out) {
+                                                   uint32_t d, uint32_t n, float* __restrict__ out,
+                                                   float* __restrict__ scales) {
     const uint32_t i = blockIdx.x * 16 + (threadIdx.x >> 4), sc = threadIdx.x & 15;
-    const uint32_t ld4 = ld / 4;
-    float s = 0.f;
+    const uint32_t d4 = (d + 3) / 4;
+    float s = 0.f, m = 0.f;
     if (i < n) {
         const float4* x = reinterpret_cast<const float4*>(X + (size_t)i * ld);
-        for (uint32_t q = sc; q < ld4; q += 16) {
+        for (uint32_t q = sc; q < d4; q += 16) {
             const float4 v = x[q];
             const uint32_t j = 4 * q;
-            if (j + 0 < d) s = __builtin_fmaf(v.x, v.x, s);
-            if (j + 1 < d) s = __builtin_fmaf(v.y, v.y, s);
-            if (j + 2 < d) s = __builtin_fmaf(v.z, v.z, s);
-            if (j + 3 < d) s = __builtin_fmaf(v.w, v.w, s);
+            if (j + 0 < d) { s = __builtin_fmaf(v.x, v.x, s); m = fmaxf(m, fabsf(v.x)); }
+            if (j + 1 < d) { s = __builtin_fmaf(v.y, v.y, s); m = fmaxf(m, fabsf(v.y)); }
+            if (j + 2 < d) { s = __builtin_fmaf(v.z, v.z, s); m = fmaxf(m, fabsf(v.z)); }
+            if (j + 3 < d) { s = __builtin_fmaf(v.w, v.w, s); m = fmaxf(m, fabsf(v.w)); }
         }
     }
-    for (uint32_t o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
-    if (i < n && sc == 0) out[i] = s;
+    for (uint32_t o = 8; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 16);
+        m = fmaxf(m, __shfl_xor(m, o, 16));
+    }
+    if (i < n && sc == 0) {
+        out[i] = s;
+        // fp16 screen: the row's power-of-two scale exponent, largest |value| below 2^14
+        if (scales) {
+            int e = 0;
+            if (m > 0.f) frexpf(m, &e);
+            scales[i] = (float)(14 - e);
+        }
+    }
 }
 
 void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float* out,
-                      hipStream_t st) {
-    if (n) k_row_norms<<<(n + 15) / 16, 256, 0, st>>>(X, ld, d, n, out);
+                      float* scales, hipStream_t st) {
+    if (n) k_row_norms<<<(n + 15) / 16, 256, 0, st>>>(X, ld, d, n, out, scales);
 }
 
 // ---- launchers ----------------------------------------------------------------------------------------
@@ -1244,6 +1311,9 @@ void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {
         case 216: go(&k_non_reverse<16, 2>); break;
         case 225: go(&k_non_reverse<25, 2>); break;
         case 232: go(&k_non_reverse<32, 2>); break;
+        case 316: go(&k_non_reverse<16, 3>); break;
+        case 325: go(&k_non_reverse<25, 3>); break;
+        case 332: go(&k_non_reverse<32, 3>); break;
         default: go(&k_non_reverse<0, 0>); break;
     }
 }

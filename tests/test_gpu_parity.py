@@ -288,11 +288,11 @@ def test_non_matches_oracle(flags):
 
 @pytest.mark.parametrize("gen,d", [("lowrank", 64), ("mixture", 100), ("lowrank", 100), ("lowrank", 128),
                                    ("mixture", 37)])
-@pytest.mark.parametrize("flags", [0, K.FLAG_NON_SCREEN16])
+@pytest.mark.parametrize("flags", [0, K.FLAG_NON_FP32_GRAM, K.FLAG_NON_SCREEN16])
 def test_non_screens_match_oracle(gen, d, flags):
-    # the reverse kernel's screens: the FP32 Gram (every width) or the 16-bit Gram of rows centred
-    # on each q with the exact margin (widths 64, 100, 128; others fall back to the FP32 Gram); the
-    # lists must be the oracle's
+    # the reverse kernel's screens, each with its exact margin: the fp16 Gram of the scaled rows
+    # (default for widths 64, 100, 128), the FP32 Gram (every width, the only one for others), the
+    # 16-bit Gram of rows centred on each q; the lists must be the oracle's
     X = lowrank(8000, d, 70 + d) if gen == "lowrank" else mixture(8000, d, 40, 70 + d)
     k = 50
     kw = dict(k=k, leaf_max=128, small_max=3000, iterations=1, seed=12)
