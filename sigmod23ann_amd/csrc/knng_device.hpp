@@ -447,4 +447,40 @@ __device__ inline float4_t mfma16(half4_t a, half4_t b, float4_t c, uint32_t hal
                                                      __builtin_bit_cast(short4_t, b), c, 0, 0, 0);
 }
 
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+// A lane's share of a 16-bit screening row (rows padded to 16 KS dims) for the 16x16 tile Gram on
+// gfx950: k-steps of 32 dims on v_mfma_f32_16x16x32_{f16,bf16} (lane group g = lane >> 4 holds
+// dims 32 m + 8 g .. + 7), and, when KS is odd, one last 16-dim step on 16x16x16 (dims
+// 32 (KS / 2) + 4 g .. + 3). Rows and columns take the same dims per lane, so each product pairs
+// the same dim of both.
+template <uint32_t HALF, uint32_t KS>
+struct HalfRow {
+    static constexpr uint32_t K2 = KS / 2, K1 = KS % 2;
+    uint4 w[K2 ? K2 : 1];
+    half4_t t;
+    __device__ inline void load(const uint16_t* __restrict__ row, uint32_t g) {
+#pragma unroll
+        for (uint32_t m = 0; m < K2; ++m) w[m] = *reinterpret_cast<const uint4*>(row + 32 * m + 8 * g);
+        if constexpr (K1 != 0) t = *reinterpret_cast<const half4_t*>(row + 32 * K2 + 4 * g);
+    }
+};
+
+template <uint32_t HALF, uint32_t KS>
+__device__ inline float4_t gram16(const HalfRow<HALF, KS>& a, const HalfRow<HALF, KS>& b) {
+    float4_t c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (uint32_t m = 0; m < HalfRow<HALF, KS>::K2; ++m) {
+        if constexpr (HALF == 0)
+            c = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8_t, a.w[m]),
+                                                       __builtin_bit_cast(half8_t, b.w[m]), c, 0, 0, 0);
+        else
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a.w[m]),
+                                                        __builtin_bit_cast(bf16x8_t, b.w[m]), c, 0, 0, 0);
+    }
+    if constexpr (HalfRow<HALF, KS>::K1 != 0) c = mfma16(a.t, b.t, c, HALF);
+    return c;
+}
+
 }  // namespace knng

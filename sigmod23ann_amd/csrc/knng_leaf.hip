@@ -4,8 +4,8 @@
 // One workgroup (8 or 16 waves) per bucket, rows resident in LDS:
 //  phase 0  gather the bucket's rows, centre them on the bucket's first row, scale each row by a
 //           power of two into fp16 (or bf16) range; FP32 squared norms of the centred rows.
-//  phase 1  screening Gram on MFMA (16x16x16, fp32 accumulate), one 16-row strip per wave over all
-//           columns: approximate distance n_i + n_j - 2 g_ij against the point's current bound.
+//  phase 1  screening Gram on MFMA (16x16x32, and 16x16x16 for a last 16 dims; fp32 accumulate),
+//           one 16-row strip per wave over all columns: approximate distance n_i + n_j - 2 g_ij against the point's current bound.
 //           Exact mode adds a rigorous error margin, so every pair whose exact FP32 distance beats
 //           the bound survives; survivors go to a per-row bitmask in LDS.
 //  phase 2  one wave per row: exact reference-order FP32 distance for each survivor, then a
@@ -134,12 +134,10 @@ __device__ inline void leaf_bucket(const LeafArgs& a, const LeafDesc lf) {
     // ---- phase 1: screening Gram --------------------------------------------------------------
     const bool exact = a.screen == kScreenExact;
     const float margin_c = HalfTraits<HALF>::kMargin;
-    const uint32_t lrow = lane & 15, lk = (lane >> 4) * 4;
+    const uint32_t lrow = lane & 15;
     for (uint32_t s = wave; s < nstrip; s += kLeafWaves) {
-        half4_t af[KS];
-#pragma unroll
-        for (uint32_t kk = 0; kk < KS; ++kk)
-            af[kk] = *reinterpret_cast<const half4_t*>(s_rows + (16 * s + lrow) * L.rs + kk * 16 + lk);
+        HalfRow<HALF, KS> af;
+        af.load(s_rows + (16 * s + lrow) * L.rs, lane >> 4);
         float rn[4], rinv[4], thr_base[4];
         uint32_t rr[4];
 #pragma unroll
@@ -151,13 +149,9 @@ __device__ inline void leaf_bucket(const LeafArgs& a, const LeafDesc lf) {
             thr_base[i] = exact ? bnd * (1.0f + 1.0f / 65536.0f) : bnd;
         }
         for (uint32_t t = 0; t < nstrip; ++t) {
-            half4_t bf[KS];
-#pragma unroll
-            for (uint32_t kk = 0; kk < KS; ++kk)
-                bf[kk] = *reinterpret_cast<const half4_t*>(s_rows + (16 * t + lrow) * L.rs + kk * 16 + lk);
-            float4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (uint32_t kk = 0; kk < KS; ++kk) acc = mfma16(af[kk], bf[kk], acc, HALF);
+            HalfRow<HALF, KS> bf;
+            bf.load(s_rows + (16 * t + lrow) * L.rs, lane >> 4);
+            const float4_t acc = gram16(af, bf);
             const uint32_t c = 16 * t + lrow;
             const float cn = s_nrm[c], cinv = s_rinv[c];
             uint64_t bits[4];
@@ -405,7 +399,7 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
 
     const bool exact = a.screen == kScreenExact;
     const float margin_c = HalfTraits<HALF>::kMargin;
-    const uint32_t lrow = lane & 15, lk = (lane >> 4) * 4;
+    const uint32_t lrow = lane & 15;
     const uint32_t strip = wave >> 2, tile = wave & 3;  // phase 1: 4 strips x 4 tiles
     bool changed[kBigRowsPerWave];
     // Offers below a row's bound wait in a pending batch of 64 (LDS) across column blocks and go
@@ -473,10 +467,8 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
 
         // ---- phase 1: screen this wave's tile ------------------------------------------------
         {
-            half4_t af[KS];
-#pragma unroll
-            for (uint32_t kk = 0; kk < KS; ++kk)
-                af[kk] = *reinterpret_cast<const half4_t*>(s_arow + (16 * strip + lrow) * L.rs + kk * 16 + lk);
+            HalfRow<HALF, KS> af;
+            af.load(s_arow + (16 * strip + lrow) * L.rs, lane >> 4);
             float rn[4], rinv[4], thr_base[4];
             uint32_t rr[4];
 #pragma unroll
@@ -489,13 +481,9 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
             }
             {
                 const uint32_t t = tile;
-                half4_t bf[KS];
-#pragma unroll
-                for (uint32_t kk = 0; kk < KS; ++kk)
-                    bf[kk] = *reinterpret_cast<const half4_t*>(s_crow + (16 * t + lrow) * L.rs + kk * 16 + lk);
-                float4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (uint32_t kk = 0; kk < KS; ++kk) acc = mfma16(af[kk], bf[kk], acc, HALF);
+                HalfRow<HALF, KS> bf;
+                bf.load(s_crow + (16 * t + lrow) * L.rs, lane >> 4);
+                const float4_t acc = gram16(af, bf);
                 const uint32_t c = 16 * t + lrow;
                 const float cn = s_cnrm[c], cinv = s_crinv[c];
                 uint64_t bits[4];

@@ -441,8 +441,6 @@ __device__ inline float4_t gram_tile_f32(const float* __restrict__ arow,
 // result), scale the row by a power of two so that its largest |value| is below 2^14 (as the leaf
 // stage's rows), and store it as 16-bit values. Relative rounding per element therefore holds for
 // every element of the row, which is what the screen's margin assumes.
-typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 constexpr uint32_t kNrSPass = 4;  // S(q) rows converted 32 at a time: k <= 128
 
 template <int CTRL>
@@ -568,8 +566,6 @@ void k_non_reverse(NonRevArgs a) {
     uint32_t* s_bid = reinterpret_cast<uint32_t*>(lds + L.off_bid);
     float* s_bnrm = reinterpret_cast<float*>(lds + L.off_bnrm);
     float* s_brow = reinterpret_cast<float*>(lds + L.off_brow);
-    __attribute__((address_space(3))) float* s_brow_lds =
-        (__attribute__((address_space(3))) float*)(s_brow);
     uint32_t* s_hkey = reinterpret_cast<uint32_t*>(lds + kNrOffHkey);
     uint32_t* s_hcnt = reinterpret_cast<uint32_t*>(lds + kNrOffHcnt);
     // reverse ids of the current q and of the next one (the two swap every q)
