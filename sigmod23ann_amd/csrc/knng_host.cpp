@@ -307,7 +307,7 @@ struct Builder {
     bool have_prev = false;  // snap_prev holds the previous pass's snapshot of the current lists
     uint32_t pass_no = 0;    // neighbours-of-neighbours passes so far in this build
     DevBuf<uint64_t> roff, scan_tmp;
-    DevBuf<float> nbuf_d, norms32, scales32;
+    DevBuf<float> nbuf_d, norms32;
     // the second candidate-buffer set: sub-passes alternate, so a sub-pass's merge (on sst) runs
     // beside the next sub-pass's screen
     DevBuf<uint32_t> ncnt2, nbuf_id2;
@@ -1171,10 +1171,8 @@ struct Builder {
         a.exact = prm.precision == KNNG_PREC_FP32 || screen16 == 3 ? 1u : 0u;
         a.counters = verbose ? counters.p : nullptr;  // same-address atomics: diagnostics only
         KNNG_TRY(norms32.ensure(n));
-        if (screen16 == 3) KNNG_TRY(scales32.ensure(n));
-        launch_row_norms(X, ld, d, n, norms32.p, screen16 == 3 ? scales32.p : nullptr, st);
+        launch_row_norms(X, ld, d, n, norms32.p, st);
         a.norms = norms32.p;
-        a.scales = screen16 == 3 ? scales32.p : nullptr;
         // Sub-passes over interleaved parts of the tree order (every subs-th q, so a point's q are
         // spread over all of them), each merged before the next: the bounds tighten as the pass
         // goes, which keeps the candidate buffers small.
@@ -2023,11 +2021,11 @@ uint64_t knng_job_footprint(uint32_t n, uint32_t d, const knng_params* params) {
     const bool kd = (params->flags & KNNG_FLAG_KD_SPLIT) != 0;
     // Per point: the points copy; top-k ids/dists; outputs; the snapshot, its reverse-list slots
     // and the reverse lists (plus the previous snapshot in incremental passes); two sets of NoN
-    // candidate buffers; three permutations, side flags, bounds, norms, fp16 scales, seven per-point counters,
+    // candidate buffers; three permutations, side flags, bounds, norms, seven per-point counters,
     // overflow list, reverse offsets; two sets of leaf descriptors (at most one leaf per point);
     // projected rows for the KD splitter.
     uint64_t per_point = 4 * ld + 8 * k + 8 * k + (incremental ? 16 : 12) * k +
-                         16 * (uint64_t)Builder::kNonCap + 12 + 1 + 4 + 4 + 4 + 28 + 4 + 8 + 16;
+                         16 * (uint64_t)Builder::kNonCap + 12 + 1 + 4 + 4 + 28 + 4 + 8 + 16;
     if (kd) per_point += 4 * (uint64_t)knng::kKdStride;
     // Split tables of one level: at most one node per leaf_max + 1 points, each with its
     // descriptor, three centre/hyperplane rows, an offset and two FP64 partial rows per chunk

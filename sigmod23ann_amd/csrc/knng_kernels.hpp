@@ -160,16 +160,13 @@ struct NonRevArgs {
     uint32_t cap;
     uint32_t half_kind;
     uint32_t screen16;      // 0: FP32 Gram of the rows; 1: fp16, 2: bf16 Gram of rows centred on q;
-                            // 3: fp16 Gram of the rows, each scaled by its power of two (scales)
+                            // 3: fp16 Gram of the rows, each scaled by a power of two from its norm
     uint32_t exact;         // 16-bit screens: 1 = rigorous error margin, 0 = plain approximate test
     const float* norms;     // n FP32 squared norms
-    const float* scales;    // n fp16 scale exponents (screen16 = 3)
     uint64_t* counters;     // [0] screened pairs, [1] appended (nullable)
 };
-// FP32 squared norms; with scales (nullable) also each row's fp16 scale exponent (largest |value|
-// times 2^scale below 2^14).
 void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float* out,
-                      float* scales, hipStream_t st);
+                      hipStream_t st);
 
 // Sorted snapshot; an entry absent from `prev` (nullable: all) carries kNewBit, an expanded one
 // kExpBit. rank_lo = 0: unfilled slots -> id 0, every entry expanded; rank_lo > 0: unfilled

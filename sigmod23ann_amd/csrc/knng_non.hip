@@ -277,8 +277,7 @@ constexpr uint32_t kNrOffSrid = kNrOffSpid + kNrSurv * 4;            // accepted
 constexpr uint32_t kNrOffSd = kNrOffSrid + kNrSurv * 4;
 constexpr uint32_t kNrOffCnt = kNrOffSd + kNrSurv * 4;               // [0] accepted, [1] screened, [2] new
 constexpr uint32_t kNrOffXq = (kNrOffCnt + 16 + 15) & ~15u;         // q's FP32 row (16-bit screens)
-constexpr uint32_t kNrOffNbs = kNrOffXq + 128 * 4;                   // next S(q): row scales (fp16 Gram)
-constexpr uint32_t kNrOffBscl = kNrOffNbs + 128 * 4;                 // S(q)'s row scales
+constexpr uint32_t kNrOffBscl = kNrOffXq + 128 * 4;                  // S(q)'s row scale exponents
 constexpr uint32_t kNrOffAscl = kNrOffBscl + 128 * 4;                // chunk rows' scales
 constexpr uint32_t kNrFixed = kNrOffAscl + kNrChunk * 4;
 static_assert(kNrOffMeta % 8 == 0, "s_meta holds 64-bit words");
@@ -501,6 +500,15 @@ __device__ inline void centre_row16(float4 v0, float4 v1, float4 x0, float4 x1, 
     if (64 + 4 * sc < lim) *reinterpret_cast<uint2*>(row + 64 + 4 * sc) = make_uint2(w[2], w[3]);
 }
 
+// A row's fp16 scale exponent from its squared norm n: 2^(14 - e) with sqrt(n) < 2^e, so every
+// |value| of the row, at most sqrt(n), scales below 2^14, and the largest, at least sqrt(n / d),
+// stays far above fp16's subnormal range.
+__device__ inline float scale_exp(float n) {
+    int e = 0;
+    if (n > 0.f) frexpf(sqrtf(n), &e);
+    return (float)(14 - e);
+}
+
 // fp16 Gram of a 16x16 tile straight from FP32 rows in LDS: lane group g = lane >> 4 takes dims
 // 32 m + 8 g .. + 7 of k-step m for the row (arow) and the column (brow) alike, each row scaled by
 // its power of two (sa, sb: exact) into fp16 range; dims at or past 4 KS read as zero. Both operands
@@ -573,7 +581,6 @@ void k_non_reverse(NonRevArgs a) {
     uint32_t* s_rsbuf1 = reinterpret_cast<uint32_t*>(lds + kNrOffRsid2);
     uint32_t* s_nsq = reinterpret_cast<uint32_t*>(lds + kNrOffNsq);  // next S(q): snapshot slots
     float* s_nbn = reinterpret_cast<float*>(lds + kNrOffNbn);        // ... and their norms
-    float* s_nbs = reinterpret_cast<float*>(lds + kNrOffNbs);        // ... and their row scales
     float* s_bscl = reinterpret_cast<float*>(lds + kNrOffBscl);
     float* s_ascl = reinterpret_cast<float*>(lds + kNrOffAscl);
     constexpr bool kScaled = SCR == 3;  // fp16 Gram of the FP32 LDS rows, per-row scales
@@ -599,7 +606,7 @@ void k_non_reverse(NonRevArgs a) {
     float4 pf_row[kNrPfRows];
     uint32_t pf_snap[kNrPfSnap];
     uint32_t pf_id = kNoId, pf_na = 0;
-    float pf_bnd = 0.f, pf_nrm = 0.f, pf_scl = 0.f;
+    float pf_bnd = 0.f, pf_nrm = 0.f;
     // ids[sr] for chunk rows sr < na (16 threads per row): bound, norm, row, snapshot -> registers
     auto fetch_rows = [&](uint32_t tid, const uint32_t* ids, uint32_t na) {
         const uint32_t sr = tid >> 4, sc = tid & 15;
@@ -608,7 +615,6 @@ void k_non_reverse(NonRevArgs a) {
         if (sc == 0 && sr < na) {
             pf_bnd = a.bound[pf_id];
             pf_nrm = a.norms[pf_id];
-            if constexpr (kScaled) pf_scl = a.scales[pf_id];
         }
         const float* xr = a.X + (size_t)(sr < na ? pf_id : 0) * ld;
 #pragma unroll
@@ -672,8 +678,8 @@ void k_non_reverse(NonRevArgs a) {
         if (wave == 0) {
             const uint32_t v0 = lane < k ? s_nsq[lane] : 0u, v1 = lane + 64 < k ? s_nsq[lane + 64] : 0u;
             const float bn0 = lane < k ? s_nbn[lane] : 0.f, bn1 = lane + 64 < k ? s_nbn[lane + 64] : 0.f;
-            const float bs0 = kScaled && lane < k ? s_nbs[lane] : 0.f;
-            const float bs1 = kScaled && lane + 64 < k ? s_nbs[lane + 64] : 0.f;
+            const float bs0 = kScaled ? scale_exp(bn0) : 0.f;
+            const float bs1 = kScaled ? scale_exp(bn1) : 0.f;
             const bool n0 = lane < k && (v0 & kNewBit), n1 = lane + 64 < k && (v1 & kNewBit);
             const uint64_t m0 = __ballot(n0), m1 = __ballot(n1);
             const uint64_t below = (1ull << lane) - 1ull;
@@ -748,7 +754,6 @@ void k_non_reverse(NonRevArgs a) {
         const uint32_t v = a.snap[(size_t)q * k + t];
         s_nsq[t] = v;
         s_nbn[t] = a.norms[v & kIdMask];
-        if constexpr (kScaled) s_nbs[t] = a.scales[v & kIdMask];
     }
     for (uint32_t t = tid0; t < min(a1 - a0, (uint64_t)kNrIds); t += kNrThreads) s_rsbuf0[t] = a.rs[a0 + t];
     uint32_t nq_l = 0;  // the next q's position (lane 0 of each wave)
@@ -820,12 +825,7 @@ void k_non_reverse(NonRevArgs a) {
                 __builtin_amdgcn_global_load_lds(a.norms + (s_nsq[lane] & kIdMask), (kLdsU32*)s_nbn, 4, 0, 0);
             if (lane + 64 < k)
                 __builtin_amdgcn_global_load_lds(a.norms + (s_nsq[lane + 64] & kIdMask), (kLdsU32*)(s_nbn + 64), 4, 0, 0);
-            if constexpr (kScaled) {
-                if (lane < k)
-                    __builtin_amdgcn_global_load_lds(a.scales + (s_nsq[lane] & kIdMask), (kLdsU32*)s_nbs, 4, 0, 0);
-                if (lane + 64 < k)
-                    __builtin_amdgcn_global_load_lds(a.scales + (s_nsq[lane + 64] & kIdMask), (kLdsU32*)(s_nbs + 64), 4, 0, 0);
-            }
+
         }
     };
     // old rows pair with S(q)'s new entries only; with none of either there is nothing to offer
@@ -847,7 +847,7 @@ void k_non_reverse(NonRevArgs a) {
             s_aid[sr] = pf_id;
             s_abnd[sr] = sr < pf_na ? pf_bnd : 0.f;
             s_anrm[sr] = sr < pf_na ? pf_nrm : 0.f;
-            if constexpr (kScaled) s_ascl[sr] = sr < pf_na ? pf_scl : 0.f;
+            if constexpr (kScaled) s_ascl[sr] = sr < pf_na ? scale_exp(pf_nrm) : 0.f;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kNrPfRows; ++u) {
@@ -1089,7 +1089,7 @@ void k_non_reverse(NonRevArgs a) {
     for (uint32_t u = 0; u < kNrPfSnap; ++u) pf_snap[u] = 0u;
     pf_id = kNoId;
     pf_na = 0;
-    pf_bnd = pf_nrm = pf_scl = 0.f;
+    pf_bnd = pf_nrm = 0.f;
     if (!has_next) {
         append();
         break;
@@ -1201,40 +1201,28 @@ __global__ __launch_bounds__(kNonWaves * 64) void k_non_direct(NonArgs a, const 
 // FP32 squared norms for the screen (any summation order: the screen's margin covers it). Sixteen
 // lanes per row, float4 loads, so a wave reads four whole rows at a time.
 __global__ __launch_bounds__(256) void k_row_norms(const float* __restrict__ X, uint32_t ld,
-                                                   uint32_t d, uint32_t n, float* __restrict__ out,
-                                                   float* __restrict__ scales) {
+                                                   uint32_t d, uint32_t n, float* __restrict__ out) {
     const uint32_t i = blockIdx.x * 16 + (threadIdx.x >> 4), sc = threadIdx.x & 15;
     const uint32_t d4 = (d + 3) / 4;
-    float s = 0.f, m = 0.f;
+    float s = 0.f;
     if (i < n) {
         const float4* x = reinterpret_cast<const float4*>(X + (size_t)i * ld);
-        for (uint32_t q = sc; q < d4; q += 16) {
+        for (uint32_t q = sc; q < d4; q += 16) {  // the row's first ceil(d/4) float4 only
             const float4 v = x[q];
             const uint32_t j = 4 * q;
-            if (j + 0 < d) { s = __builtin_fmaf(v.x, v.x, s); m = fmaxf(m, fabsf(v.x)); }
-            if (j + 1 < d) { s = __builtin_fmaf(v.y, v.y, s); m = fmaxf(m, fabsf(v.y)); }
-            if (j + 2 < d) { s = __builtin_fmaf(v.z, v.z, s); m = fmaxf(m, fabsf(v.z)); }
-            if (j + 3 < d) { s = __builtin_fmaf(v.w, v.w, s); m = fmaxf(m, fabsf(v.w)); }
+            if (j + 0 < d) s = __builtin_fmaf(v.x, v.x, s);
+            if (j + 1 < d) s = __builtin_fmaf(v.y, v.y, s);
+            if (j + 2 < d) s = __builtin_fmaf(v.z, v.z, s);
+            if (j + 3 < d) s = __builtin_fmaf(v.w, v.w, s);
         }
     }
-    for (uint32_t o = 8; o > 0; o >>= 1) {
-        s += __shfl_xor(s, o, 16);
-        m = fmaxf(m, __shfl_xor(m, o, 16));
-    }
-    if (i < n && sc == 0) {
-        out[i] = s;
-        // fp16 screen: the row's power-of-two scale exponent, largest |value| below 2^14
-        if (scales) {
-            int e = 0;
-            if (m > 0.f) frexpf(m, &e);
-            scales[i] = (float)(14 - e);
-        }
-    }
+    for (uint32_t o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (i < n && sc == 0) out[i] = s;
 }
 
 void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float* out,
-                      float* scales, hipStream_t st) {
-    if (n) k_row_norms<<<(n + 15) / 16, 256, 0, st>>>(X, ld, d, n, out, scales);
+                      hipStream_t st) {
+    if (n) k_row_norms<<<(n + 15) / 16, 256, 0, st>>>(X, ld, d, n, out);
 }
 
 // ---- launchers ----------------------------------------------------------------------------------------
