@@ -983,6 +983,11 @@ void k_non_reverse(NonRevArgs a) {
                 rn[i] = s_anrm[rr[i]];
                 pid[i] = s_aid[rr[i]];
                 thr0[i] = SCR == 0 || exact16 ? s_abnd[rr[i]] * (1.0f + 1.0f / 65536.0f) : s_abnd[rr[i]];
+                // fp16 Gram of the scaled rows (exact form): the test n_p + n_r - 2 g <= thr0 +
+                // c (n_p + n_r) is evaluated as -2 g + (1 - c) n_r <= thr0 - (1 - c) n_p, the
+                // row's side once per strip; the reordering's FP32 rounding (~2^-23 of the terms)
+                // is far inside the margin's slack
+                if constexpr (SCR == 3) thr0[i] = thr0[i] - (1.0f - margin_c) * rn[i];
             }
             const float* arow = s_arow + (16 * strip + lrow) * ldp;
             // 16-bit screens: the strip's rows centred on q (lane group lk >> 2 takes halves
@@ -1021,13 +1026,16 @@ void k_non_reverse(NonRevArgs a) {
                 const uint32_t rid = s_bid[c];
                 // compact the survivors of this tile into the wave's queue as (row << 7 | col)
                 const uint32_t qn_before = qn;
+                const float cnm = (1.0f - margin_c) * cn;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float nsum = rn[i] + cn;
                     // FP32 Gram: 2 acc is exact. 16-bit: acc * cinv * (-2 rinv) is exact scaling.
                     const float approx = SCR == 0 ? __builtin_fmaf(-2.0f, acc[i], nsum)
-                                                  : __builtin_fmaf(acc[i] * cinv, m2r[i], nsum);
-                    const float thr = SCR == 0 || exact16 ? __builtin_fmaf(margin_c, nsum, thr0[i]) : thr0[i];
+                                         : SCR == 3 ? __builtin_fmaf(acc[i] * cinv, m2r[i], cnm)
+                                                    : __builtin_fmaf(acc[i] * cinv, m2r[i], nsum);
+                    const float thr = SCR == 3 ? thr0[i]
+                                      : SCR == 0 || exact16 ? __builtin_fmaf(margin_c, nsum, thr0[i]) : thr0[i];
                     // <=: a distance equal to a zero bound can still win on its id
                     const bool ok = cok && rok[i] && rid != pid[i] && !((mw[i] >> cb) & 1u) &&
                                     approx <= thr;
