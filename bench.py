@@ -89,15 +89,25 @@ def parse():
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_non_reverse_traffic_lowrank10M.md")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary, if it covers it."""
+def traffic_key(a):
+    """The workload a traffic figure belongs to: the build's sizes and its screen forms."""
+    flags = [f for f, on in (("non-fp32-gram", a.non_fp32_gram), ("non-screen16", a.non_screen16))
+             if on]
+    return (f"{a.workload} n={a.n} d={a.d} k={a.k} rounds={a.rounds} leaf={a.leaf} non={a.non} "
+            f"precision={a.precision} {a.splitter}" + "".join(" " + f for f in flags))
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed counter summary, if it covers this
+    kernel on this workload (the summary names both; another size or screen form moves the
+    figure, so it is then not reported)."""
     try:
         with open(TRAFFIC_FILE) as f:
             vals = dict(line.strip()[2:].split(": ", 1) for line in f
                         if line.startswith("- ") and ": " in line)
     except OSError:
         return None
-    if vals.get("kernel") != kernel:
+    if vals.get("kernel") != kernel or vals.get("workload") != workload:
         return None
     try:
         return float(vals["traffic_bytes_per_launch"])
@@ -476,7 +486,7 @@ def main():
                 "peak": MFMA_FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / MFMA_FP16_PEAK_TFLOPS, 5), "traffic": None,
                 "flops_per_launch": work / max(1, launches)}
-    roof["traffic"] = measured_traffic(name)
+    roof["traffic"] = measured_traffic(name, traffic_key(a))
     roof["launches"] = launches
     roof["avg_launch_ms"] = round(ms / max(1, launches), 4)
     roof["share_of_build"] = round(ms / st["total_ms"], 3) if st["total_ms"] > 0 else None
