@@ -505,60 +505,105 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
         }
         __syncthreads();
 
-        // ---- phase 2: exact re-rank of each owned row's survivors, one batched offer ----------
-        for (uint32_t j = 0; j < kBigRowsPerWave; ++j) {
-            const uint32_t rr = kBigRowsPerWave * wave + j;
-            if (rr >= nrows) break;
-            const uint64_t w64 = *reinterpret_cast<const uint64_t*>(s_mask + rr * 4);
-            const bool last = c0 + kBigCols >= b;  // the last column block: the batch goes out
-            uint32_t np0 = npend[0];
+        // ---- phase 2: exact re-rank of the wave's survivors, batched offers per row --------------
+        // The block's survivors of the wave's rows are compacted into one list ((row << 6) | column,
+        // in the wave's offer scratch), so each pass of 64 lanes re-ranks 64 of them whatever rows
+        // they belong to; each row's results below its bound join the row's pending batch.
+        {
+            const bool last = c0 + kBigCols >= b;  // the last column block: the batches go out
+            // a row's state in and out of fixed registers (constant-index selects keep the arrays
+            // in VGPRs) around an offer of its pending batch
+            auto offer_pending = [&](uint32_t j) {
+                WaveList cur = lst[0];
+                uint32_t np = npend[0];
 #pragma unroll
-            for (uint32_t jj = 1; jj < kBigRowsPerWave; ++jj)
-                if (j == jj) np0 = npend[jj];
-            if (!w64 && !(last && np0)) continue;
-            // this row's state into fixed registers (constant-index selects keep the arrays in VGPRs)
-            WaveList cur = lst[0];
-            uint32_t np = npend[0];
+                for (uint32_t jj = 1; jj < kBigRowsPerWave; ++jj)
+                    if (j == jj) {
+                        cur = lst[jj];
+                        np = npend[jj];
+                    }
+                const uint64_t pd = lane < np ? s_pend[j * 64 + lane] : WaveList::kEmpty;
+                const bool ch = cur.offer64(pd, k, lane, s_scr);
 #pragma unroll
-            for (uint32_t jj = 1; jj < kBigRowsPerWave; ++jj)
-                if (j == jj) {
-                    cur = lst[jj];
-                    np = npend[jj];
-                }
-            uint64_t* pq = s_pend + j * 64;
-            bool ch = false;
-            // the block adds at most popc(w64) offers: offer the batch first if they might not fit
-            // (before this block's distances, which then need no registers across the offer)
-            if (np + __popcll(w64) > 64) {
-                const uint64_t pd = lane < np ? pq[lane] : WaveList::kEmpty;
-                ch = cur.offer64(pd, k, lane, s_scr);
-                np = 0;
+                for (uint32_t jj = 0; jj < kBigRowsPerWave; ++jj)
+                    if (j == jj) {
+                        lst[jj] = cur;
+                        changed[jj] |= ch;
+                        npend[jj] = 0;
+                    }
+            };
+            uint64_t wm[kBigRowsPerWave];
+            uint32_t tot = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < kBigRowsPerWave; ++j) {
+                const uint32_t rr = kBigRowsPerWave * wave + j;
+                wm[j] = rr < nrows ? *reinterpret_cast<const uint64_t*>(s_mask + rr * 4) : 0ull;
+                tot += __popcll(wm[j]);
             }
-            uint64_t x = WaveList::kEmpty;
-            if ((w64 >> lane) & 1ull)
-                x = WaveList::key(dist_ref(s_af32 + (size_t)rr * L.fs, s_cf32 + (size_t)lane * L.fs, d),
-                                  s_cid[lane]);
-            n_surv += __popcll(w64);
-            const bool below = x < cur.kth;
-            const uint64_t m = __ballot(below);
-            n_ins += __popcll(m);
-            __builtin_amdgcn_wave_barrier();
-            if (below) pq[np + __popcll(m & ((1ull << lane) - 1ull))] = x;
-            __builtin_amdgcn_wave_barrier();
-            np += __popcll(m);
-            if (last && np) {
-                const uint64_t pd = lane < np ? pq[lane] : WaveList::kEmpty;
-                ch |= cur.offer64(pd, k, lane, s_scr);
-                np = 0;
-            }
+            // the block adds at most popc(wm[j]) offers to row j's batch: offer the batch first if
+            // they might not fit
+#pragma unroll 1
+            for (uint32_t j = 0; j < kBigRowsPerWave; ++j) {
+                uint64_t w = wm[0];
+                uint32_t np = npend[0];
 #pragma unroll
-            for (uint32_t jj = 0; jj < kBigRowsPerWave; ++jj)
-                if (j == jj) {
-                    lst[jj] = cur;
-                    changed[jj] |= ch;
-                    npend[jj] = np;
+                for (uint32_t jj = 1; jj < kBigRowsPerWave; ++jj)
+                    if (j == jj) {
+                        w = wm[jj];
+                        np = npend[jj];
+                    }
+                if (np + __popcll(w) > 64) offer_pending(j);
+            }
+            uint16_t* s_cand = reinterpret_cast<uint16_t*>(s_scr);  // <= 256 entries in 1 KiB
+            const uint64_t below_l = (1ull << lane) - 1ull;
+            __builtin_amdgcn_wave_barrier();
+            {
+                uint32_t base = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < kBigRowsPerWave; ++j) {
+                    if ((wm[j] >> lane) & 1ull) s_cand[base + __popcll(wm[j] & below_l)] = (uint16_t)((j << 6) | lane);
+                    base += __popcll(wm[j]);
                 }
-            if (lane == 0) s_abnd[rr] = cur.bound;
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t e0 = 0; e0 < tot; e0 += 64) {
+                const uint32_t e = e0 + lane;
+                uint64_t x = WaveList::kEmpty;
+                uint32_t j = kBigRowsPerWave;
+                if (e < tot) {
+                    const uint32_t ent = s_cand[e], c = ent & 63u;
+                    j = ent >> 6;
+                    const float* ra = s_af32 + (size_t)(kBigRowsPerWave * wave + j) * L.fs;
+                    const float* rc = s_cf32 + (size_t)c * L.fs;
+                    const float dd = d == 16 * KS ? dist_ref_fixed<16 * KS>(ra, rc)
+                                   : d == 100 ? dist_ref_fixed<100>(ra, rc) : dist_ref(ra, rc, d);
+                    x = WaveList::key(dd, s_cid[c]);
+                }
+#pragma unroll
+                for (uint32_t jj = 0; jj < kBigRowsPerWave; ++jj) {
+                    const bool below = j == jj && x < lst[jj].kth;
+                    const uint64_t m = __ballot(below);
+                    n_ins += __popcll(m);
+                    if (below) s_pend[jj * 64 + npend[jj] + __popcll(m & below_l)] = x;
+                    npend[jj] += __popcll(m);
+                }
+            }
+            n_surv += tot;
+            __builtin_amdgcn_wave_barrier();
+            if (last)
+#pragma unroll 1
+                for (uint32_t j = 0; j < kBigRowsPerWave; ++j) {
+                    uint32_t np = npend[0];
+#pragma unroll
+                    for (uint32_t jj = 1; jj < kBigRowsPerWave; ++jj)
+                        if (j == jj) np = npend[jj];
+                    if (np) offer_pending(j);
+                }
+            if (lane == 0) {
+#pragma unroll
+                for (uint32_t j = 0; j < kBigRowsPerWave; ++j)
+                    if (kBigRowsPerWave * wave + j < nrows) s_abnd[kBigRowsPerWave * wave + j] = lst[j].bound;
+            }
         }
     }
 
