@@ -511,8 +511,10 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
         // they belong to; each row's results below its bound join the row's pending batch.
         {
             const bool last = c0 + kBigCols >= b;  // the last column block: the batches go out
-            // a row's state in and out of fixed registers (constant-index selects keep the arrays
-            // in VGPRs) around an offer of its pending batch
+            // a row's list selected in and out around an offer of its pending batch. With this one
+            // offer site the compiler keeps the four lists in scratch memory between offers; a form
+            // that keeps them in registers spills 53 VGPRs and ran the 10M x 128 Zipf leaf stage in
+            // 520 ms against 437 ms
             auto offer_pending = [&](uint32_t j) {
                 WaveList cur = lst[0];
                 uint32_t np = npend[0];
