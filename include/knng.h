@@ -77,7 +77,8 @@ typedef struct knng_params {
     uint32_t rounds;       /* split rounds */
     uint32_t non_passes;   /* neighbours-of-neighbours passes after the rounds */
     uint32_t precision;    /* KNNG_PREC_* */
-    uint32_t rerank;       /* reserved: per-point re-rank cap of the approximate screens */
+    uint32_t rerank;       /* accepted and ignored: the approximate screens re-rank every survivor
+                              exactly (a superset of a per-point top-rerank cap) */
     uint32_t flags;        /* KNNG_FLAG_* */
     uint32_t search_k;     /* list width while building, <= 128 (0 = k); the output keeps each
                               list's first k. Small k search poorly through neighbours of

@@ -806,7 +806,6 @@ struct Builder {
         a.knn_d = dists;
         a.bound = bnd;
         a.screen = prm.precision == KNNG_PREC_FP32 ? kScreenExact : kScreenApprox;
-        a.rerank_cap = prm.rerank;
         a.half_kind = screen_half();
         a.counters = leaf_timing ? nullptr : cnt;
         a.tune = tune && !leaf_timing ? 1u : 0u;
@@ -893,7 +892,6 @@ struct Builder {
         a.knn_d = knn_d.p;
         a.bound = bound.p;
         a.screen = prm.precision == KNNG_PREC_FP32 ? kScreenExact : kScreenApprox;
-        a.rerank_cap = prm.rerank;
         a.half_kind = screen_half();
         a.counters = leaf_timing ? nullptr : counters.p;
         a.tune = 0;

@@ -93,7 +93,6 @@ struct LeafArgs {
     float* knn_d;
     float* bound;          // n: knn_d[row][k-1]
     uint32_t screen;       // kScreenExact: rigorous margin; kScreenApprox: plain approximate filter
-    uint32_t rerank_cap;   // kScreenApprox: at most this many candidates per point re-ranked
     uint32_t half_kind;    // 0 = fp16, 1 = bf16 screening Gram
     uint32_t max_leaf;     // largest leaf size in this launch
     uint64_t* counters;    // [0] survivors screened, [1] exact distances, [2] inserts (nullable)
