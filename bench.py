@@ -441,7 +441,7 @@ def main():
             del Xh, ds_all
             non_screen = ("fp32 Gram with exact error margin" if a.non_fp32_gram or d not in (61, 62, 63, 64, 97, 98, 99, 100, 125, 126, 127, 128)
                           else "16-bit Gram of rows centred on each q" + (" with exact error margin" if a.precision == "fp32" else ", no margin")
-                          if a.non_screen16 else "fp16 Gram of power-of-two-scaled rows with exact error margin")
+                          if a.non_screen16 else "fp16 Gram of the rows (scaled by powers of two unless every squared norm lies in [2^-12, 2^28]) with exact error margin")
             prec_block = {"precision": a.precision,
                           "screen": {"fp32": "leaf: fp16 Gram with exact error margin",
                                      "fp16": "leaf: fp16 Gram, no margin",

@@ -308,6 +308,7 @@ struct Builder {
     uint32_t pass_no = 0;    // neighbours-of-neighbours passes so far in this build
     DevBuf<uint64_t> roff, scan_tmp;
     DevBuf<float> nbuf_d, norms32;
+    DevBuf<uint32_t> norm_rng;  // smallest nonzero and largest squared norm (float bits)
     // the second candidate-buffer set: sub-passes alternate, so a sub-pass's merge (on sst) runs
     // beside the next sub-pass's screen
     DevBuf<uint32_t> ncnt2, nbuf_id2;
@@ -1171,6 +1172,21 @@ struct Builder {
         KNNG_TRY(norms32.ensure(n));
         launch_row_norms(X, ld, d, n, norms32.p, st);
         a.norms = norms32.p;
+        // The fp16 screen may convert rows without their power-of-two scales when every nonzero
+        // squared norm lies in [2^-12, 2^28]: no value then exceeds fp16's range, and the absolute
+        // rounding of values in fp16's subnormal range stays inside the margin's slack.
+        a.unscaled = 0;
+        if (screen16 == 3) {
+            KNNG_TRY(norm_rng.ensure(2));
+            launch_norm_range(norms32.p, n, norm_rng.p, st);
+            uint32_t rng[2] = {0u, 0u};
+            KNNG_HIP(hipMemcpyAsync(rng, norm_rng.p, sizeof rng, hipMemcpyDeviceToHost, st));
+            KNNG_HIP(hipStreamSynchronize(st));
+            float lo = 0.f, hi = 0.f;
+            std::memcpy(&lo, &rng[0], 4);
+            std::memcpy(&hi, &rng[1], 4);
+            a.unscaled = lo >= 0x1p-12f && hi <= 0x1p28f ? 1u : 0u;  // all-zero rows: lo = +inf
+        }
         // Sub-passes over interleaved parts of the tree order (every subs-th q, so a point's q are
         // spread over all of them), each merged before the next: the bounds tighten as the pass
         // goes, which keeps the candidate buffers small.

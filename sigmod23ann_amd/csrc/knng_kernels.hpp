@@ -161,11 +161,15 @@ struct NonRevArgs {
     uint32_t screen16;      // 0: FP32 Gram of the rows; 1: fp16, 2: bf16 Gram of rows centred on q;
                             // 3: fp16 Gram of the rows, each scaled by a power of two from its norm
     uint32_t exact;         // 16-bit screens: 1 = rigorous error margin, 0 = plain approximate test
+    uint32_t unscaled;      // screen 3: every nonzero squared norm lies in [2^-12, 2^28], so rows
+                            // convert to fp16 without their power-of-two scales
     const float* norms;     // n FP32 squared norms
     uint64_t* counters;     // [0] screened pairs, [1] appended (nullable)
 };
 void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float* out,
                       hipStream_t st);
+// out[0]: the smallest nonzero, out[1]: the largest of n non-negative floats, as float bits
+void launch_norm_range(const float* norms, uint32_t n, uint32_t* out, hipStream_t st);
 
 // Sorted snapshot; an entry absent from `prev` (nullable: all) carries kNewBit, an expanded one
 // kExpBit. rank_lo = 0: unfilled slots -> id 0, every entry expanded; rank_lo > 0: unfilled

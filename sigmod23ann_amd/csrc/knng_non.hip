@@ -511,12 +511,17 @@ __device__ inline float scale_exp(float n) {
 
 // fp16 Gram of a 16x16 tile straight from FP32 rows in LDS: lane group g = lane >> 4 takes dims
 // 32 m + 8 g .. + 7 of k-step m for the row (arow) and the column (brow) alike, each row scaled by
-// its power of two (sa, sb: exact) into fp16 range; dims at or past 4 KS read as zero. Both operands
+// its power of two (sa, sb: exact) into fp16 range, or taken as it is (UNSC: every row's values
+// already lie where the margin holds unscaled); dims at or past 4 KS read as zero. Both operands
 // are converted per tile: holding the strip's converted rows across tiles costs registers the
 // kernel does not have (measured: spills, 313 ms per launch against 217).
-template <uint32_t KS>
+template <uint32_t KS, uint32_t UNSC>
 __device__ inline float4_t gram_tile_f16s(const float* __restrict__ arow, const float* __restrict__ brow,
                                           float sa, float sb, uint32_t g) {
+    if constexpr (UNSC != 0) {
+        sa = 1.0f;  // a multiply by a constant 1 folds away
+        sb = 1.0f;
+    }
     constexpr uint32_t K32 = (4 * KS + 31) / 32;
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
     float4_t c = {0.f, 0.f, 0.f, 0.f};
@@ -547,9 +552,10 @@ __device__ inline float4_t mfma32(const uint16_t* a, const uint16_t* b, float4_t
                                                        __builtin_bit_cast(bf16x8_t, bv), c, 0, 0, 0);
 }
 
-template <uint32_t KS, uint32_t SCR>
+template <uint32_t KS, uint32_t SCR, uint32_t UNSC>
 __global__ __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void k_non_reverse(NonRevArgs a) {
+    static_assert(UNSC == 0 || SCR == 3, "only the fp16 Gram of the FP32 rows has an unscaled form");
     static_assert(SCR == 0 || KS > 0, "the 16-bit screens take a compile-time width");
     constexpr uint32_t K32 = SCR ? (4 * KS + 31) / 32 : 1;
     extern __shared__ __align__(16) unsigned char lds[];
@@ -583,7 +589,8 @@ void k_non_reverse(NonRevArgs a) {
     float* s_nbn = reinterpret_cast<float*>(lds + kNrOffNbn);        // ... and their norms
     float* s_bscl = reinterpret_cast<float*>(lds + kNrOffBscl);
     float* s_ascl = reinterpret_cast<float*>(lds + kNrOffAscl);
-    constexpr bool kScaled = SCR == 3;  // fp16 Gram of the FP32 LDS rows, per-row scales
+    constexpr bool kFold = SCR == 3;                // fp16 Gram of the FP32 LDS rows, folded test
+    constexpr bool kScaled = SCR == 3 && UNSC == 0;  // ... with per-row power-of-two scales
     uint64_t* s_meta = reinterpret_cast<uint64_t*>(lds + kNrOffMeta);  // next q: q, roff[q], roff[q+1]
     uint32_t* s_aid = reinterpret_cast<uint32_t*>(lds + kNrOffAid);
     float* s_abnd = reinterpret_cast<float*>(lds + kNrOffAbnd);
@@ -987,7 +994,7 @@ void k_non_reverse(NonRevArgs a) {
                 // c (n_p + n_r) is evaluated as -2 g + (1 - c) n_r <= thr0 - (1 - c) n_p, the
                 // row's side once per strip; the reordering's FP32 rounding (~2^-23 of the terms)
                 // is far inside the margin's slack
-                if constexpr (SCR == 3) thr0[i] = thr0[i] - (1.0f - margin_c) * rn[i];
+                if constexpr (kFold) thr0[i] = thr0[i] - (1.0f - margin_c) * rn[i];
             }
             const float* arow = s_arow + (16 * strip + lrow) * ldp;
             // 16-bit screens: the strip's rows centred on q (lane group lk >> 2 takes halves
@@ -997,9 +1004,9 @@ void k_non_reverse(NonRevArgs a) {
             float m2r[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                m2r[i] = SCR == 3 ? -2.0f * ldexpf(1.0f, -(int)s_ascl[rr[i]])
+                m2r[i] = kScaled ? -2.0f * ldexpf(1.0f, -(int)s_ascl[rr[i]]) : kFold ? -2.0f
                                   : SCR ? -2.0f * s_arinv[rr[i]] : 0.f;
-            const float sa = SCR == 3 ? ldexpf(1.0f, (int)s_ascl[16 * strip + lrow]) : 0.f;
+            const float sa = kScaled ? ldexpf(1.0f, (int)s_ascl[16 * strip + lrow]) : 1.f;
             for (uint32_t j = wv; j < nstrip * tt; j += kNrThreads / 64) {
                 const uint32_t t = nstrip == 2 ? j >> 1 : j;
                 if (qn + 256 > kNrQueue) flush();
@@ -1011,8 +1018,8 @@ void k_non_reverse(NonRevArgs a) {
                 if constexpr (SCR == 0) {
                     acc = gram_tile_f32<KS>(arow, s_brow + (16 * t + lrow) * ldp, lk, ksteps);
                 } else if constexpr (SCR == 3) {
-                    const float sb = ldexpf(1.0f, (int)s_bscl[16 * t + lrow]);
-                    acc = gram_tile_f16s<KS>(arow, s_brow + (16 * t + lrow) * ldp, sa, sb, lane >> 4);
+                    const float sb = kScaled ? ldexpf(1.0f, (int)s_bscl[16 * t + lrow]) : 1.f;
+                    acc = gram_tile_f16s<KS, UNSC>(arow, s_brow + (16 * t + lrow) * ldp, sa, sb, lane >> 4);
                 } else {
                     const uint16_t* b16 = s_b16 + (16 * t + lrow) * rs16 + 8 * (lane >> 4);
                     acc = float4_t{0.f, 0.f, 0.f, 0.f};
@@ -1022,7 +1029,7 @@ void k_non_reverse(NonRevArgs a) {
                 const uint32_t c = 16 * t + lrow, cb = c & 31;
                 const bool cok = c < k;
                 const float cn = s_bnrm[c];  // zero past k
-                const float cinv = SCR == 3 ? ldexpf(1.0f, -(int)s_bscl[c]) : SCR ? s_brinv[c] : 0.f;
+                const float cinv = kScaled ? ldexpf(1.0f, -(int)s_bscl[c]) : kFold ? 1.f : SCR ? s_brinv[c] : 0.f;
                 const uint32_t rid = s_bid[c];
                 // compact the survivors of this tile into the wave's queue as (row << 7 | col)
                 const uint32_t qn_before = qn;
@@ -1032,9 +1039,9 @@ void k_non_reverse(NonRevArgs a) {
                     const float nsum = rn[i] + cn;
                     // FP32 Gram: 2 acc is exact. 16-bit: acc * cinv * (-2 rinv) is exact scaling.
                     const float approx = SCR == 0 ? __builtin_fmaf(-2.0f, acc[i], nsum)
-                                         : SCR == 3 ? __builtin_fmaf(acc[i] * cinv, m2r[i], cnm)
+                                         : kFold ? __builtin_fmaf(kScaled ? acc[i] * cinv : acc[i], m2r[i], cnm)
                                                     : __builtin_fmaf(acc[i] * cinv, m2r[i], nsum);
-                    const float thr = SCR == 3 ? thr0[i]
+                    const float thr = kFold ? thr0[i]
                                       : SCR == 0 || exact16 ? __builtin_fmaf(margin_c, nsum, thr0[i]) : thr0[i];
                     // <=: a distance equal to a zero bound can still win on its id
                     const bool ok = cok && rok[i] && rid != pid[i] && !((mw[i] >> cb) & 1u) &&
@@ -1233,6 +1240,46 @@ void launch_row_norms(const float* X, uint32_t ld, uint32_t d, uint32_t n, float
     if (n) k_row_norms<<<(n + 15) / 16, 256, 0, st>>>(X, ld, d, n, out);
 }
 
+// The smallest nonzero and the largest squared norm, as the bits of non-negative floats (whose
+// unsigned order is their order): out[0] starts at +inf, out[1] at 0. A grid of a few hundred
+// blocks, so two same-address atomics per block.
+__global__ void k_norm_range_init(uint32_t* out) {
+    out[0] = 0x7F800000u;
+    out[1] = 0u;
+}
+__global__ __launch_bounds__(256) void k_norm_range(const float* __restrict__ nrm, uint32_t n,
+                                                    uint32_t* __restrict__ out) {
+    uint32_t lo = 0x7F800000u, hi = 0u;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const uint32_t b = __float_as_uint(nrm[i]);
+        if (b != 0u) lo = min(lo, b);
+        hi = max(hi, b);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
+    }
+    __shared__ uint32_t s_lo[4], s_hi[4];
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            lo = min(lo, s_lo[w]);
+            hi = max(hi, s_hi[w]);
+        }
+        atomicMin(&out[0], lo);
+        atomicMax(&out[1], hi);
+    }
+}
+
+void launch_norm_range(const float* norms, uint32_t n, uint32_t* out, hipStream_t st) {
+    k_norm_range_init<<<1, 1, 0, st>>>(out);
+    if (n) k_norm_range<<<std::min<uint32_t>(512, (n + 255) / 256), 256, 0, st>>>(norms, n, out);
+}
+
 // ---- launchers ----------------------------------------------------------------------------------------
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
                          const uint32_t* prev, uint32_t rank_lo, uint32_t* rcnt, uint32_t* rnew,
@@ -1293,20 +1340,24 @@ void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {
         const uint32_t grid = 8 * std::min((a.nq + 7) / 8, slots);
         kernel<<<grid, kNrThreads, lds, st>>>(a);
     };
-    switch ((a.d + 3) / 4 + 100 * scr) {  // the FP32 Gram's k-steps, and the screen
-        case 16: go(&k_non_reverse<16, 0>); break;
-        case 25: go(&k_non_reverse<25, 0>); break;
-        case 32: go(&k_non_reverse<32, 0>); break;
-        case 116: go(&k_non_reverse<16, 1>); break;
-        case 125: go(&k_non_reverse<25, 1>); break;
-        case 132: go(&k_non_reverse<32, 1>); break;
-        case 216: go(&k_non_reverse<16, 2>); break;
-        case 225: go(&k_non_reverse<25, 2>); break;
-        case 232: go(&k_non_reverse<32, 2>); break;
-        case 316: go(&k_non_reverse<16, 3>); break;
-        case 325: go(&k_non_reverse<25, 3>); break;
-        case 332: go(&k_non_reverse<32, 3>); break;
-        default: go(&k_non_reverse<0, 0>); break;
+    const uint32_t unsc = scr == 3 && a.unscaled ? 1000u : 0u;
+    switch ((a.d + 3) / 4 + 100 * scr + unsc) {  // the FP32 Gram's k-steps, the screen, unscaled rows
+        case 16: go(&k_non_reverse<16, 0, 0>); break;
+        case 25: go(&k_non_reverse<25, 0, 0>); break;
+        case 32: go(&k_non_reverse<32, 0, 0>); break;
+        case 116: go(&k_non_reverse<16, 1, 0>); break;
+        case 125: go(&k_non_reverse<25, 1, 0>); break;
+        case 132: go(&k_non_reverse<32, 1, 0>); break;
+        case 216: go(&k_non_reverse<16, 2, 0>); break;
+        case 225: go(&k_non_reverse<25, 2, 0>); break;
+        case 232: go(&k_non_reverse<32, 2, 0>); break;
+        case 316: go(&k_non_reverse<16, 3, 0>); break;
+        case 325: go(&k_non_reverse<25, 3, 0>); break;
+        case 332: go(&k_non_reverse<32, 3, 0>); break;
+        case 1316: go(&k_non_reverse<16, 3, 1>); break;
+        case 1325: go(&k_non_reverse<25, 3, 1>); break;
+        case 1332: go(&k_non_reverse<32, 3, 1>); break;
+        default: go(&k_non_reverse<0, 0, 0>); break;
     }
 }
 

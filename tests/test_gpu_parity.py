@@ -287,13 +287,20 @@ def test_non_matches_oracle(flags):
 
 
 @pytest.mark.parametrize("gen,d", [("lowrank", 64), ("mixture", 100), ("lowrank", 100), ("lowrank", 128),
-                                   ("mixture", 37)])
+                                   ("mixture", 37), ("lowrank_tiny", 100), ("mixture_huge", 64)])
 @pytest.mark.parametrize("flags", [0, K.FLAG_NON_FP32_GRAM, K.FLAG_NON_SCREEN16])
 def test_non_screens_match_oracle(gen, d, flags):
-    # the reverse kernel's screens, each with its exact margin: the fp16 Gram of the scaled rows
-    # (default for widths 64, 100, 128), the FP32 Gram (every width, the only one for others), the
-    # 16-bit Gram of rows centred on each q; the lists must be the oracle's
-    X = lowrank(8000, d, 70 + d) if gen == "lowrank" else mixture(8000, d, 40, 70 + d)
+    # the reverse kernel's screens, each with its exact margin: the fp16 Gram (default for widths
+    # 64, 100, 128) of the rows as they are when every squared norm lies in [2^-12, 2^28]
+    # (unit-norm and mixture points), else of the rows scaled by powers of two (the tiny and huge
+    # sets); the FP32 Gram (every width, the only one for others); the 16-bit Gram of rows centred
+    # on each q. The lists must be the oracle's.
+    if gen == "lowrank_tiny":
+        X = lowrank(8000, d, 70 + d) * np.float32(2.0 ** -8)   # squared norms 2^-16
+    elif gen == "mixture_huge":
+        X = mixture(8000, d, 40, 70 + d) * np.float32(2.0 ** 12)  # squared norms ~2^34
+    else:
+        X = lowrank(8000, d, 70 + d) if gen == "lowrank" else mixture(8000, d, 40, 70 + d)
     k = 50
     kw = dict(k=k, leaf_max=128, small_max=3000, iterations=1, seed=12)
     o_prm = oracle.make_params(**kw)
