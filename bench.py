@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--non-screen16", action="store_true",
                     help="neighbours-of-neighbours screen on a 16-bit Gram of rows centred on each "
                          "q instead of the FP32 Gram of the rows")
+    ap.add_argument("--no-bucket-limit", action="store_true",
+                    help="exact form: leaf screen against each point's list bound only, without the "
+                         "in-bucket limit (the k-th smallest upper bound of its bucket distances)")
     ap.add_argument("--autotune", action="store_true",
                     help="leaf workgroup shape picked by timing both shapes on the first launch of "
                          "each size bin (the warmup builds pay for it)")
@@ -127,7 +130,8 @@ def copy_calibration(mib):
 
 
 def leaf_flags(a, K):
-    return {"binned": 0, "batched": K.FLAG_LEAF_BATCHED, "graph": K.FLAG_LEAF_GRAPH}[a.leaf_mode]
+    return ({"binned": 0, "batched": K.FLAG_LEAF_BATCHED, "graph": K.FLAG_LEAF_GRAPH}[a.leaf_mode]
+            | (K.FLAG_NO_BUCKET_LIMIT if a.no_bucket_limit else 0))
 
 
 def zipf_round(a, K):
@@ -167,7 +171,7 @@ def zipf_round(a, K):
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"Zipf(1.1) mixture {n}x{d}, 20000 clusters, sigma U[0.3,2], "
                                f"leaf<={leaf}, k={k}", "n": n, "d": d, "k": k, "leaf_max": leaf,
-                   "leaf_mode": a.leaf_mode},
+                   "leaf_mode": a.leaf_mode, "bucket_limit": not a.no_bucket_limit},
         "stages_ms": {"split": round(st["split_ms"], 2), "side": round(st["side_ms"], 2),
                       "leaf": round(st["leaf_ms"], 2), "total": round(st["total_ms"], 2)},
         "buckets": int(st["leaves"]), "leaf_pairs": int(st["leaf_pairs"]),
@@ -447,7 +451,11 @@ def main():
                                      "fp16": "leaf: fp16 Gram, no margin",
                                      "bf16": "leaf: bf16 Gram, no margin"}[a.precision]
                                     + "; neighbours of neighbours: " + non_screen,
-                          "rerank": "exact reference-order FP32 for every screen survivor",
+                          "rerank": ("exact reference-order FP32 for every screen survivor (the leaf screen's "
+                                     "bound lowered to each point's k-th smallest in-bucket upper bound)"
+                                     if a.precision == "fp32" else
+                                     "exact reference-order FP32 for the screen survivors among about the "
+                                     "160 nearest by the screen per point and leaf bucket"),
                           "dist_abs_err_vs_fp64": {"max": float(err.max()), "mean": float(err.mean())},
                           "dist_rel_err_vs_fp64": {"max": float(relerr.max()),
                                                    "mean": float(relerr.mean())},
@@ -537,7 +545,7 @@ def main():
         "config": {"workload": f"low-rank {n}x{d} (24 latent, unit norm), k={k}, leaf<={a.leaf}",
                    "n": n, "d": d, "k": k, "rounds": a.rounds, "iterations": a.iterations,
                    "non_passes": a.non, "precision": a.precision, "splitter": a.splitter, "autotune": a.autotune,
-                   "leaf_mode": a.leaf_mode,
+                   "leaf_mode": a.leaf_mode, "bucket_limit": not a.no_bucket_limit,
                    "recall_at_k": None if rec is None else round(rec, 4),
                    "recall_queries": nq, "parallelism": "single GPU"},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),

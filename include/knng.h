@@ -60,6 +60,10 @@ extern "C" {
                                         size) instead of one launch per size bin; same lists */
 #define KNNG_FLAG_LEAF_GRAPH 256u    /* as KNNG_FLAG_LEAF_BATCHED, its launches captured once in a
                                         hipGraph and replayed every round */
+#define KNNG_FLAG_NO_BUCKET_LIMIT 1024u /* exact form: leaf buckets screen each row against its list's
+                                        bound only, without the in-bucket limit (the k-th smallest
+                                        upper bound of the row's distances in the bucket); the same
+                                        lists, for comparisons */
 #define KNNG_FLAG_AUTOTUNE 32u       /* leaf stage: the first launch of each size bin above 128
                                         points (per device, precision and width, once per process)
                                         also times both workgroup shapes, 8 and 16 waves, and the
@@ -77,8 +81,12 @@ typedef struct knng_params {
     uint32_t rounds;       /* split rounds */
     uint32_t non_passes;   /* neighbours-of-neighbours passes after the rounds */
     uint32_t precision;    /* KNNG_PREC_* */
-    uint32_t rerank;       /* accepted and ignored: the approximate screens re-rank every survivor
-                              exactly (a superset of a per-point top-rerank cap) */
+    uint32_t rerank;       /* approximate forms (KNNG_PREC_FP16/BF16): the leaf stage re-ranks, per
+                              point and bucket, only the survivors whose approximate distance is at
+                              most the m-th smallest of the point's approximate distances in the
+                              bucket, m = max(rerank, k) (taken over each lane's smallest keys: a
+                              little above the m-th smallest); 0 or above 160: every survivor. The
+                              exact form ignores it. Default 160 */
     uint32_t flags;        /* KNNG_FLAG_* */
     uint32_t search_k;     /* list width while building, <= 128 (0 = k); the output keeps each
                               list's first k. Small k search poorly through neighbours of

@@ -44,6 +44,7 @@ FLAG_LEAF_BATCHED = 128  # leaf stage: one persistent kernel over the round's wo
 FLAG_LEAF_GRAPH = 256    # ... replayed from a captured hipGraph
 FLAG_KD_SPLIT = 16  # random-rotation KD splitter (SolutionRandomKD) instead of 2-means
 FLAG_AUTOTUNE = 32  # leaf workgroup shape picked by timing both on the first launch of a size bin
+FLAG_NO_BUCKET_LIMIT = 1024  # exact form: leaf screen without the in-bucket k-th upper-bound limit
 
 GEN_LOWRANK = 0
 GEN_MIXTURE = 1

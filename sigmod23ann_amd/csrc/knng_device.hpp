@@ -362,8 +362,6 @@ struct WaveList {
 };
 
 // ---- wave helpers and the 16-bit screening rows ----------------------------------------------
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // Row stride of the 16-bit rows in LDS (elements); +8 staggers banks between rows.
@@ -396,6 +394,87 @@ __device__ inline uint32_t wave_excl_scan(uint32_t v, uint32_t lane) {
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
     return x - v;
+}
+
+// ---- in-bucket limits ------------------------------------------------------------------------
+// A limit is the m-th smallest of a row's in-bucket distance bounds. Each lane keeps the J smallest
+// 16-bit keys it sees for a row (the top 16 bits of a non-negative float, which order as the
+// floats do; 0x7F80 is +inf), and the limit is the m-th smallest key over a group of lanes, read
+// back as the largest float with that key. The keys the lanes keep are a subset of all keys, so
+// their m-th smallest is at least the m-th smallest of all: a valid upper bound.
+constexpr uint32_t kKeyInf = 0x7F80u;
+__device__ inline uint32_t key16(float v) { return __float_as_uint(fmaxf(v, 0.f)) >> 16; }
+// key16(v) when ok (0 or 1), else kKeyInf, by arithmetic: no branch between the Gram's MFMA and
+// the reads of its result
+__device__ inline uint32_t key16_valid(float v, uint32_t ok) {
+    return min(key16(v) | (0u - (ok ^ 1u)), kKeyInf);
+}
+__device__ inline float key16_upper(uint32_t x) {
+    return x >= kKeyInf ? __builtin_inff() : __uint_as_float((x << 16) | 0xFFFFu);
+}
+template <uint32_t J>
+__device__ inline void top_insert(uint32_t (&t)[J], uint32_t h) {
+#pragma unroll
+    for (uint32_t j = 0; j < J; ++j) {
+        const uint32_t lo = min(t[j], h);
+        h = max(t[j], h);
+        t[j] = lo;
+    }
+}
+// Keys kept per lane and row: the LDS leaf kernel (a row's keys over 16 lanes) and the big-bucket
+// kernel (over 64 owners).
+constexpr uint32_t kLimKeys = 16, kBigLimKeys = 4;
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+// Two rows' keys at once, one per 16-bit half (packed 16-bit min and max).
+template <uint32_t J>
+__device__ inline void top_insert2(u16x2 (&t)[J], u16x2 h) {
+#pragma unroll
+    for (uint32_t j = 0; j < J; ++j) {
+        const u16x2 lo = __builtin_elementwise_min(t[j], h);
+        h = __builtin_elementwise_max(t[j], h);
+        t[j] = lo;
+    }
+}
+// Sum over the 16 lanes of a DPP row (row_ror:8, 4, 2, 1): every lane of the row ends with it.
+__device__ inline uint32_t row16_sum_u32(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x122, 0xF, 0xF, false);
+    return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x121, 0xF, 0xF, false);
+}
+// The m-th smallest key of a DPP row's 16 x J keys, for the two rows of packed key pairs (x0:
+// the low halves, x1: the high halves): the smallest x with at least m keys at or below it, found
+// one bit of x at a time from the top (x only grows while fewer than m keys lie below x + 2^bit;
+// every key is at most kKeyInf, so x ends at most there). Both counts travel in one 32-bit sum
+// (each at most 16 J < 2^16).
+template <uint32_t J>
+__device__ inline void row16_select2(const u16x2 (&t)[J], uint32_t m, uint32_t& x0, uint32_t& x1) {
+    x0 = 0;
+    x1 = 0;
+#pragma unroll
+    for (int bit = 14; bit >= 0; --bit) {
+        const uint32_t p0 = x0 + (1u << bit) - 1u, p1 = x1 + (1u << bit) - 1u;
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < J; ++j) c += ((uint32_t)t[j].x <= p0 ? 1u : 0u) + ((uint32_t)t[j].y <= p1 ? 0x10000u : 0u);
+        c = row16_sum_u32(c);
+        if ((c & 0xFFFFu) < m) x0 += 1u << bit;
+        if ((c >> 16) < m) x1 += 1u << bit;
+    }
+}
+// The same over the 64 lanes of a wave (J keys each); x is wave-uniform.
+template <uint32_t J>
+__device__ inline uint32_t wave_select(const uint32_t (&t)[J], uint32_t m) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int bit = 14; bit >= 0; --bit) {
+        const uint32_t probe = x + (1u << bit) - 1u;
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < J; ++j) c += (uint32_t)__popcll(__ballot(t[j] <= probe));
+        if (c < m) x += 1u << bit;
+    }
+    return x;
 }
 
 // Margin coefficient c of the exact screen: |approx - exact| <= c * (n_i + n_j).
@@ -441,29 +520,25 @@ __device__ inline void screen_vals(float v0, float v1, uint32_t d, uint32_t rs,
     rinv = ldexpf(1.0f, -sh);
 }
 
-__device__ inline float4_t mfma16(half4_t a, half4_t b, float4_t c, uint32_t half_kind) {
-    if (half_kind == 0) return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4_t, a),
-                                                     __builtin_bit_cast(short4_t, b), c, 0, 0, 0);
-}
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
 // A lane's share of a 16-bit screening row (rows padded to 16 KS dims) for the 16x16 tile Gram on
 // gfx950: k-steps of 32 dims on v_mfma_f32_16x16x32_{f16,bf16} (lane group g = lane >> 4 holds
-// dims 32 m + 8 g .. + 7), and, when KS is odd, one last 16-dim step on 16x16x16 (dims
-// 32 (KS / 2) + 4 g .. + 3). Rows and columns take the same dims per lane, so each product pairs
-// the same dim of both.
+// dims 32 m + 8 g .. + 7); when KS is odd, the last step takes its 16 dims in lane groups 0 and 1
+// and zeros in groups 2 and 3 (a 16x16x16 MFMA for those dims gave lists that were wrong, and
+// different from run to run, in the persistent leaf kernel once the in-bucket limit read the
+// Gram). Rows and columns take the same dims per lane, so each product pairs the same dim of both.
 template <uint32_t HALF, uint32_t KS>
 struct HalfRow {
-    static constexpr uint32_t K2 = KS / 2, K1 = KS % 2;
-    uint4 w[K2 ? K2 : 1];
-    half4_t t;
+    static constexpr uint32_t K2 = KS / 2, K1 = KS % 2, KM = K2 + K1;
+    uint4 w[KM];
     __device__ inline void load(const uint16_t* __restrict__ row, uint32_t g) {
 #pragma unroll
         for (uint32_t m = 0; m < K2; ++m) w[m] = *reinterpret_cast<const uint4*>(row + 32 * m + 8 * g);
-        if constexpr (K1 != 0) t = *reinterpret_cast<const half4_t*>(row + 32 * K2 + 4 * g);
+        if constexpr (K1 != 0)
+            w[K2] = g < 2 ? *reinterpret_cast<const uint4*>(row + 32 * K2 + 8 * g) : make_uint4(0u, 0u, 0u, 0u);
     }
 };
 
@@ -471,7 +546,7 @@ template <uint32_t HALF, uint32_t KS>
 __device__ inline float4_t gram16(const HalfRow<HALF, KS>& a, const HalfRow<HALF, KS>& b) {
     float4_t c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (uint32_t m = 0; m < HalfRow<HALF, KS>::K2; ++m) {
+    for (uint32_t m = 0; m < HalfRow<HALF, KS>::KM; ++m) {
         if constexpr (HALF == 0)
             c = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8_t, a.w[m]),
                                                        __builtin_bit_cast(half8_t, b.w[m]), c, 0, 0, 0);
@@ -479,7 +554,6 @@ __device__ inline float4_t gram16(const HalfRow<HALF, KS>& a, const HalfRow<HALF
             c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a.w[m]),
                                                         __builtin_bit_cast(bf16x8_t, b.w[m]), c, 0, 0, 0);
     }
-    if constexpr (HalfRow<HALF, KS>::K1 != 0) c = mfma16(a.t, b.t, c, HALF);
     return c;
 }
 

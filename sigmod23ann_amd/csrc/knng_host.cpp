@@ -427,6 +427,15 @@ struct Builder {
     // list width while building; the output keeps each list's first prm.k
     uint32_t kl() const { return std::max(prm.k, prm.search_k); }
 
+    // LeafArgs::rank_cap: in the exact screen the list width (the k-th smallest upper bound of a
+    // bucket loses none of the k nearest), unless KNNG_FLAG_NO_BUCKET_LIMIT; in the approximate
+    // screens the re-rank cap max(rerank, width), up to 160 (0, or a larger cap: no cap).
+    uint32_t leaf_rank_cap() const {
+        if (prm.precision == KNNG_PREC_FP32) return (prm.flags & KNNG_FLAG_NO_BUCKET_LIMIT) ? 0u : kl();
+        const uint32_t m = prm.rerank ? std::max(prm.rerank, kl()) : 0u;
+        return m <= 160 ? m : 0u;
+    }
+
     uint32_t leaf_max_for(uint32_t round) const {
         return (round % 2 == 1 && prm.leaf_max_odd) ? prm.leaf_max_odd : prm.leaf_max;
     }
@@ -810,6 +819,7 @@ struct Builder {
         a.half_kind = screen_half();
         a.counters = leaf_timing ? nullptr : cnt;
         a.tune = tune && !leaf_timing ? 1u : 0u;
+        a.rank_cap = leaf_rank_cap();
         KNNG_HIP(hipEventRecord(e0, st));
         size_t off = 0;
         for (auto& v : pl.bin) {
@@ -896,6 +906,7 @@ struct Builder {
         a.half_kind = screen_half();
         a.counters = leaf_timing ? nullptr : counters.p;
         a.tune = 0;
+        a.rank_cap = leaf_rank_cap();
         a.leaves = leaves[slot].p;
         a.nleaves = (uint32_t)small.size();
         a.max_leaf = lim;
@@ -1634,6 +1645,11 @@ int knng_leaf_stage(knng_handle* h, const float* points, uint32_t n, uint32_t d,
     KNNG_TRY(upload_state(b, ids, dists, start_empty != 0));
     KNNG_HIP(hipMemsetAsync(b.counters.p, 0, 4 * sizeof(uint64_t), b.st));
     KNNG_TRY(b.leaf_stage(lv, b.perm.p, 0, true, b.ev[4], b.ev[5]));
+    uint64_t c[4] = {0, 0, 0, 0};
+    KNNG_HIP(hipMemcpyAsync(c, b.counters.p, sizeof c, hipMemcpyDeviceToHost, b.st));
+    KNNG_HIP(hipStreamSynchronize(b.st));
+    b.stats.survivors = c[0];
+    b.stats.inserts = c[2];
     return download_state(b, ids, dists);
 }
 

@@ -97,6 +97,11 @@ struct LeafArgs {
     uint32_t max_leaf;     // largest leaf size in this launch
     uint64_t* counters;    // [0] survivors screened, [1] exact distances, [2] inserts (nullable)
     uint32_t tune;         // 1: pick the workgroup shape by timing both (KNNG_FLAG_AUTOTUNE)
+    // In-bucket limit m (0: none, else at most 160). A row's screen bound is lowered to the m-th
+    // smallest of its in-bucket distance bounds: upper bounds of the exact distances in the exact
+    // screen (m = k: no pair of the k nearest is lost, so the lists are the same), the approximate
+    // distances otherwise (the re-rank cap: about the m nearest by the screen are re-ranked).
+    uint32_t rank_cap;
 };
 void launch_leaf(const LeafArgs& a, hipStream_t st);
 

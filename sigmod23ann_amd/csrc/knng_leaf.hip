@@ -4,10 +4,12 @@
 // One workgroup (8 or 16 waves) per bucket, rows resident in LDS:
 //  phase 0  gather the bucket's rows, centre them on the bucket's first row, scale each row by a
 //           power of two into fp16 (or bf16) range; FP32 squared norms of the centred rows.
-//  phase 1  screening Gram on MFMA (16x16x32, and 16x16x16 for a last 16 dims; fp32 accumulate),
-//           one 16-row strip per wave over all columns: approximate distance n_i + n_j - 2 g_ij against the point's current bound.
-//           Exact mode adds a rigorous error margin, so every pair whose exact FP32 distance beats
-//           the bound survives; survivors go to a per-row bitmask in LDS.
+//  phase 1  screening Gram on MFMA (16x16x32; fp32 accumulate), one 16-row strip per wave over
+//           all columns: approximate distance n_i + n_j - 2 g_ij against the point's current
+//           bound. Exact mode adds a rigorous error margin, so every pair whose exact FP32
+//           distance beats the bound survives; survivors go to a per-row bitmask in LDS. Where
+//           a row keeps many survivors (an empty list's first bucket), the strip is screened
+//           again against the k-th smallest upper bound of the row's in-bucket distances.
 //  phase 2  one wave per row: exact reference-order FP32 distance for each survivor, then a
 //           wave-wide insert into the row's sorted (dist, id) list with id dedupe.
 //
@@ -134,42 +136,102 @@ __device__ inline void leaf_bucket(const LeafArgs& a, const LeafDesc lf) {
     // ---- phase 1: screening Gram --------------------------------------------------------------
     const bool exact = a.screen == kScreenExact;
     const float margin_c = HalfTraits<HALF>::kMargin;
-    const uint32_t lrow = lane & 15;
+    const uint32_t lrow = lane & 15, g4 = lane >> 4;
+    // In-bucket limit (LeafArgs::rank_cap = m): a strip where some row keeps more than `trig`
+    // survivors is screened again with each row's bound lowered to the m-th smallest of its
+    // in-bucket distance bounds (the exact screen's upper bounds, or the approximate distances).
+    // In the exact screen that only pays when many survivors are cut (an empty list's first
+    // bucket), so it waits for m + 64; the re-rank cap of the approximate screens applies
+    // whenever it binds.
+    const uint32_t cap = a.rank_cap;
+    const uint32_t trig = exact ? cap + 64 : cap;
     for (uint32_t s = wave; s < nstrip; s += kLeafWaves) {
         HalfRow<HALF, KS> af;
-        af.load(s_rows + (16 * s + lrow) * L.rs, lane >> 4);
-        float rn[4], rinv[4], thr_base[4];
+        af.load(s_rows + (16 * s + lrow) * L.rs, g4);
+        float rn[4], rinv[4], lim[4];
         uint32_t rr[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            rr[i] = 16 * s + (lane >> 4) * 4 + i;
+            rr[i] = 16 * s + g4 * 4 + i;
             rn[i] = s_nrm[rr[i]];
             rinv[i] = s_rinv[rr[i]];
-            const float bnd = s_bnd[rr[i]];
-            thr_base[i] = exact ? bnd * (1.0f + 1.0f / 65536.0f) : bnd;
+            lim[i] = s_bnd[rr[i]];
         }
-        for (uint32_t t = 0; t < nstrip; ++t) {
-            HalfRow<HALF, KS> bf;
-            bf.load(s_rows + (16 * t + lrow) * L.rs, lane >> 4);
-            const float4_t acc = gram16(af, bf);
-            const uint32_t c = 16 * t + lrow;
-            const float cn = s_nrm[c], cinv = s_rinv[c];
-            uint64_t bits[4];
+        // one pass over the column tiles: the strip's survivor masks for the bounds lim; returns
+        // the most survivors of a row of this lane group
+        auto sweep = [&]() -> uint32_t {
+            float thr_base[4];
+            uint32_t cnt[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const float g = acc[i] * (rinv[i] * cinv);
-                const float nsum = rn[i] + cn;
-                const float approx = nsum - 2.0f * g;
-                const float thr = exact ? thr_base[i] + margin_c * nsum : thr_base[i];
-                // <=: a zero distance (both rows at the centre) may meet a zero bound and win on its id
-                const bool ok = c < b && rr[i] < b && c != rr[i] && approx <= thr;
-                bits[i] = __ballot(ok);
+            for (int i = 0; i < 4; ++i) thr_base[i] = exact ? lim[i] * (1.0f + 1.0f / 65536.0f) : lim[i];
+            for (uint32_t t = 0; t < nstrip; ++t) {
+                HalfRow<HALF, KS> bf;
+                bf.load(s_rows + (16 * t + lrow) * L.rs, g4);
+                const float4_t acc = gram16(af, bf);
+                const uint32_t c = 16 * t + lrow;
+                const float cn = s_nrm[c], cinv = s_rinv[c];
+                uint64_t bits[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float g = acc[i] * (rinv[i] * cinv);
+                    const float nsum = rn[i] + cn;
+                    const float approx = nsum - 2.0f * g;
+                    const float thr = exact ? thr_base[i] + margin_c * nsum : thr_base[i];
+                    // <=: a zero distance (both rows at the centre) may meet a zero bound and win on its id
+                    const bool ok = c < b && rr[i] < b && c != rr[i] && approx <= thr;
+                    bits[i] = __ballot(ok);
+                    cnt[i] += (uint32_t)__popcll((bits[i] >> (16 * g4)) & 0xFFFFull);
+                }
+                if (lane < 16) {
+                    const uint32_t g = lane >> 2, i = lane & 3;
+                    const uint64_t bw = i == 0 ? bits[0] : i == 1 ? bits[1] : i == 2 ? bits[2] : bits[3];
+                    s_mask[(16 * s + lane) * L.w16 + t] = (uint16_t)(bw >> (16 * g));
+                }
             }
-            if (lane < 16) {
-                const uint32_t g = lane >> 2, i = lane & 3;
-                const uint64_t bw = i == 0 ? bits[0] : i == 1 ? bits[1] : i == 2 ? bits[2] : bits[3];
-                s_mask[(16 * s + lane) * L.w16 + t] = (uint16_t)(bw >> (16 * g));
+            return max(max(cnt[0], cnt[1]), max(cnt[2], cnt[3]));
+        };
+        // each lane's 16 smallest keys of its rows' bounds (rows 2p and 2p + 1 in the halves of
+        // pair p's registers), then the m-th smallest of the 256 keys of a row's lane group: with
+        // m <= 160 a lane meets 10 of a row's m nearest on average, so few of them are missed
+        auto limits = [&]() {
+            constexpr uint32_t J = kLimKeys;
+            u16x2 top[2][J];
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (uint32_t j = 0; j < J; ++j) top[p][j] = u16x2{(uint16_t)kKeyInf, (uint16_t)kKeyInf};
+            for (uint32_t t = 0; t < nstrip; ++t) {
+                HalfRow<HALF, KS> bf;
+                bf.load(s_rows + (16 * t + lrow) * L.rs, g4);
+                const float4_t acc = gram16(af, bf);
+                const uint32_t c = 16 * t + lrow;
+                const float cn = s_nrm[c], cinv = s_rinv[c];
+                uint32_t key[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float g = acc[i] * (rinv[i] * cinv);
+                    const float nsum = rn[i] + cn;
+                    const float approx = nsum - 2.0f * g;
+                    // the exact screen's upper bound: exact <= approx + c (n_i + n_j)
+                    const float u = exact ? approx + margin_c * nsum : approx;
+                    const uint32_t ok = (uint32_t)(c < b) & (uint32_t)(rr[i] < b) & (uint32_t)(c != rr[i]);
+                    key[i] = key16_valid(u, ok);
+                }
+                top_insert2(top[0], u16x2{(uint16_t)key[0], (uint16_t)key[1]});
+                top_insert2(top[1], u16x2{(uint16_t)key[2], (uint16_t)key[3]});
             }
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                uint32_t x0, x1;
+                row16_select2(top[p], cap, x0, x1);
+                lim[2 * p] = fminf(lim[2 * p], key16_upper(x0));
+                lim[2 * p + 1] = fminf(lim[2 * p + 1], key16_upper(x1));
+            }
+        };
+        const uint32_t most = sweep();
+        if (cap != 0 && __ballot(most > trig) != 0) {
+            limits();
+            sweep();
         }
     }
     __syncthreads();
@@ -298,7 +360,7 @@ constexpr uint32_t kBigRowsPerWave = kBigRows / kBigWaves;  // 4
 struct BigLds {
     uint32_t rs, fs;  // 16-bit row stride (elements), FP32 row stride (floats)
     uint32_t off_arow, off_anrm, off_arinv, off_abnd, off_crow, off_cnrm, off_crinv, off_cid;
-    uint32_t off_af32, off_cf32, off_mask, off_scr, off_pend, total;
+    uint32_t off_af32, off_cf32, off_mask, off_scr, off_pend, off_tlim, total;
 };
 
 __host__ __device__ inline BigLds big_lds_layout(uint32_t d, uint32_t ld) {
@@ -323,6 +385,7 @@ __host__ __device__ inline BigLds big_lds_layout(uint32_t d, uint32_t ld) {
     L.off_mask = o;  o += kBigRows * 8;
     L.off_scr = o;   o += kBigWaves * 128 * 8;
     L.off_pend = o;  o += kBigWaves * kBigRowsPerWave * 64 * 8;  // pending offers, 64 per owned row
+    L.off_tlim = o;  o += kBigRows * 4;                          // the rows' in-bucket limits
     L.total = (o + 15) & ~15u;
     return L;
 }
@@ -456,6 +519,84 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
             if (lane == 0) s_cid[cc] = cid[u];
         }
     };
+
+    // ---- in-bucket limits (LeafArgs::rank_cap = m) ---------------------------------------------
+    // In the approximate screens (the re-rank cap) the block's rows get their bound lowered to the
+    // m-th smallest of their approximate in-bucket distances before the screen. (The exact
+    // screen's limit does not pay here: on the 10M x 128 Zipf round, whose rows all start with
+    // empty lists, it cut the re-ranks but took the round from 477 to 564 ms, because each block
+    // re-stages the whole bucket for it.) One extra pass over the column blocks screens the block's tiles as phase 1
+    // does and keeps, per (tile, lane) owner of a row, its 4 smallest keys (64 owners: with m <= 160
+    // an owner meets 2.5 of a row's m nearest on average);
+    // the keys meet in LDS (the pending-offer area, unused until the main pass) and the wave that
+    // owns a row selects its m-th smallest by ballot counts.
+    float* s_tlim = reinterpret_cast<float*>(lds + L.off_tlim);
+    if (lane < kBigRowsPerWave) s_tlim[kBigRowsPerWave * wave + lane] = __builtin_inff();
+    __syncthreads();
+    const uint32_t cap = a.rank_cap;
+    const bool need_lim = cap != 0 && !exact;
+    auto limits = [&]() {
+        constexpr uint32_t J = kBigLimKeys;
+        uint32_t top[4][J];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (uint32_t j = 0; j < J; ++j) top[i][j] = kKeyInf;
+        HalfRow<HALF, KS> af;
+        af.load(s_arow + (16 * strip + lrow) * L.rs, lane >> 4);
+        uint32_t rr[4];
+        float rn[4], rinv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            rr[i] = 16 * strip + (lane >> 4) * 4 + i;
+            rn[i] = s_anrm[rr[i]];
+            rinv[i] = s_arinv[rr[i]];
+        }
+        fetch(0);
+        for (uint32_t c0 = 0; c0 < b; c0 += kBigCols) {
+            const uint32_t nc = min(kBigCols, b - c0);
+            __syncthreads();
+            commit();
+            __syncthreads();
+            if (c0 + kBigCols < b) fetch(c0 + kBigCols);
+            HalfRow<HALF, KS> bf;
+            bf.load(s_crow + (16 * tile + lrow) * L.rs, lane >> 4);
+            const float4_t acc = gram16(af, bf);
+            const uint32_t c = 16 * tile + lrow;
+            const float cn = s_cnrm[c], cinv = s_crinv[c];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float g = acc[i] * (rinv[i] * cinv);
+                const float nsum = rn[i] + cn;
+                const float approx = nsum - 2.0f * g;
+                const float u = exact ? approx + margin_c * nsum : approx;
+                const uint32_t ok = (uint32_t)(c < nc) & (uint32_t)(rr[i] < nrows) & (uint32_t)(r0 + rr[i] != c0 + c);
+                top_insert(top[i], key16_valid(u, ok));
+            }
+        }
+        uint16_t* s_keys = reinterpret_cast<uint16_t*>(lds + L.off_pend);  // 64 rows x 64 owners x J
+        static_assert(kBigRows * 64 * J * 2 <= kBigWaves * kBigRowsPerWave * 64 * 8, "keys fit the pending area");
+        const uint32_t owner = 16 * tile + lrow;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (uint32_t j = 0; j < J; ++j) s_keys[(rr[i] * 64 + owner) * J + j] = (uint16_t)top[i][j];
+        __syncthreads();
+#pragma unroll 1
+        for (uint32_t j2 = 0; j2 < kBigRowsPerWave; ++j2) {
+            const uint32_t row = kBigRowsPerWave * wave + j2;
+            uint32_t key[J];
+#pragma unroll
+            for (uint32_t j = 0; j < J; ++j) key[j] = s_keys[(row * 64 + lane) * J + j];
+            const float t = key16_upper(wave_select(key, cap));
+            if (lane == 0 && row < nrows) {
+                s_tlim[row] = t;
+                s_abnd[row] = fminf(s_abnd[row], t);
+            }
+        }
+        __syncthreads();  // the pending area is free again, the limits are in place
+    };
+    if (need_lim) limits();
     fetch(0);
 
     for (uint32_t c0 = 0; c0 < b; c0 += kBigCols) {
@@ -604,7 +745,8 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
             if (lane == 0) {
 #pragma unroll
                 for (uint32_t j = 0; j < kBigRowsPerWave; ++j)
-                    if (kBigRowsPerWave * wave + j < nrows) s_abnd[kBigRowsPerWave * wave + j] = lst[j].bound;
+                    if (kBigRowsPerWave * wave + j < nrows)
+                        s_abnd[kBigRowsPerWave * wave + j] = fminf(lst[j].bound, s_tlim[kBigRowsPerWave * wave + j]);
             }
         }
     }

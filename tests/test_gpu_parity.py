@@ -454,6 +454,68 @@ def test_precision_forms_valid_and_close(prec, flags):
     assert r_p > r_f - 0.02, (r_p, r_f)
 
 
+@pytest.mark.parametrize("k,leaf,d,flags", [(100, 400, 100, 0), (10, 400, 64, 0), (100, 1500, 100, 0),
+                                             (100, 400, 128, K.FLAG_BIG_LEAF), (32, 2000, 37, 0)])
+def test_leaf_bucket_limit_keeps_lists_and_cuts_survivors(k, leaf, d, flags):
+    # The exact screen of the LDS leaf kernel lowers a row's bound to the k-th smallest upper
+    # bound of its distances in the bucket (no pair of the k nearest is dropped): the lists must
+    # be those of the screen without it, bit for bit, on empty lists (where it applies) and on a
+    # second round merged into them, with fewer pairs re-ranked where every bucket is LDS-resident;
+    # and both must match the oracle. Buckets above 512 points (or forced onto the big-bucket
+    # kernel) take no limit in the exact screen: same lists, no more re-ranks.
+    X = lowrank(16000, d, 31) if d != 37 else mixture(16000, d, 8, 31)
+    kw = dict(k=k, leaf_max=leaf, small_max=3000, iterations=1, seed=6)
+    o_prm = oracle.make_params(**kw)
+    on = K.Builder(K.params(flags=flags, **kw))
+    off = K.Builder(K.params(flags=flags | K.FLAG_NO_BUCKET_LIMIT, **kw))
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    ids1, ds1 = on.leaf_stage(X, perm, ls, lz)
+    s_on = on.stats()["survivors"]
+    ids0, ds0 = off.leaf_stage(X, perm, ls, lz)
+    s_off = off.stats()["survivors"]
+    np.testing.assert_array_equal(ds1, ds0)
+    np.testing.assert_array_equal(ids1, ids0)
+    assert s_on < 0.8 * s_off if leaf <= 512 and not flags else s_on <= s_off, (s_on, s_off)
+    o_ids, o_ds, o_sz = oracle.leaf_stage(X, k, perm, ls, lz)
+    assert_lists_equal_up_to_ties(X, ids1, ds1, o_ids, o_ds)
+    perm2, ls2, lz2 = oracle.split_round(X, o_prm, 1)
+    a1 = on.leaf_stage(X, perm2, ls2, lz2, state=(ids1, ds1))
+    a0 = off.leaf_stage(X, perm2, ls2, lz2, state=(ids0, ds0))
+    np.testing.assert_array_equal(a1[1], a0[1])
+    np.testing.assert_array_equal(a1[0], a0[0])
+    o2 = oracle.leaf_stage(X, k, perm2, ls2, lz2, state=(o_ids, o_ds, o_sz))
+    assert_lists_equal_up_to_ties(X, a1[0], a1[1], o2[0], o2[1])
+
+
+@pytest.mark.parametrize("prec", [K.PREC_FP16, K.PREC_BF16])
+@pytest.mark.parametrize("flags", [0, K.FLAG_BIG_LEAF])
+def test_rerank_cap_of_the_approximate_forms(prec, flags):
+    # Approximate screens re-rank, per point and bucket, about the `rerank` nearest by the screen:
+    # fewer pairs than every survivor (rerank = 0), every returned distance still the exact FP32
+    # one, and recall against FP64 within 0.01 of the uncapped form's.
+    X = lowrank(20000, 100, 14)
+    kw = dict(k=100, leaf_max=400, small_max=3000, iterations=1, rounds=3, non_passes=0, seed=2)
+    # one leaf stage on empty lists, where every pair passes the bound: about 160 re-ranks a point
+    perm, ls, lz = oracle.split_round(X, oracle.make_params(**kw), 0)
+    capped = K.Builder(K.params(precision=prec, flags=flags, rerank=160, **kw))
+    full = K.Builder(K.params(precision=prec, flags=flags, rerank=0, **kw))
+    capped.leaf_stage(X, perm, ls, lz)
+    full.leaf_stage(X, perm, ls, lz)
+    s_cap, s_full = capped.stats()["survivors"], full.stats()["survivors"]
+    # every pair passes an empty list's bound; the big-bucket kernel tightens the bound between
+    # its column blocks, so it re-ranks fewer
+    all_pairs = int((lz.astype(np.int64) * (lz - 1)).sum())
+    assert s_full == all_pairs if not flags else s_full <= all_pairs
+    assert s_cap < 1.1 * 160 * 20000 and s_cap < 0.75 * s_full, (s_cap, s_full)
+    ids, ds = capped.build(X, return_dists=True)
+    f_ids, _ = full.build(X, return_dists=True)
+    assert_valid_lists(X, ids, ds)
+    q = np.arange(0, 20000, 40, dtype=np.uint32)
+    truth, _ = oracle.exact_knn(X, q, 100)
+    r_c, r_f = K.recall(ids[q], truth), K.recall(f_ids[q], truth)
+    assert r_c > r_f - 0.01, (r_c, r_f)
+
+
 def test_exact_knn_gpu_matches_oracle():
     X = mixture(4000, 100, 20, 51)
     q = np.arange(0, 4000, 37, dtype=np.uint32)
