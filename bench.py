@@ -79,8 +79,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--compare-fp32", action="store_true",
-                    help="with --precision bf16/fp16: also build the FP32 form at the same rounds and "
-                         "report the recall delta and the speed ratio")
+                    help="also build the plain FP32 GPU form (FP32 Gram in both stages) and, with "
+                         "--precision bf16/fp16, the FP32 headline form, at the same rounds, and "
+                         "report the recall deltas and the speed ratios")
     ap.add_argument("--error-queries", type=int, default=10_000,
                     help="sampled points whose returned distances are checked against FP64")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -387,15 +388,23 @@ def main():
             f"({time.perf_counter() - t0:.1f} s)")
         return rec, got
 
+    # Reference forms at the same rounds and seed: the headline FP32 form (16-bit screens with
+    # exact margins) when another precision is measured, and the plain FP32 GPU form (FP32 Gram in
+    # both stages, no 16-bit Gram anywhere); both give the exact FP32 lists.
     cmp32 = None
-    if a.compare_fp32 and a.precision != "fp32" and nq > 0:
-        prm32 = K.params(k=a.k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
-                         rounds=a.rounds, non_passes=a.non, precision=K.PREC_FP32, seed=a.seed)
-        b32 = K.Builder(prm32)
-        t32, _ = run_builds(b32, "fp32")
-        r32, _ = measure_recall(b32)
-        cmp32 = {"fp32_points_per_s": round(n / float(np.mean(t32)), 1), "fp32_recall": round(r32, 4)}
-        del b32
+    if a.compare_fp32 and nq > 0:
+        cmp32 = {}
+        forms = ([("fp32", 0)] if a.precision != "fp32" else []) + \
+                [("plain_fp32", K.FLAG_LEAF_FP32_GRAM | K.FLAG_NON_FP32_GRAM)]
+        for tag, fl in forms:
+            b32 = K.Builder(K.params(k=a.k, leaf_max=a.leaf, small_max=3000, iterations=a.iterations,
+                                     rounds=a.rounds, non_passes=a.non, precision=K.PREC_FP32,
+                                     seed=a.seed, flags=fl))
+            t32, _ = run_builds(b32, tag)
+            r32, _ = measure_recall(b32)
+            cmp32[f"{tag}_points_per_s"] = round(n / float(np.mean(t32)), 1)
+            cmp32[f"{tag}_recall"] = round(r32, 4)
+            del b32
 
     times, stats = run_builds(b, a.precision)
     mean_t = float(np.mean(times))
@@ -462,8 +471,10 @@ def main():
                           "error_queries": ne}
             if cmp32:
                 prec_block.update(cmp32)
-                prec_block["recall_delta_vs_fp32"] = round(rec - cmp32["fp32_recall"], 4)
-                prec_block["speedup_vs_fp32"] = round((n / mean_t) / cmp32["fp32_points_per_s"], 3)
+                for tag in ("fp32", "plain_fp32"):
+                    if f"{tag}_points_per_s" in cmp32:
+                        prec_block[f"recall_delta_vs_{tag}"] = round(rec - cmp32[f"{tag}_recall"], 4)
+                        prec_block[f"speedup_vs_{tag}"] = round((n / mean_t) / cmp32[f"{tag}_points_per_s"], 3)
             log(f"[bench] distance error vs FP64 on {ne} points: max {err.max():.3e} "
                 f"mean {err.mean():.3e} (relative max {relerr.max():.3e})")
 

@@ -64,6 +64,11 @@ extern "C" {
                                         bound only, without the in-bucket limit (the k-th smallest
                                         upper bound of the row's distances in the bucket); the same
                                         lists, for comparisons */
+#define KNNG_FLAG_LEAF_FP32_GRAM 2048u /* exact form: leaf screen on the FP32 Gram (16x16x4 MFMA) of
+                                        the FP32 rows with its rigorous margin, every bucket through
+                                        the big-bucket kernel (FP32 rows do not fit the LDS-resident
+                                        kernel); the same lists. With KNNG_FLAG_NON_FP32_GRAM: the
+                                        plain FP32 GPU form, no 16-bit Gram anywhere */
 #define KNNG_FLAG_AUTOTUNE 32u       /* leaf stage: the first launch of each size bin above 128
                                         points (per device, precision and width, once per process)
                                         also times both workgroup shapes, 8 and 16 waves, and the

@@ -45,6 +45,7 @@ FLAG_LEAF_GRAPH = 256    # ... replayed from a captured hipGraph
 FLAG_KD_SPLIT = 16  # random-rotation KD splitter (SolutionRandomKD) instead of 2-means
 FLAG_AUTOTUNE = 32  # leaf workgroup shape picked by timing both on the first launch of a size bin
 FLAG_NO_BUCKET_LIMIT = 1024  # exact form: leaf screen without the in-bucket k-th upper-bound limit
+FLAG_LEAF_FP32_GRAM = 2048  # exact form: leaf screen on the FP32 Gram, every bucket on the big kernel
 
 GEN_LOWRANK = 0
 GEN_MIXTURE = 1

@@ -477,6 +477,18 @@ __device__ inline uint32_t wave_select(const uint32_t (&t)[J], uint32_t m) {
     return x;
 }
 
+// LeafArgs::half_kind of the FP32 Gram form: 16x16x4 FP32 MFMA over the FP32 rows (lane l gives
+// row l & 15's dims 4 s + (l >> 4) at k-step s; the accumulator holds rows 4 (l >> 4) + i of
+// column l & 15), d4 = ceil(d / 4) steps.
+constexpr uint32_t kHalfF32 = 2;
+__device__ inline float4_t gram_f32(const float* __restrict__ arow, const float* __restrict__ brow,
+                                    uint32_t lk, uint32_t d4) {
+    float4_t c = {0.f, 0.f, 0.f, 0.f};
+    for (uint32_t s = 0; s < d4; ++s)
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * s + lk], brow[4 * s + lk], c, 0, 0, 0);
+    return c;
+}
+
 // Margin coefficient c of the exact screen: |approx - exact| <= c * (n_i + n_j).
 // fp16: 2u + u^2 + accumulation (K<=128) + norm and final rounding, u = 2^-11 -> 9.9e-4; bf16:
 // u = 2^-8 -> 7.9e-3. 25% slack on top.

@@ -145,6 +145,10 @@ bool valid_params(const knng_params& p, std::string& why) {
     if (p.leaf_max_odd > kMaxLeafAny) { why = "leaf_max_odd must be <= 65536"; return false; }
     if (p.small_max > kMaxSmallNode) { why = "small_max must be <= 4096"; return false; }
     if (p.precision > KNNG_PREC_BF16) { why = "unknown precision"; return false; }
+    if ((p.flags & KNNG_FLAG_LEAF_FP32_GRAM) && p.precision != KNNG_PREC_FP32) {
+        why = "KNNG_FLAG_LEAF_FP32_GRAM is a form of the exact precision (KNNG_PREC_FP32)";
+        return false;
+    }
     return true;
 }
 
@@ -779,7 +783,7 @@ struct Builder {
     int plan_leaves(const std::vector<LeafDesc>& lv, LeafPlan& pl) const {
         if (d > 128) return fail(KNNG_ERR_UNSUPPORTED, "leaf stage supports d <= 128");
         static constexpr uint32_t kEdges[4] = {64, 128, 256, kMaxLeaf};
-        const bool force_big = (prm.flags & KNNG_FLAG_BIG_LEAF) != 0;
+        const bool force_big = (prm.flags & (KNNG_FLAG_BIG_LEAF | KNNG_FLAG_LEAF_FP32_GRAM)) != 0;
         pl.nleaves = lv.size();
         for (auto& l : lv) {
             pl.pairs += (uint64_t)l.size * (l.size - 1) / 2;
@@ -952,6 +956,7 @@ struct Builder {
     // The 16-bit type of the leaf screen: the precision's in the approximate forms; in the exact
     // form fp16 unless the tuner kept bf16.
     uint32_t screen_half() const {
+        if (prm.flags & KNNG_FLAG_LEAF_FP32_GRAM) return knng::kHalfF32;
         if (prm.precision == KNNG_PREC_BF16) return 1;
         if (prm.precision == KNNG_PREC_FP16) return 0;
         return leaf_half;
@@ -964,6 +969,8 @@ struct Builder {
             KNNG_HIP(hipEventRecord(e1, st));
             return KNNG_OK;
         }
+        if (prm.flags & KNNG_FLAG_LEAF_FP32_GRAM)  // one form: every bucket on the big-bucket kernel
+            return leaf_stage_binned(lv, pm, slot, sync, e0, e1);
         if (prm.flags & KNNG_FLAG_AUTOTUNE) {
             bool ran = false;
             KNNG_TRY(tune_leaf_stage(lv, pm, slot, sync, e0, e1, ran));

@@ -93,7 +93,7 @@ struct LeafArgs {
     float* knn_d;
     float* bound;          // n: knn_d[row][k-1]
     uint32_t screen;       // kScreenExact: rigorous margin; kScreenApprox: plain approximate filter
-    uint32_t half_kind;    // 0 = fp16, 1 = bf16 screening Gram
+    uint32_t half_kind;    // 0 = fp16, 1 = bf16 screening Gram, 2 (kHalfF32) = FP32 Gram (big kernel)
     uint32_t max_leaf;     // largest leaf size in this launch
     uint64_t* counters;    // [0] survivors screened, [1] exact distances, [2] inserts (nullable)
     uint32_t tune;         // 1: pick the workgroup shape by timing both (KNNG_FLAG_AUTOTUNE)

@@ -442,8 +442,13 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
             if (lane < ld) frow[lane] = x0;
             if (lane + 64 < ld) frow[lane + 64] = x1;
             float nrm, rinv;
-            screen_vals<HALF>(lane < d ? x0 - mu0 : 0.f, lane + 64 < d ? x1 - mu1 : 0.f, d,
-                              L.rs, hrow, lane, nrm, rinv);
+            if constexpr (HALF == kHalfF32) {  // the FP32 Gram reads the FP32 rows: their norms
+                nrm = wave_sum(__builtin_fmaf(x0, x0, x1 * x1));
+                rinv = 1.f;
+            } else {
+                screen_vals<HALF>(lane < d ? x0 - mu0 : 0.f, lane + 64 < d ? x1 - mu1 : 0.f, d,
+                                  L.rs, hrow, lane, nrm, rinv);
+            }
             if (lane == 0) {
                 s_anrm[rr] = nrm;
                 s_arinv[rr] = rinv;
@@ -461,7 +466,10 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
     }
 
     const bool exact = a.screen == kScreenExact;
-    const float margin_c = HalfTraits<HALF>::kMargin;
+    // FP32 Gram over K = 4 ceil(d/4) terms: |approx - exact| <= (K + 8) 2^-23 (n_i + n_j)
+    const uint32_t d4 = (d + 3) / 4;
+    const float margin_c = HALF == kHalfF32 ? (float)(4 * d4 + 8) * (1.0f / 8388608.0f)
+                                            : HalfTraits<HALF == kHalfF32 ? 0 : HALF>::kMargin;
     const uint32_t lrow = lane & 15;
     const uint32_t strip = wave >> 2, tile = wave & 3;  // phase 1: 4 strips x 4 tiles
     bool changed[kBigRowsPerWave];
@@ -503,8 +511,13 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
             if (lane + 64 < ld) frow[lane + 64] = cx1[u];
             if (cid[u] != kNoId) {
                 float nrm, rinv;
-                screen_vals<HALF>(lane < d ? cx0[u] - mu0 : 0.f, lane + 64 < d ? cx1[u] - mu1 : 0.f, d,
-                                  L.rs, hrow, lane, nrm, rinv);
+                if constexpr (HALF == kHalfF32) {
+                    nrm = wave_sum(__builtin_fmaf(cx0[u], cx0[u], cx1[u] * cx1[u]));
+                    rinv = 1.f;
+                } else {
+                    screen_vals<HALF>(lane < d ? cx0[u] - mu0 : 0.f, lane + 64 < d ? cx1[u] - mu1 : 0.f, d,
+                                      L.rs, hrow, lane, nrm, rinv);
+                }
                 if (lane == 0) {
                     s_cnrm[cc] = nrm;
                     s_crinv[cc] = rinv;
@@ -596,7 +609,9 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
         }
         __syncthreads();  // the pending area is free again, the limits are in place
     };
-    if (need_lim) limits();
+    if constexpr (HALF != kHalfF32) {  // the FP32 Gram form is exact: no cap
+        if (need_lim) limits();
+    }
     fetch(0);
 
     for (uint32_t c0 = 0; c0 < b; c0 += kBigCols) {
@@ -608,8 +623,8 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
 
         // ---- phase 1: screen this wave's tile ------------------------------------------------
         {
-            HalfRow<HALF, KS> af;
-            af.load(s_arow + (16 * strip + lrow) * L.rs, lane >> 4);
+            HalfRow<HALF == kHalfF32 ? 0 : HALF, KS> af;
+            if constexpr (HALF != kHalfF32) af.load(s_arow + (16 * strip + lrow) * L.rs, lane >> 4);
             float rn[4], rinv[4], thr_base[4];
             uint32_t rr[4];
 #pragma unroll
@@ -622,9 +637,15 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
             }
             {
                 const uint32_t t = tile;
-                HalfRow<HALF, KS> bf;
-                bf.load(s_crow + (16 * t + lrow) * L.rs, lane >> 4);
-                const float4_t acc = gram16(af, bf);
+                float4_t acc;
+                if constexpr (HALF == kHalfF32) {
+                    acc = gram_f32(s_af32 + (16 * strip + lrow) * L.fs, s_cf32 + (16 * t + lrow) * L.fs,
+                                   lane >> 4, d4);
+                } else {
+                    HalfRow<HALF, KS> bf;
+                    bf.load(s_crow + (16 * t + lrow) * L.rs, lane >> 4);
+                    acc = gram16(af, bf);
+                }
                 const uint32_t c = 16 * t + lrow;
                 const float cn = s_cnrm[c], cinv = s_crinv[c];
                 uint64_t bits[4];
@@ -837,7 +858,8 @@ void launch_leaf_big(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks
     if (!nblocks) return;
     const size_t lds = leaf_big_lds_bytes(a.d, a.ld);
     if (a.half_kind == 0) launch_leaf_big_h<0>(a, blocks, nblocks, lds, st);
-    else launch_leaf_big_h<1>(a, blocks, nblocks, lds, st);
+    else if (a.half_kind == 1) launch_leaf_big_h<1>(a, blocks, nblocks, lds, st);
+    else launch_leaf_big_h<kHalfF32>(a, blocks, nblocks, lds, st);
 }
 
 size_t leaf_lds_bytes(uint32_t max_leaf, uint32_t d, uint32_t ld) {

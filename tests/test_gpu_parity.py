@@ -184,6 +184,39 @@ def test_leaf_stage_big_kernel_matches_oracle(k, leaf, d):
     assert_lists_equal_up_to_ties(X, ids2, ds2, o2[0], o2[1])
 
 
+@pytest.mark.parametrize("k,leaf,d", [(100, 256, 100), (10, 400, 100), (32, 128, 64), (100, 400, 128),
+                                      (20, 1500, 37)])
+def test_leaf_fp32_gram_matches_oracle(k, leaf, d):
+    # the FP32 Gram form of the leaf screen (16x16x4 MFMA over the FP32 rows, FP32 margin), every
+    # bucket on the big-bucket kernel: the same lists as the oracle, on empty lists and merged
+    X = mixture(12000, d, 40, 27)
+    kw = dict(k=k, leaf_max=leaf, small_max=3000, iterations=1, seed=10)
+    o_prm = oracle.make_params(**kw)
+    b = K.Builder(K.params(flags=K.FLAG_LEAF_FP32_GRAM, **kw))
+    perm, ls, lz = oracle.split_round(X, o_prm, 0)
+    ids, ds = b.leaf_stage(X, perm, ls, lz)
+    o_ids, o_ds, o_sz = oracle.leaf_stage(X, k, perm, ls, lz)
+    assert_lists_equal_up_to_ties(X, ids, ds, o_ids, o_ds)
+    perm2, ls2, lz2 = oracle.split_round(X, o_prm, 1)
+    ids2, ds2 = b.leaf_stage(X, perm2, ls2, lz2, state=(ids, ds))
+    o2 = oracle.leaf_stage(X, k, perm2, ls2, lz2, state=(o_ids, o_ds, o_sz))
+    assert_lists_equal_up_to_ties(X, ids2, ds2, o2[0], o2[1])
+
+
+def test_plain_fp32_form_equals_the_headline_form():
+    # FP32 Gram in both stages (no 16-bit Gram anywhere) against the fp16-screened headline form:
+    # both exact, so the same lists bit for bit
+    X = lowrank(30000, 100, 17)
+    kw = dict(k=100, leaf_max=400, small_max=3000, iterations=1, rounds=3, non_passes=1, seed=5)
+    ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
+    plain = K.Builder(K.params(flags=K.FLAG_LEAF_FP32_GRAM | K.FLAG_NON_FP32_GRAM, **kw))
+    p_ids, p_ds = plain.build(X, return_dists=True)
+    np.testing.assert_array_equal(p_ds, ds)
+    np.testing.assert_array_equal(p_ids, ids)
+    with pytest.raises(K.KnngError):
+        K.Builder(K.params(precision=K.PREC_BF16, flags=K.FLAG_LEAF_FP32_GRAM, **kw))
+
+
 @pytest.mark.parametrize("d,leaf", [(100, 2000), (128, 1500), (37, 3000)])
 def test_leaf_stage_large_buckets(d, leaf):
     # buckets beyond the LDS kernel (up to `leaf` points) against brute force
