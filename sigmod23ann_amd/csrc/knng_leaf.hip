@@ -392,8 +392,9 @@ __host__ __device__ inline BigLds big_lds_layout(uint32_t d, uint32_t ld) {
 
 size_t leaf_big_lds_bytes(uint32_t d, uint32_t ld) { return big_lds_layout(d, ld).total; }
 
-// One 64-row block of a bucket, by the whole workgroup (kBigThreads).
-template <uint32_t HALF, uint32_t KS>
+// One 64-row block of a bucket, by the whole workgroup (kBigThreads). CAP: the approximate
+// screens' re-rank cap is compiled in (the exact forms' instances leave it out, and its registers).
+template <uint32_t HALF, uint32_t KS, bool CAP>
 __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t b = bb.size, r0 = bb.row0;
@@ -546,71 +547,70 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
     float* s_tlim = reinterpret_cast<float*>(lds + L.off_tlim);
     if (lane < kBigRowsPerWave) s_tlim[kBigRowsPerWave * wave + lane] = __builtin_inff();
     __syncthreads();
-    const uint32_t cap = a.rank_cap;
-    const bool need_lim = cap != 0 && !exact;
-    auto limits = [&]() {
-        constexpr uint32_t J = kBigLimKeys;
-        uint32_t top[4][J];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (uint32_t j = 0; j < J; ++j) top[i][j] = kKeyInf;
-        HalfRow<HALF, KS> af;
-        af.load(s_arow + (16 * strip + lrow) * L.rs, lane >> 4);
-        uint32_t rr[4];
-        float rn[4], rinv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            rr[i] = 16 * strip + (lane >> 4) * 4 + i;
-            rn[i] = s_anrm[rr[i]];
-            rinv[i] = s_arinv[rr[i]];
-        }
-        fetch(0);
-        for (uint32_t c0 = 0; c0 < b; c0 += kBigCols) {
-            const uint32_t nc = min(kBigCols, b - c0);
-            __syncthreads();
-            commit();
-            __syncthreads();
-            if (c0 + kBigCols < b) fetch(c0 + kBigCols);
-            HalfRow<HALF, KS> bf;
-            bf.load(s_crow + (16 * tile + lrow) * L.rs, lane >> 4);
-            const float4_t acc = gram16(af, bf);
-            const uint32_t c = 16 * tile + lrow;
-            const float cn = s_cnrm[c], cinv = s_crinv[c];
-#pragma unroll
+    if constexpr (CAP && HALF != kHalfF32) {  // the FP32 Gram form is exact: no cap
+        const uint32_t cap = a.rank_cap;
+        auto limits = [&]() {
+            constexpr uint32_t J = kBigLimKeys;
+            uint32_t top[4][J];
+    #pragma unroll
+            for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                for (uint32_t j = 0; j < J; ++j) top[i][j] = kKeyInf;
+            HalfRow<HALF, KS> af;
+            af.load(s_arow + (16 * strip + lrow) * L.rs, lane >> 4);
+            uint32_t rr[4];
+            float rn[4], rinv[4];
+    #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float g = acc[i] * (rinv[i] * cinv);
-                const float nsum = rn[i] + cn;
-                const float approx = nsum - 2.0f * g;
-                const float u = exact ? approx + margin_c * nsum : approx;
-                const uint32_t ok = (uint32_t)(c < nc) & (uint32_t)(rr[i] < nrows) & (uint32_t)(r0 + rr[i] != c0 + c);
-                top_insert(top[i], key16_valid(u, ok));
+                rr[i] = 16 * strip + (lane >> 4) * 4 + i;
+                rn[i] = s_anrm[rr[i]];
+                rinv[i] = s_arinv[rr[i]];
             }
-        }
-        uint16_t* s_keys = reinterpret_cast<uint16_t*>(lds + L.off_pend);  // 64 rows x 64 owners x J
-        static_assert(kBigRows * 64 * J * 2 <= kBigWaves * kBigRowsPerWave * 64 * 8, "keys fit the pending area");
-        const uint32_t owner = 16 * tile + lrow;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (uint32_t j = 0; j < J; ++j) s_keys[(rr[i] * 64 + owner) * J + j] = (uint16_t)top[i][j];
-        __syncthreads();
-#pragma unroll 1
-        for (uint32_t j2 = 0; j2 < kBigRowsPerWave; ++j2) {
-            const uint32_t row = kBigRowsPerWave * wave + j2;
-            uint32_t key[J];
-#pragma unroll
-            for (uint32_t j = 0; j < J; ++j) key[j] = s_keys[(row * 64 + lane) * J + j];
-            const float t = key16_upper(wave_select(key, cap));
-            if (lane == 0 && row < nrows) {
-                s_tlim[row] = t;
-                s_abnd[row] = fminf(s_abnd[row], t);
+            fetch(0);
+            for (uint32_t c0 = 0; c0 < b; c0 += kBigCols) {
+                const uint32_t nc = min(kBigCols, b - c0);
+                __syncthreads();
+                commit();
+                __syncthreads();
+                if (c0 + kBigCols < b) fetch(c0 + kBigCols);
+                HalfRow<HALF, KS> bf;
+                bf.load(s_crow + (16 * tile + lrow) * L.rs, lane >> 4);
+                const float4_t acc = gram16(af, bf);
+                const uint32_t c = 16 * tile + lrow;
+                const float cn = s_cnrm[c], cinv = s_crinv[c];
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float g = acc[i] * (rinv[i] * cinv);
+                    const float nsum = rn[i] + cn;
+                    const float approx = nsum - 2.0f * g;
+                    const float u = exact ? approx + margin_c * nsum : approx;
+                    const uint32_t ok = (uint32_t)(c < nc) & (uint32_t)(rr[i] < nrows) & (uint32_t)(r0 + rr[i] != c0 + c);
+                    top_insert(top[i], key16_valid(u, ok));
+                }
             }
-        }
-        __syncthreads();  // the pending area is free again, the limits are in place
-    };
-    if constexpr (HALF != kHalfF32) {  // the FP32 Gram form is exact: no cap
-        if (need_lim) limits();
+            uint16_t* s_keys = reinterpret_cast<uint16_t*>(lds + L.off_pend);  // 64 rows x 64 owners x J
+            static_assert(kBigRows * 64 * J * 2 <= kBigWaves * kBigRowsPerWave * 64 * 8, "keys fit the pending area");
+            const uint32_t owner = 16 * tile + lrow;
+    #pragma unroll
+            for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                for (uint32_t j = 0; j < J; ++j) s_keys[(rr[i] * 64 + owner) * J + j] = (uint16_t)top[i][j];
+            __syncthreads();
+    #pragma unroll 1
+            for (uint32_t j2 = 0; j2 < kBigRowsPerWave; ++j2) {
+                const uint32_t row = kBigRowsPerWave * wave + j2;
+                uint32_t key[J];
+    #pragma unroll
+                for (uint32_t j = 0; j < J; ++j) key[j] = s_keys[(row * 64 + lane) * J + j];
+                const float t = key16_upper(wave_select(key, cap));
+                if (lane == 0 && row < nrows) {
+                    s_tlim[row] = t;
+                    s_abnd[row] = fminf(s_abnd[row], t);
+                }
+            }
+            __syncthreads();  // the pending area is free again, the limits are in place
+        };
+        if (cap != 0 && !exact) limits();
     }
     fetch(0);
 
@@ -785,9 +785,9 @@ __device__ inline void big_block(const LeafArgs& a, const BigBlock bb) {
     }
 }
 
-template <uint32_t HALF, uint32_t KS>
+template <uint32_t HALF, uint32_t KS, bool CAP>
 __global__ __launch_bounds__(kBigThreads) void k_leaf_big(LeafArgs a, const BigBlock* __restrict__ blocks) {
-    big_block<HALF, KS>(a, blocks[blockIdx.x]);
+    big_block<HALF, KS, CAP>(a, blocks[blockIdx.x]);
 }
 
 struct LeafQueue;  // knng_kernels.hpp
@@ -805,16 +805,16 @@ __global__ void k_leaf_queue_reset(LeafQueue* q) {
 
 // The two item kinds as separate functions: each gets its own register allocation, as in its
 // stand-alone kernel, instead of one allocation for both.
-template <uint32_t HALF, uint32_t KS>
+template <uint32_t HALF, uint32_t KS, bool CAP>
 __device__ __attribute__((noinline)) void big_block_item(const LeafArgs& a, const BigBlock bb) {
-    big_block<HALF, KS>(a, bb);
+    big_block<HALF, KS, CAP>(a, bb);
 }
 template <uint32_t HALF, uint32_t KS>
 __device__ __attribute__((noinline)) void leaf_bucket_item(const LeafArgs& a, const LeafDesc lf) {
     leaf_bucket<HALF, KS, kLeafWavesLarge>(a, lf);
 }
 
-template <uint32_t HALF, uint32_t KS>
+template <uint32_t HALF, uint32_t KS, bool CAP>
 __global__ __launch_bounds__(kBigThreads) void k_leaf_batched(LeafArgs a, const BigBlock* __restrict__ blocks,
                                                               LeafQueue* __restrict__ q) {
     static_assert(kBigThreads == kLeafWavesLarge * 64, "both item kinds take 16 waves");
@@ -824,42 +824,48 @@ __global__ __launch_bounds__(kBigThreads) void k_leaf_batched(LeafArgs a, const 
         if (threadIdx.x == 0) s_item = atomicAdd(&q->next, 1u);
         __syncthreads();
         const uint32_t it = s_item;
-        if (it < nbig) big_block_item<HALF, KS>(a, blocks[it]);
+        if (it < nbig) big_block_item<HALF, KS, CAP>(a, blocks[it]);
         else if (it - nbig < nleaf) leaf_bucket_item<HALF, KS>(a, a.leaves[it - nbig]);
         __syncthreads();  // the item's LDS use and s_item read are over before the next item
         if (it >= nbig + nleaf) break;
     }
 }
 
-template <uint32_t HALF, uint32_t KS>
+template <uint32_t HALF, uint32_t KS, bool CAP>
 static void launch_leaf_big_t(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks, size_t lds,
                               hipStream_t st) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf_big<HALF, KS>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf_big<HALF, KS, CAP>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    k_leaf_big<HALF, KS><<<nblocks, kBigThreads, lds, st>>>(a, blocks);
+    k_leaf_big<HALF, KS, CAP><<<nblocks, kBigThreads, lds, st>>>(a, blocks);
 }
 
-template <uint32_t HALF>
+template <uint32_t HALF, bool CAP>
 static void launch_leaf_big_h(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks, size_t lds,
                               hipStream_t st) {
     switch ((a.d + 15) / 16) {
-        case 1: launch_leaf_big_t<HALF, 1>(a, blocks, nblocks, lds, st); break;
-        case 2: launch_leaf_big_t<HALF, 2>(a, blocks, nblocks, lds, st); break;
-        case 3: launch_leaf_big_t<HALF, 3>(a, blocks, nblocks, lds, st); break;
-        case 4: launch_leaf_big_t<HALF, 4>(a, blocks, nblocks, lds, st); break;
-        case 5: launch_leaf_big_t<HALF, 5>(a, blocks, nblocks, lds, st); break;
-        case 6: launch_leaf_big_t<HALF, 6>(a, blocks, nblocks, lds, st); break;
-        case 7: launch_leaf_big_t<HALF, 7>(a, blocks, nblocks, lds, st); break;
-        default: launch_leaf_big_t<HALF, 8>(a, blocks, nblocks, lds, st); break;
+        case 1: launch_leaf_big_t<HALF, 1, CAP>(a, blocks, nblocks, lds, st); break;
+        case 2: launch_leaf_big_t<HALF, 2, CAP>(a, blocks, nblocks, lds, st); break;
+        case 3: launch_leaf_big_t<HALF, 3, CAP>(a, blocks, nblocks, lds, st); break;
+        case 4: launch_leaf_big_t<HALF, 4, CAP>(a, blocks, nblocks, lds, st); break;
+        case 5: launch_leaf_big_t<HALF, 5, CAP>(a, blocks, nblocks, lds, st); break;
+        case 6: launch_leaf_big_t<HALF, 6, CAP>(a, blocks, nblocks, lds, st); break;
+        case 7: launch_leaf_big_t<HALF, 7, CAP>(a, blocks, nblocks, lds, st); break;
+        default: launch_leaf_big_t<HALF, 8, CAP>(a, blocks, nblocks, lds, st); break;
     }
 }
+
+// The approximate screens' re-rank cap is compiled into its own instances of the big-bucket kernel.
+static bool leaf_cap(const LeafArgs& a) { return a.rank_cap != 0 && a.screen == kScreenApprox; }
 
 void launch_leaf_big(const LeafArgs& a, const BigBlock* blocks, uint32_t nblocks, hipStream_t st) {
     if (!nblocks) return;
     const size_t lds = leaf_big_lds_bytes(a.d, a.ld);
-    if (a.half_kind == 0) launch_leaf_big_h<0>(a, blocks, nblocks, lds, st);
-    else if (a.half_kind == 1) launch_leaf_big_h<1>(a, blocks, nblocks, lds, st);
-    else launch_leaf_big_h<kHalfF32>(a, blocks, nblocks, lds, st);
+    const bool cap = leaf_cap(a);
+    if (a.half_kind == 0 && cap) launch_leaf_big_h<0, true>(a, blocks, nblocks, lds, st);
+    else if (a.half_kind == 0) launch_leaf_big_h<0, false>(a, blocks, nblocks, lds, st);
+    else if (a.half_kind == 1 && cap) launch_leaf_big_h<1, true>(a, blocks, nblocks, lds, st);
+    else if (a.half_kind == 1) launch_leaf_big_h<1, false>(a, blocks, nblocks, lds, st);
+    else launch_leaf_big_h<kHalfF32, false>(a, blocks, nblocks, lds, st);
 }
 
 size_t leaf_lds_bytes(uint32_t max_leaf, uint32_t d, uint32_t ld) {
@@ -879,10 +885,10 @@ size_t leaf_batched_lds(uint32_t d, uint32_t ld) {
     return std::max(a, leaf_big_lds_bytes(d, ld));
 }
 
-template <uint32_t HALF, uint32_t KS>
+template <uint32_t HALF, uint32_t KS, bool CAP>
 static void launch_leaf_batched_t(const LeafArgs& a, const BigBlock* blocks, LeafQueue* q, hipStream_t st) {
     const size_t lds = leaf_batched_lds(a.d, a.ld);
-    auto kern = &k_leaf_batched<HALF, KS>;
+    auto kern = &k_leaf_batched<HALF, KS, CAP>;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     int dev = 0, cus = 0, per_cu = 0;
@@ -894,23 +900,26 @@ static void launch_leaf_batched_t(const LeafArgs& a, const BigBlock* blocks, Lea
     kern<<<grid, kBigThreads, lds, st>>>(a, blocks, q);
 }
 
-template <uint32_t HALF>
+template <uint32_t HALF, bool CAP>
 static void launch_leaf_batched_h(const LeafArgs& a, const BigBlock* blocks, LeafQueue* q, hipStream_t st) {
     switch ((a.d + 15) / 16) {
-        case 1: launch_leaf_batched_t<HALF, 1>(a, blocks, q, st); break;
-        case 2: launch_leaf_batched_t<HALF, 2>(a, blocks, q, st); break;
-        case 3: launch_leaf_batched_t<HALF, 3>(a, blocks, q, st); break;
-        case 4: launch_leaf_batched_t<HALF, 4>(a, blocks, q, st); break;
-        case 5: launch_leaf_batched_t<HALF, 5>(a, blocks, q, st); break;
-        case 6: launch_leaf_batched_t<HALF, 6>(a, blocks, q, st); break;
-        case 7: launch_leaf_batched_t<HALF, 7>(a, blocks, q, st); break;
-        default: launch_leaf_batched_t<HALF, 8>(a, blocks, q, st); break;
+        case 1: launch_leaf_batched_t<HALF, 1, CAP>(a, blocks, q, st); break;
+        case 2: launch_leaf_batched_t<HALF, 2, CAP>(a, blocks, q, st); break;
+        case 3: launch_leaf_batched_t<HALF, 3, CAP>(a, blocks, q, st); break;
+        case 4: launch_leaf_batched_t<HALF, 4, CAP>(a, blocks, q, st); break;
+        case 5: launch_leaf_batched_t<HALF, 5, CAP>(a, blocks, q, st); break;
+        case 6: launch_leaf_batched_t<HALF, 6, CAP>(a, blocks, q, st); break;
+        case 7: launch_leaf_batched_t<HALF, 7, CAP>(a, blocks, q, st); break;
+        default: launch_leaf_batched_t<HALF, 8, CAP>(a, blocks, q, st); break;
     }
 }
 
 void launch_leaf_batched(const LeafArgs& a, const BigBlock* blocks, LeafQueue* q, hipStream_t st) {
-    if (a.half_kind == 0) launch_leaf_batched_h<0>(a, blocks, q, st);
-    else launch_leaf_batched_h<1>(a, blocks, q, st);
+    const bool cap = leaf_cap(a);
+    if (a.half_kind == 0 && cap) launch_leaf_batched_h<0, true>(a, blocks, q, st);
+    else if (a.half_kind == 0) launch_leaf_batched_h<0, false>(a, blocks, q, st);
+    else if (cap) launch_leaf_batched_h<1, true>(a, blocks, q, st);
+    else launch_leaf_batched_h<1, false>(a, blocks, q, st);
 }
 
 template <uint32_t HALF, uint32_t KS, uint32_t W>
