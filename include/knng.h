@@ -43,6 +43,10 @@ extern "C" {
 #define KNNG_FLAG_SMALL_NON_CAP 8u   /* tests: 16 neighbours-of-neighbours candidate slots per point
                                         instead of 512, so the overflow hand-off to the per-point
                                         direct kernel runs */
+#define KNNG_FLAG_NON_ONE_SET 4096u  /* tests: one set of neighbours-of-neighbours candidate buffers
+                                        (each sub-pass's merge before the next screen), as a build
+                                        takes when a second set would not leave half the free
+                                        device memory; same lists */
 #define KNNG_FLAG_NON_SCREEN16 64u   /* neighbours-of-neighbours screen on a 16-bit Gram (16x16x32
                                         MFMA) of rows centred on each q instead of the FP32 Gram
                                         (d of 61-64, 97-100 or 125-128; fp16 with the exact margin

@@ -46,6 +46,7 @@ FLAG_KD_SPLIT = 16  # random-rotation KD splitter (SolutionRandomKD) instead of 
 FLAG_AUTOTUNE = 32  # leaf workgroup shape picked by timing both on the first launch of a size bin
 FLAG_NO_BUCKET_LIMIT = 1024  # exact form: leaf screen without the in-bucket k-th upper-bound limit
 FLAG_LEAF_FP32_GRAM = 2048  # exact form: leaf screen on the FP32 Gram, every bucket on the big kernel
+FLAG_NON_ONE_SET = 4096  # tests: one set of NoN candidate buffers (as a build near device capacity)
 
 GEN_LOWRANK = 0
 GEN_MIXTURE = 1

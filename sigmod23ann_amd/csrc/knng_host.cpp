@@ -317,6 +317,7 @@ struct Builder {
     // beside the next sub-pass's screen
     DevBuf<uint32_t> ncnt2, nbuf_id2;
     DevBuf<float> nbuf_d2;
+    bool non_two_sets = true;  // this pass uses the second set
 
     // generator scratch
     DevBuf<float> gen_a, gen_b, gen_c;
@@ -1143,9 +1144,22 @@ struct Builder {
         KNNG_TRY(ncnt.ensure(n));
         KNNG_TRY(nbuf_d.ensure((size_t)n * non_cap()));
         KNNG_TRY(nbuf_id.ensure((size_t)n * non_cap()));
-        KNNG_TRY(ncnt2.ensure(n));
-        KNNG_TRY(nbuf_d2.ensure((size_t)n * non_cap()));
-        KNNG_TRY(nbuf_id2.ensure((size_t)n * non_cap()));
+        // The second candidate set (the merge of one sub-pass beside the next screen; 13 ms per
+        // 10M build) is taken only while it leaves half the free memory: at 10M points its 41 GB
+        // fit, and a build near 40M points on one GPU still runs, on one set with each merge
+        // before the next screen. KNNG_FLAG_NON_ONE_SET (tests) asks for one set.
+        {
+            const size_t set_bytes = (size_t)n * non_cap() * 8 + (size_t)n * 4;
+            size_t free_b = 0, total_b = 0;
+            KNNG_HIP(hipMemGetInfo(&free_b, &total_b));
+            const bool have = nbuf_id2.cap >= (size_t)n * non_cap() && ncnt2.cap >= n;
+            non_two_sets = !(prm.flags & KNNG_FLAG_NON_ONE_SET) && (have || set_bytes <= free_b / 2);
+        }
+        if (non_two_sets) {
+            KNNG_TRY(ncnt2.ensure(n));
+            KNNG_TRY(nbuf_d2.ensure((size_t)n * non_cap()));
+            KNNG_TRY(nbuf_id2.ensure((size_t)n * non_cap()));
+        }
         KNNG_TRY(ovf.ensure((size_t)n + 1));
         KNNG_TRY(ovf_flag.ensure(n));
         KNNG_TRY(perm.ensure(n));
@@ -1231,10 +1245,13 @@ struct Builder {
             a.order = perm.p + sp;
             a.order_stride = subs;
             a.nq = (n - sp + subs - 1) / subs;
-            a.ncnt = (sp & 1) ? ncnt2.p : ncnt.p;
-            a.nbuf_d = (sp & 1) ? nbuf_d2.p : nbuf_d.p;
-            a.nbuf_id = (sp & 1) ? nbuf_id2.p : nbuf_id.p;
-            if (sp >= 2) KNNG_HIP(hipStreamWaitEvent(st, spev[4 * (sp - 2) + 3], 0));
+            const bool second = non_two_sets && (sp & 1);
+            a.ncnt = second ? ncnt2.p : ncnt.p;
+            a.nbuf_d = second ? nbuf_d2.p : nbuf_d.p;
+            a.nbuf_id = second ? nbuf_id2.p : nbuf_id.p;
+            // the screen that reuses a set waits for that set's merge
+            const uint32_t nsets = non_two_sets ? 2u : 1u;
+            if (sp >= nsets) KNNG_HIP(hipStreamWaitEvent(st, spev[4 * (sp - nsets) + 3], 0));
             KNNG_HIP(hipMemsetAsync(a.ncnt, 0, (size_t)n * 4, st));
             KNNG_HIP(hipEventRecord(spev[4 * sp], st));
             launch_non_reverse(a, st);

@@ -203,6 +203,17 @@ def test_leaf_fp32_gram_matches_oracle(k, leaf, d):
     assert_lists_equal_up_to_ties(X, ids2, ds2, o2[0], o2[1])
 
 
+def test_non_one_candidate_set_equals_two_sets():
+    # one set of candidate buffers (each sub-pass merged before the next screen, as a build near
+    # the device's capacity runs) against the default two sets: the same lists bit for bit
+    X = lowrank(30000, 100, 19)
+    kw = dict(k=100, leaf_max=400, small_max=3000, iterations=1, rounds=3, non_passes=2, seed=8)
+    ids, ds = K.Builder(K.params(**kw)).build(X, return_dists=True)
+    o_ids, o_ds = K.Builder(K.params(flags=K.FLAG_NON_ONE_SET, **kw)).build(X, return_dists=True)
+    np.testing.assert_array_equal(o_ds, ds)
+    np.testing.assert_array_equal(o_ids, ids)
+
+
 def test_plain_fp32_form_equals_the_headline_form():
     # FP32 Gram in both stages (no 16-bit Gram anywhere) against the fp16-screened headline form:
     # both exact, so the same lists bit for bit
