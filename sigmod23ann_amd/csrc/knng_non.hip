@@ -528,10 +528,20 @@ __device__ inline float4_t gram_tile_f16s(const float* __restrict__ arow, const 
 #pragma unroll
     for (uint32_t m = 0; m < K32; ++m) {
         const uint32_t q0 = 8 * m + 2 * g;  // float4 index of the lane's first dims
-        const float4 a0 = q0 < KS ? reinterpret_cast<const float4*>(arow)[q0] : z;
-        const float4 a1 = q0 + 1 < KS ? reinterpret_cast<const float4*>(arow)[q0 + 1] : z;
-        const float4 b0 = q0 < KS ? reinterpret_cast<const float4*>(brow)[q0] : z;
-        const float4 b1 = q0 + 1 < KS ? reinterpret_cast<const float4*>(brow)[q0 + 1] : z;
+        // past 4 KS dims: load an in-row float4 and select zero (a select per value, where a
+        // guarded load per value made a branch and an LDS wait of each)
+        const bool v0 = q0 < KS, v1 = q0 + 1 < KS;
+        const uint32_t i0 = v0 ? q0 : 0u, i1 = v1 ? q0 + 1 : 0u;
+        float4 a0 = reinterpret_cast<const float4*>(arow)[i0], a1 = reinterpret_cast<const float4*>(arow)[i1];
+        float4 b0 = reinterpret_cast<const float4*>(brow)[i0], b1 = reinterpret_cast<const float4*>(brow)[i1];
+        if (!v0) {
+            a0 = z;
+            b0 = z;
+        }
+        if (!v1) {
+            a1 = z;
+            b1 = z;
+        }
         const half8_t ha = {(_Float16)(a0.x * sa), (_Float16)(a0.y * sa), (_Float16)(a0.z * sa), (_Float16)(a0.w * sa),
                             (_Float16)(a1.x * sa), (_Float16)(a1.y * sa), (_Float16)(a1.z * sa), (_Float16)(a1.w * sa)};
         const half8_t hb = {(_Float16)(b0.x * sb), (_Float16)(b0.y * sb), (_Float16)(b0.z * sb), (_Float16)(b0.w * sb),
