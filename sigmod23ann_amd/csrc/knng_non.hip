@@ -1054,8 +1054,10 @@ void k_non_reverse(NonRevArgs a) {
                     const float thr = kFold ? thr0[i]
                                       : SCR == 0 || exact16 ? __builtin_fmaf(margin_c, nsum, thr0[i]) : thr0[i];
                     // <=: a distance equal to a zero bound can still win on its id
-                    const bool ok = cok && rok[i] && rid != pid[i] && !((mw[i] >> cb) & 1u) &&
-                                    approx <= thr;
+                    // every term evaluated (bitwise &): a short-circuit chain here became
+                    // branches around the approximate distance
+                    const bool ok = cok & rok[i] & (rid != pid[i]) & !((mw[i] >> cb) & 1u) &
+                                    (approx <= thr);
                     const uint64_t bal = __ballot(ok);
                     if (ok) {
                         const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
