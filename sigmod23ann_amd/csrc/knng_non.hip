@@ -239,10 +239,10 @@ constexpr uint32_t kNrPfRows = kNrChunk * kNrMaxK4 / kNrThreads;  // float4 per 
 constexpr uint32_t kNrPfSnap = kNrChunk * 128 / kNrThreads;        // snapshot ids per thread
 constexpr uint32_t kNrSurv = 512;     // accepted offers held in LDS per workgroup before appending
 typedef __attribute__((address_space(3))) uint32_t kLdsU32;
-// persistent workgroups per resident slot: the dispatcher hands a freed slot the next workgroup, so
-// smaller shares balance the end of a launch (NoN per 10M build: 4 -> 1606 ms, 16 -> 1581, 32 -> 1574,
-// 64 -> 1575)
-constexpr uint32_t kNrOversub = 32;
+// consecutive q per workgroup. Workgroups end often enough for the dispatcher to interleave the
+// previous sub-pass's merge, which runs beside on another stream (a persistent grid holding every
+// slot for the whole launch delayed the merges: 1554 ms of NoN per 10M build against 1338)
+constexpr uint32_t kNrRun = 32;
 static_assert(kNrIds == kNrThreads, "a q's first reverse ids are staged one per thread");
 static_assert(kNrThreads / 16 == kNrChunk, "16 threads commit each chunk row (one wave holds a row)");
 static_assert(kNrChunk == 32 && (kNrThreads / 64) % 2 == 0,
@@ -570,13 +570,14 @@ void k_non_reverse(NonRevArgs a) {
     constexpr uint32_t K32 = SCR ? (4 * KS + 31) / 32 : 1;
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t k = a.k, d = a.d, ld = a.ld;
-    // Persistent, XCD-aware schedule: the workgroups of one XCD (blockIdx % 8) share one contiguous
-    // stretch of this launch's q list, each taking every nslot-th q of it. A q's metadata (its
-    // position, reverse-list offsets, snapshot slots and first reverse ids) is loaded while the
-    // previous q is processed, so only the row gathers start cold.
-    const uint32_t per = (a.nq + 7) / 8, xcd = blockIdx.x & 7, nslot = gridDim.x >> 3;
-    const uint32_t i_end = min(a.nq, (xcd + 1) * per);
-    uint32_t idx = xcd * per + (blockIdx.x >> 3);
+    // XCD-aware schedule: the workgroups of one XCD (blockIdx % 8) share one contiguous stretch of
+    // this launch's q list, each taking a run of kNrRun consecutive q of it, so a workgroup's
+    // consecutive q are neighbours in the list (tree order) and find each other's rows in the
+    // XCD's L2. A q's metadata (its position, reverse-list offsets, snapshot slots and first
+    // reverse ids) is loaded while the previous q is processed, so only the row gathers start cold.
+    const uint32_t per = (a.nq + 7) / 8, xcd = blockIdx.x & 7, nslot = 1;
+    uint32_t idx = xcd * per + (blockIdx.x >> 3) * kNrRun;
+    const uint32_t i_end = min(min(a.nq, (xcd + 1) * per), idx + kNrRun);
     if (idx >= i_end) return;
     const NrLds L = nr_layout(k, ld, d, SCR);
     const uint32_t rs16 = L.rs16;
@@ -1087,6 +1088,7 @@ void k_non_reverse(NonRevArgs a) {
     // the chunk loop ends with a barrier: the list is complete
     nsurv = s_cnt[0];
     } else {
+        __syncthreads();  // every wave has read the s_meta entries prefetch_rs overwrites
         prefetch_rs();
     }
     // q's accepted offers go to the candidate buffers
@@ -1343,13 +1345,8 @@ void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {
     auto go = [&](auto kernel) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        // persistent grid: kNrOversub workgroups per resident slot, a multiple of the 8 XCDs
-        int dev = 0, cus = 0, per_cu = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kNrThreads, lds);
-        const uint32_t slots = std::max(1u, (uint32_t)(per_cu * cus) / 8u) * kNrOversub;
-        const uint32_t grid = 8 * std::min((a.nq + 7) / 8, slots);
+        // one workgroup per run of kNrRun q in each XCD's stretch, a multiple of the 8 XCDs
+        const uint32_t grid = 8 * (((a.nq + 7) / 8 + kNrRun - 1) / kNrRun);
         kernel<<<grid, kNrThreads, lds, st>>>(a);
     };
     const uint32_t unsc = scr == 3 && a.unscaled ? 1000u : 0u;
