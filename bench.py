@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--no-bucket-limit", action="store_true",
                     help="exact form: leaf screen against each point's list bound only, without the "
                          "in-bucket limit (the k-th smallest upper bound of its bucket distances)")
+    ap.add_argument("--side-fp32", action="store_true",
+                    help="split side test on the FP32 rows only (default: fp16 copy of the rows with "
+                         "an exact error bound, the rows inside it decided on their FP32 values)")
     ap.add_argument("--autotune", action="store_true",
                     help="leaf workgroup shape picked by timing both shapes on the first launch of "
                          "each size bin (the warmup builds pay for it)")
@@ -132,7 +135,15 @@ def copy_calibration(mib):
 
 def leaf_flags(a, K):
     return ({"binned": 0, "batched": K.FLAG_LEAF_BATCHED, "graph": K.FLAG_LEAF_GRAPH}[a.leaf_mode]
-            | (K.FLAG_NO_BUCKET_LIMIT if a.no_bucket_limit else 0))
+            | (K.FLAG_NO_BUCKET_LIMIT if a.no_bucket_limit else 0)
+            | (K.FLAG_SIDE_FP32 if a.side_fp32 else 0))
+
+
+def side_row_bytes(a, d):
+    """Algorithmic bytes per side-test position: the row (FP32, or its fp16 copy by default), the
+    permutation entry and the flag byte. The fp16 form's re-reads of FP32 rows inside its error
+    bound are not counted."""
+    return (4 * d if a.side_fp32 or a.splitter == "kd" else 2 * d) + 4 + 1
 
 
 def zipf_round(a, K):
@@ -164,7 +175,7 @@ def zipf_round(a, K):
             stats.append(st)
     st = stats[-1]
     mean_t = float(np.mean(times))
-    side_bytes = st["side_rows"] * (4 * d + 4 + 1)
+    side_bytes = st["side_rows"] * side_row_bytes(a, d)
     gram_flops = 2.0 * d * st["leaf_pairs"]
     out = {
         "metric": "one round (split + all buckets) of the batching form, 10M x 128 Zipf mixture",
@@ -172,7 +183,8 @@ def zipf_round(a, K):
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"Zipf(1.1) mixture {n}x{d}, 20000 clusters, sigma U[0.3,2], "
                                f"leaf<={leaf}, k={k}", "n": n, "d": d, "k": k, "leaf_max": leaf,
-                   "leaf_mode": a.leaf_mode, "bucket_limit": not a.no_bucket_limit},
+                   "leaf_mode": a.leaf_mode, "bucket_limit": not a.no_bucket_limit,
+                   "side_test": "fp32" if a.side_fp32 else "fp16 with exact bound"},
         "stages_ms": {"split": round(st["split_ms"], 2), "side": round(st["side_ms"], 2),
                       "leaf": round(st["leaf_ms"], 2), "total": round(st["total_ms"], 2)},
         "buckets": int(st["leaves"]), "leaf_pairs": int(st["leaf_pairs"]),
@@ -480,11 +492,12 @@ def main():
 
     # ---- roofline of the dominant kernel -----------------------------------------------------
     # Algorithmic bytes/flops per launch over the HIP-event launch time:
-    #   side test: 4d + 4 + 1 bytes per position (row, permutation entry, flag byte)
+    #   side test: 2d + 4 + 1 bytes per position (fp16 row, permutation entry, flag byte; 4d for
+    #   the FP32-only form)
     #   leaf: d * b(b-1) flops per bucket of b points (unique pairs x 2d)
     #   NoN screen: 4dk bytes per q (rows of its list) + (4d + 4k) per reverse entry (row, snapshot)
     cands = []
-    side_bytes = st["side_rows"] * (4 * d + 4 + 1)          # row, perm entry, flag byte
+    side_bytes = st["side_rows"] * side_row_bytes(a, d)     # row, perm entry, flag byte
     if st["side_ms"] > 0:
         cands.append(("k_side_count", "hbm", side_bytes, st["side_ms"], st["side_launches"]))
     leaf_flops = 2.0 * d * st["leaf_pairs"]                  # d * sum b(b-1): unique pairs x 2d
@@ -557,6 +570,7 @@ def main():
                    "n": n, "d": d, "k": k, "rounds": a.rounds, "iterations": a.iterations,
                    "non_passes": a.non, "precision": a.precision, "splitter": a.splitter, "autotune": a.autotune,
                    "leaf_mode": a.leaf_mode, "bucket_limit": not a.no_bucket_limit,
+                   "side_test": "fp32" if a.side_fp32 or a.splitter == "kd" else "fp16 with exact bound",
                    "recall_at_k": None if rec is None else round(rec, 4),
                    "recall_queries": nq, "parallelism": "single GPU"},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),

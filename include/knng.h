@@ -73,6 +73,9 @@ extern "C" {
                                         the big-bucket kernel (FP32 rows do not fit the LDS-resident
                                         kernel); the same lists. With KNNG_FLAG_NON_FP32_GRAM: the
                                         plain FP32 GPU form, no 16-bit Gram anywhere */
+#define KNNG_FLAG_SIDE_FP32 8192u    /* split side test on the FP32 rows only, without the fp16
+                                        copy whose rigorous error bound decides most points (the
+                                        rest re-read their FP32 rows); the same partitions */
 #define KNNG_FLAG_AUTOTUNE 32u       /* leaf stage: the first launch of each size bin above 128
                                         points (per device, precision and width, once per process)
                                         also times both workgroup shapes, 8 and 16 waves, and the

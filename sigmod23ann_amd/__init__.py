@@ -47,6 +47,7 @@ FLAG_AUTOTUNE = 32  # leaf workgroup shape picked by timing both on the first la
 FLAG_NO_BUCKET_LIMIT = 1024  # exact form: leaf screen without the in-bucket k-th upper-bound limit
 FLAG_LEAF_FP32_GRAM = 2048  # exact form: leaf screen on the FP32 Gram, every bucket on the big kernel
 FLAG_NON_ONE_SET = 4096  # tests: one set of NoN candidate buffers (as a build near device capacity)
+FLAG_SIDE_FP32 = 8192  # split side test on the FP32 rows only (no fp16 copy with its exact bound)
 
 GEN_LOWRANK = 0
 GEN_MIXTURE = 1

@@ -245,6 +245,8 @@ struct Builder {
 
     // points (internal copy when the caller's layout is not usable directly)
     DevBuf<float> xbuf;
+    DevBuf<uint16_t> x16;  // fp16 copy of the bound rows for the side test (split_ld16(d) halves each)
+    bool x16_ready = false;
     const float* X = nullptr;
     uint32_t n = 0, d = 0, ld = 0;
 
@@ -455,6 +457,7 @@ struct Builder {
         if (nn && !pts) return fail(KNNG_ERR_INVALID, "points is null");
         n = nn;
         d = dd;
+        x16_ready = false;
         const bool usable = on_device && dd % 4 == 0 && stride % 4 == 0 &&
                             (reinterpret_cast<uintptr_t>(pts) & 15) == 0;
         if (usable) {
@@ -475,11 +478,25 @@ struct Builder {
         return KNNG_OK;
     }
 
+    // The fp16 rows of the side test, converted once per binding of the points (on stream s).
+    int ensure_x16(hipStream_t s) {
+        if (x16_ready || (prm.flags & KNNG_FLAG_SIDE_FP32)) return KNNG_OK;
+        const uint32_t ld16 = split_ld16(d);
+        KNNG_TRY(x16.ensure((size_t)n * ld16));
+        launch_rows_to_half(X, ld, d, n, x16.p, ld16, s);
+        KNNG_HIP(hipGetLastError());
+        x16_ready = true;
+        return KNNG_OK;
+    }
+
     SplitLevelArgs level_args(uint32_t nn, uint32_t nc, uint32_t nt, bool lloyd) {
         SplitLevelArgs a{};
         a.X = X;
         a.ld = ld;
         a.d = d;
+        const bool half = x16_ready && !(prm.flags & KNNG_FLAG_SIDE_FP32);
+        a.X16 = half ? x16.p : nullptr;
+        a.ld16 = half ? split_ld16(d) : 0u;
         a.perm = split_perm;
         a.perm_tmp = perm_tmp.p;
         a.nodes = nodes.p;
@@ -608,6 +625,7 @@ struct Builder {
             if (n) out_leaves.push_back({0, n});
             return KNNG_OK;
         }
+        KNNG_TRY(ensure_x16(s));
         std::vector<HostNode> level{{0, n, round_key(prm.seed, round), 0}}, next;
         std::vector<uint32_t> lo, pending, again;
         while (!level.empty()) {
@@ -1643,6 +1661,7 @@ int knng_partition_injected(knng_handle* h, const float* points, uint32_t n, uin
     KNNG_HIP(hipMemcpyAsync(b.coef.p, cf.data(), cf.size() * 4, hipMemcpyHostToDevice, b.st));
     KNNG_HIP(hipMemcpyAsync(b.off.p, offsets, n_nodes * 4, hipMemcpyHostToDevice, b.st));
     b.split_perm = b.perm.p;
+    KNNG_TRY(b.ensure_x16(b.st));
     KNNG_TRY(b.run_level(n_nodes, 0, nt, false, b.st));
     for (uint32_t i = 0; i < n_nodes; ++i) lo_out[i] = b.h_nodes.p[i].lo;
     KNNG_HIP(hipMemcpy(perm, b.perm.p, (size_t)n * 4, hipMemcpyDeviceToHost));
@@ -2081,6 +2100,7 @@ uint64_t knng_job_footprint(uint32_t n, uint32_t d, const knng_params* params) {
     uint64_t per_point = 4 * ld + 8 * k + 8 * k + (incremental ? 16 : 12) * k +
                          16 * (uint64_t)Builder::kNonCap + 12 + 1 + 4 + 4 + 28 + 4 + 8 + 16;
     if (kd) per_point += 4 * (uint64_t)knng::kKdStride;
+    else if (!(params->flags & KNNG_FLAG_SIDE_FP32)) per_point += 2 * (uint64_t)knng::split_ld16(d);  // fp16 rows
     // Split tables of one level: at most one node per leaf_max + 1 points, each with its
     // descriptor, three centre/hyperplane rows, an offset and two FP64 partial rows per chunk
     // (one chunk per node plus one per 256 sample draws), and one tile per 1024 positions.

@@ -38,6 +38,8 @@ struct TileDesc {
 struct SplitLevelArgs {
     const float* X;
     uint32_t ld, d;
+    const uint16_t* X16;  // nullable: fp16 copy of the rows (launch_rows_to_half) for the side test
+    uint32_t ld16;
     uint32_t* perm;
     uint32_t* perm_tmp;
     NodeDesc* nodes;
@@ -74,6 +76,10 @@ void launch_iota(uint32_t* perm, uint32_t n, hipStream_t st);
 void launch_split_prepare(const SplitLevelArgs& a, hipStream_t st);    // start pair + Lloyd
 void launch_split_side(const SplitLevelArgs& a, hipStream_t st);       // full-range side test
 void launch_split_partition(const SplitLevelArgs& a, hipStream_t st);  // stable scatter
+// fp16 copy of the rows for the side test: ld16 = split_ld16(d) halves per row, zero past d
+inline uint32_t split_ld16(uint32_t d) { return (d + 63) & ~63u; }
+void launch_rows_to_half(const float* X, uint32_t ld, uint32_t d, uint32_t n, uint16_t* X16, uint32_t ld16,
+                         hipStream_t st);
 size_t leaf_lds_bytes(uint32_t max_leaf, uint32_t d, uint32_t ld);
 
 // ---- leaf stage -------------------------------------------------------------------------------

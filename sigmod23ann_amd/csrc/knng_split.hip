@@ -245,6 +245,150 @@ __global__ __launch_bounds__(kSideThreads) void k_side_count(
 
 size_t side_count_lds(uint32_t ld) { return (size_t)kSideRows * side_row_stride(ld) * sizeof(float); }
 
+// ---- side test on a 16-bit copy of the rows -------------------------------------------------------
+// The same decision, dot_ref(coef, x, d) >= off, with each row read as fp16 (ld16 halves, zero past
+// d: half the bytes of the FP32 row). A thread forms s~ = sum x~_i c_i and A = sum |x~_i c_i| over
+// the converted row x~ (FP32 arithmetic, any order) and bounds the distance to the reference's FP32
+// value s: the conversion moves each value by at most u |x~_i| (u = 2^-11 / (1 - 2^-11)), or 2^-25 in
+// fp16's subnormal range; each FP32 sum is within gamma_d = d 2^-24 / (1 - d 2^-24) of its exact
+// value relative to the sum of its absolute terms. For d <= 256,
+//   |s~ - s| <= (u + 2 gamma_d (1 + u)) / (1 - gamma_d) A + 2^-25 (1 + gamma_d) sum |c_i|
+//            <= 5.2e-4 A + 3.0e-8 sum |c_i|,
+// and the test takes E = 5.4e-4 A + 3.2e-8 sum |c_i| + 2^-22 (|s~| + |off|), which also covers the
+// rounding of s~ - off and of E. Where |s~ - off| <= E (or a value overflowed fp16: s~ and A are
+// then infinite or NaN, and neither test holds) the row's FP32 values decide through dot_ref itself,
+// after the tile's last pass, so the flags equal the FP32 test's bit for bit.
+constexpr uint32_t kSideTileMax = kTileSize;  // positions per tile (deferred rows at most)
+
+__host__ __device__ inline uint32_t side16_units(uint32_t ld16) { return ld16 / 8 + 1; }  // +1: bank stagger
+
+__global__ __launch_bounds__(kSideThreads) void k_side_count16(
+    const float* __restrict__ X, uint32_t ld, const uint16_t* __restrict__ X16, uint32_t ld16,
+    uint32_t d, const uint32_t* __restrict__ perm, const NodeDesc* __restrict__ nodes,
+    const TileDesc* __restrict__ tiles, const float* __restrict__ coef, const float* __restrict__ off,
+    uint8_t* __restrict__ flags, uint32_t* __restrict__ tile_lo) {
+    extern __shared__ __align__(16) uint4 s_rows16[];  // kSideRows rows of side16_units(ld16) x 8 halves
+    __shared__ __align__(16) float cf[kMaxDim];
+    __shared__ uint32_t s_id[kSideRows];
+    __shared__ uint16_t s_defer[kSideTileMax];  // tile positions left to the FP32 rows
+    __shared__ uint32_t nlo, ndefer;
+    __shared__ float s_csum;
+    const TileDesc tl = tiles[blockIdx.x];
+    const NodeDesc nd = nodes[tl.node];
+    if (nd.status == kNodeFailed) {
+        if (threadIdx.x == 0) tile_lo[blockIdx.x] = 0;
+        return;
+    }
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // coef past d reads as zero: the fp16 rows are zero there, and 0 * 0 adds nothing
+    for (uint32_t j = tid; j < kMaxDim; j += kSideThreads) cf[j] = j < d ? coef[(size_t)tl.node * ld + j] : 0.f;
+    if (tid == 0) nlo = ndefer = 0;
+    __syncthreads();
+    if (wave == 0) {
+        float c = 0.f;
+        for (uint32_t j = lane; j < kMaxDim; j += 64) c += fabsf(cf[j]);
+        c = wave_sum(c);
+        if (lane == 0) s_csum = c;
+    }
+    const float o = off[tl.node];
+    const uint32_t rs = side16_units(ld16), d8 = (d + 7) / 8;
+    // t / rs = umulhi(t, ceil(2^32 / rs)) exactly for t < 2^32 / rs (rs > 1)
+    const uint32_t inv = (uint32_t)((0x100000000ull + rs - 1) / rs);
+    __attribute__((address_space(3))) uint4* s_rows_lds = (__attribute__((address_space(3))) uint4*)(s_rows16);
+    uint32_t mylo = 0;
+    const uint32_t base = nd.start + tl.begin;
+    for (uint32_t s0 = 0; s0 < tl.count; s0 += kSideRows) {
+        const uint32_t ns = min(kSideRows, tl.count - s0);
+        s_id[tid] = tid < ns ? perm[base + s0 + tid] : 0u;
+        __syncthreads();  // ids in (and s_csum); the previous pass's rows all read
+        const uint32_t tot = ns * rs;
+        for (uint32_t t0 = 0; t0 < tot; t0 += kSideThreads) {
+            const uint32_t t = t0 + tid;
+            const uint32_t r = __umulhi(t, inv), q = t - r * rs;
+            if (t < tot && q < d8)
+                __builtin_amdgcn_global_load_lds(
+                    X16 + (size_t)s_id[r] * ld16 + 8 * q,
+                    (__attribute__((address_space(3))) void*)(s_rows_lds + t0 + wave * 64), 16, 0, 0);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid < ns) {
+            const uint4* row = s_rows16 + tid * rs;
+            float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            float aab[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (uint32_t q = 0; q < d8; ++q) {
+                const uint4 w = row[q];
+                const uint32_t wv[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (uint32_t e = 0; e < 8; ++e) {
+                    const float x = (float)__builtin_bit_cast(_Float16, (uint16_t)(wv[e >> 1] >> (16 * (e & 1))));
+                    const float c = cf[8 * q + e];
+                    acc[e] = __builtin_fmaf(x, c, acc[e]);
+                    aab[e] = __builtin_fmaf(fabsf(x), fabsf(c), aab[e]);
+                }
+            }
+            float sv = 0.f, av = 0.f;
+#pragma unroll
+            for (uint32_t e = 0; e < 8; ++e) {
+                sv += acc[e];
+                av += aab[e];
+            }
+            const float e_b = 5.4e-4f * av + 3.2e-8f * s_csum + 0x1p-22f * (fabsf(sv) + fabsf(o));
+            const float diff = sv - o;
+            if (diff > e_b || diff < -e_b) {
+                const bool lo = diff > e_b;
+                flags[base + s0 + tid] = lo;
+                mylo += lo;
+            } else {
+                s_defer[atomicAdd(&ndefer, 1u)] = (uint16_t)(s0 + tid);
+            }
+        }
+    }
+    __syncthreads();
+    // the deferred rows: the reference-order FP32 dot of the row itself
+    for (uint32_t i = tid; i < ndefer; i += kSideThreads) {
+        const uint32_t pos = s_defer[i];
+        const bool lo = dot_ref(cf, X + (size_t)perm[base + pos] * ld, d) >= o;
+        flags[base + pos] = lo;
+        mylo += lo;
+    }
+    atomicAdd(&nlo, mylo);
+    __syncthreads();
+    if (tid == 0) tile_lo[blockIdx.x] = nlo;
+}
+
+size_t side_count16_lds(uint32_t ld16) { return (size_t)kSideRows * side16_units(ld16) * 16; }
+
+// X16[i][j] = fp16(X[i][j]) (round to nearest even) for j < d, 0 for d <= j < ld16; one thread per
+// 8 halves.
+__global__ __launch_bounds__(256) void k_rows_to_half(const float* __restrict__ X, uint32_t ld, uint32_t d,
+                                                      uint32_t n, uint16_t* __restrict__ X16, uint32_t ld16) {
+    const uint32_t units = ld16 / 8;
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (uint64_t)n * units) return;
+    const uint32_t r = (uint32_t)(t / units), u = (uint32_t)(t - (uint64_t)r * units);
+    const uint32_t d4 = (d + 3) / 4;
+    const float4* xr = reinterpret_cast<const float4*>(X + (size_t)r * ld);
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 a = 2 * u < d4 ? xr[2 * u] : z, b = 2 * u + 1 < d4 ? xr[2 * u + 1] : z;
+    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t w[4];
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+        // dims past d inside the last float4 are the rows' zero padding
+        const uint16_t lo16 = __builtin_bit_cast(uint16_t, (_Float16)(8 * u + 2 * e < d ? v[2 * e] : 0.f));
+        const uint16_t hi16 = __builtin_bit_cast(uint16_t, (_Float16)(8 * u + 2 * e + 1 < d ? v[2 * e + 1] : 0.f));
+        w[e] = (uint32_t)lo16 | ((uint32_t)hi16 << 16);
+    }
+    reinterpret_cast<uint4*>(X16 + (size_t)r * ld16)[u] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+void launch_rows_to_half(const float* X, uint32_t ld, uint32_t d, uint32_t n, uint16_t* X16, uint32_t ld16,
+                         hipStream_t st) {
+    const uint64_t tot = (uint64_t)n * (ld16 / 8);
+    if (tot) k_rows_to_half<<<(uint32_t)((tot + 255) / 256), 256, 0, st>>>(X, ld, d, n, X16, ld16);
+}
+
 // Exclusive scan of tile lo counts over all tiles of the level (one block; tiles <= ~1e6).
 __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict__ tile_lo,
                                                      uint32_t ntiles,
@@ -361,6 +505,17 @@ void launch_split_prepare(const SplitLevelArgs& a, hipStream_t st) {
 }
 
 void launch_split_side(const SplitLevelArgs& a, hipStream_t st) {
+    if (a.X16) {
+        static const bool attr16 = [] {
+            return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_side_count16),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)side_count16_lds(kMaxDim)) == hipSuccess;
+        }();
+        (void)attr16;
+        k_side_count16<<<a.ntiles, kSideThreads, side_count16_lds(a.ld16), st>>>(
+            a.X, a.ld, a.X16, a.ld16, a.d, a.perm, a.nodes, a.tiles, a.coef, a.off, a.flags, a.tile_lo);
+        return;
+    }
     const size_t lds = side_count_lds(a.ld);
     static const bool attr_set = [] {
         return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_side_count),

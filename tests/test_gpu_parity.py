@@ -80,6 +80,44 @@ def test_partition_injected_bitexact(d):
     np.testing.assert_array_equal(perm, o_perm)
 
 
+@pytest.mark.parametrize("d", [100, 37, 256])
+def test_partition_injected_fp16_side_test_defers_to_fp32_rows(d):
+    # The default side test reads fp16 rows and decides only outside its error bound; points on or
+    # near the hyperplane, values that overflow fp16 and values in its subnormal range must reach
+    # the FP32 rows' decision: the same flags as the oracle and as the FP32-only side test.
+    rng = np.random.default_rng(4)
+    n = 12000
+    X = rng.integers(-3, 4, size=(n, d)).astype(np.float32)      # integer points: exact ties
+    X[3000:6000] += rng.choice([-1e-5, 0.0, 1e-5], size=(3000, d)).astype(np.float32)
+    X[6000:6100, 0] = 7e4                                          # beyond fp16's range
+    X[6100:9000] *= 1e-6                                           # fp16 subnormal range
+    perm0 = rng.permutation(n).astype(np.uint32)
+    starts = np.array([0, 3000, 6000, 9000], np.uint32)
+    sizes = np.array([3000, 3000, 3000, 3000], np.uint32)
+    coefs = rng.integers(-2, 3, size=(4, d)).astype(np.float32)
+    coefs[2] *= 1e-3
+    offs = np.array([0.0, 1.0, 0.0, 2.0], np.float32)
+    res = []
+    for flags in (0, K.FLAG_SIDE_FP32):
+        b = K.Builder(K.params(k=10, flags=flags))
+        res.append(b.partition_injected(X, perm0, starts, sizes, coefs, offs))
+    o_perm, o_lo = oracle.partition_injected(X, perm0, starts, sizes, coefs, offs)
+    for perm, lo in res:
+        np.testing.assert_array_equal(lo, o_lo)
+        np.testing.assert_array_equal(perm, o_perm)
+    # the integer nodes hold points exactly on their hyperplanes
+    on = (X[perm0[:3000]].astype(np.float64) @ coefs[0].astype(np.float64) == offs[0]).sum()
+    assert on > 0
+
+
+def test_split_round_fp16_side_test_equals_fp32_side_test():
+    X = lowrank(30000, 100, 8)
+    kw = dict(k=10, leaf_max=200, small_max=3000, iterations=2, seed=8)
+    out = [K.Builder(K.params(flags=f, **kw)).split_round(X, 1) for f in (0, K.FLAG_SIDE_FP32)]
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("n,d,leaf,small,iters,seed", [
     (50000, 100, 256, 3000, 2, 11),
     (30000, 100, 400, 3000, 1, 12),
