@@ -284,14 +284,19 @@ static_assert(kNrOffMeta % 8 == 0, "s_meta holds 64-bit words");
 
 struct NrLds {
     uint32_t kb16, ldp, k32, rs16;
-    uint32_t off_bid, off_bnrm, off_brow, off_arow, off_brinv, off_b16, off_a16, off_arinv, total;
+    uint32_t a16u;  // screen 3: 16-byte units per fp16 chunk row (odd: 16 rows' reads hit distinct banks)
+    uint32_t off_bid, off_bnrm, off_brow, off_arow, off_brinv, off_b16, off_a16, off_arinv, off_a16f, total;
 };
 
-// FP32 Gram (screen16 = 0): S(q)'s FP32 rows sit in LDS for the Gram and the exact re-rank.
+// FP32 Gram (screen16 = 0): S(q)'s FP32 rows sit in LDS for the Gram and the exact re-rank, k of
+// them: the Gram's last column tile also reads the rows up to kb16, which lie in the chunk rows that
+// follow (those columns are never offered). Screen 3 (fp16 Gram of the FP32 rows) adds the chunk's
+// rows converted to fp16 once per chunk (the Gram's A operand).
 // 16-bit screens: S(q)'s rows sit in LDS only as 16-bit rows centred on q (the exact re-rank reads
 // their FP32 rows from global memory), beside the chunk's FP32 and 16-bit rows, so two workgroups
 // still share a CU.
 __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld, uint32_t d, uint32_t screen16) {
+    const bool f16a = screen16 == 3;
     if (screen16 == 3) screen16 = 0;  // fp16 Gram of the FP32 rows in LDS: the FP32 layout
     NrLds L;
     L.kb16 = (k + 15) & ~15u;
@@ -303,12 +308,16 @@ __host__ __device__ inline NrLds nr_layout(uint32_t k, uint32_t ld, uint32_t d, 
     L.off_bnrm = o; o += L.kb16 * 4;
     L.off_brinv = o; o += screen16 ? L.kb16 * 4 : 0;
     o = (o + 15) & ~15u;
-    L.off_brow = o; o += screen16 ? 0 : L.kb16 * L.ldp * 4;
+    L.off_brow = o; o += screen16 ? 0 : k * L.ldp * 4;
     L.off_b16 = o;  o += screen16 ? L.kb16 * L.rs16 * 2 : 0;
     o = (o + 15) & ~15u;
     L.off_arow = o; o += kNrChunk * L.ldp * 4;
     L.off_a16 = o;  o += screen16 ? kNrChunk * L.rs16 * 2 : 0;
     L.off_arinv = o; o += screen16 ? kNrChunk * 4 : 0;
+    const uint32_t d4 = (d + 3) / 4;
+    L.a16u = ((d4 + 1) / 2) | 1u;
+    o = (o + 15) & ~15u;
+    L.off_a16f = o; o += f16a ? kNrChunk * L.a16u * 16 : 0;
     // 16-bit screens stage S(q)'s FP32 rows from off_b16 on, over the 16-bit and chunk regions
     if (screen16) o = o > L.off_b16 + L.kb16 * L.ldp * 4 ? o : L.off_b16 + L.kb16 * L.ldp * 4;
     L.total = (o + 15) & ~15u;
@@ -551,6 +560,35 @@ __device__ inline float4_t gram_tile_f16s(const float* __restrict__ arow, const 
     return c;
 }
 
+// gram_tile_f16s with the row operand already fp16 in LDS: a16 points at the lane's row, 8 halves
+// per 16-byte unit, dims 8 u .. 8 u + 7 in unit u (zero past d); units at or past ceil(KS / 2) read
+// as zero.
+template <uint32_t KS, uint32_t UNSC>
+__device__ inline float4_t gram_tile_f16a(const uint4* __restrict__ a16, const float* __restrict__ brow,
+                                          float sb, uint32_t g) {
+    if constexpr (UNSC != 0) sb = 1.0f;
+    constexpr uint32_t K32 = (4 * KS + 31) / 32;
+    constexpr uint32_t U = (KS + 1) / 2;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4_t c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (uint32_t m = 0; m < K32; ++m) {
+        const uint32_t ua = 4 * m + g;
+        const uint32_t q0 = 8 * m + 2 * g;  // float4 index of the lane's first column dims
+        const bool va = ua < U, v0 = q0 < KS, v1 = q0 + 1 < KS;
+        uint4 av = a16[va ? ua : 0u];
+        if (!va) av = make_uint4(0u, 0u, 0u, 0u);
+        float4 b0 = reinterpret_cast<const float4*>(brow)[v0 ? q0 : 0u];
+        float4 b1 = reinterpret_cast<const float4*>(brow)[v1 ? q0 + 1 : 0u];
+        if (!v0) b0 = z;
+        if (!v1) b1 = z;
+        const half8_t hb = {(_Float16)(b0.x * sb), (_Float16)(b0.y * sb), (_Float16)(b0.z * sb), (_Float16)(b0.w * sb),
+                            (_Float16)(b1.x * sb), (_Float16)(b1.y * sb), (_Float16)(b1.z * sb), (_Float16)(b1.w * sb)};
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8_t, av), hb, c, 0, 0, 0);
+    }
+    return c;
+}
+
 template <uint32_t SCR>
 __device__ inline float4_t mfma32(const uint16_t* a, const uint16_t* b, float4_t c) {
     const uint4 av = *reinterpret_cast<const uint4*>(a), bv = *reinterpret_cast<const uint4*>(b);
@@ -585,6 +623,7 @@ void k_non_reverse(NonRevArgs a) {
     uint16_t* s_a16 = reinterpret_cast<uint16_t*>(lds + L.off_a16);
     float* s_brinv = reinterpret_cast<float*>(lds + L.off_brinv);
     float* s_arinv = reinterpret_cast<float*>(lds + L.off_arinv);
+    uint4* s_a16f = reinterpret_cast<uint4*>(lds + L.off_a16f);  // screen 3: the chunk rows in fp16
     const uint32_t ldp = L.ldp, d4 = (d + 3) / 4, ldp4 = ldp / 4;
     // t / ldp4 = umulhi(t, ceil(2^32 / ldp4)) exactly for t < 2^32 / ldp4 (ldp4 > 1)
     const uint32_t ldp4_inv = ldp4 > 1 ? (uint32_t)((0x100000000ull + ldp4 - 1) / ldp4) : 0u;
@@ -872,6 +911,21 @@ void k_non_reverse(NonRevArgs a) {
             const uint32_t q4 = sc + 16 * u;
             if (q4 < ldp4) reinterpret_cast<float4*>(s_arow)[sr * ldp4 + q4] = pf_row[u];
         }
+        if constexpr (SCR == 3) {  // the Gram's row operand: the row in fp16 (scaled by its power of two)
+            static_assert(kNrPfRows == 2, "a thread holds dims 4 sc .. and 64 + 4 sc ..");
+            float sa_c = 1.f;
+            if constexpr (kScaled) sa_c = ldexpf(1.0f, (int)scale_exp(__shfl(pf_nrm, (int)(tid & ~15u) & 63)));
+            uint16_t* h = reinterpret_cast<uint16_t*>(s_a16f + sr * L.a16u);
+            auto pk = [&](float x, float y) {
+                return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)(x * sa_c)) |
+                       ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)(y * sa_c)) << 16);
+            };
+            const uint32_t nh = 8 * ((4 * d4 + 7) / 8);  // halves of the row's units below ceil(KS / 2)
+            if (4 * sc < nh)
+                *reinterpret_cast<uint2*>(h + 4 * sc) = make_uint2(pk(pf_row[0].x, pf_row[0].y), pk(pf_row[0].z, pf_row[0].w));
+            if (64 + 4 * sc < nh)
+                *reinterpret_cast<uint2*>(h + 64 + 4 * sc) = make_uint2(pk(pf_row[1].x, pf_row[1].y), pk(pf_row[1].z, pf_row[1].w));
+        }
         if constexpr (SCR == 1 || SCR == 2) {  // the row centred on q, 16-bit, with its centred norm
             static_assert(kNrPfRows == 2, "a thread holds dims 4 sc .. and 64 + 4 sc ..");
             float nrm, rinv;
@@ -1017,7 +1071,6 @@ void k_non_reverse(NonRevArgs a) {
             for (int i = 0; i < 4; ++i)
                 m2r[i] = kScaled ? -2.0f * ldexpf(1.0f, -(int)s_ascl[rr[i]]) : kFold ? -2.0f
                                   : SCR ? -2.0f * s_arinv[rr[i]] : 0.f;
-            const float sa = kScaled ? ldexpf(1.0f, (int)s_ascl[16 * strip + lrow]) : 1.f;
             for (uint32_t j = wv; j < nstrip * tt; j += kNrThreads / 64) {
                 const uint32_t t = nstrip == 2 ? j >> 1 : j;
                 if (qn + 256 > kNrQueue) flush();
@@ -1030,7 +1083,8 @@ void k_non_reverse(NonRevArgs a) {
                     acc = gram_tile_f32<KS>(arow, s_brow + (16 * t + lrow) * ldp, lk, ksteps);
                 } else if constexpr (SCR == 3) {
                     const float sb = kScaled ? ldexpf(1.0f, (int)s_bscl[16 * t + lrow]) : 1.f;
-                    acc = gram_tile_f16s<KS, UNSC>(arow, s_brow + (16 * t + lrow) * ldp, sa, sb, lane >> 4);
+                    acc = gram_tile_f16a<KS, UNSC>(s_a16f + (16 * strip + lrow) * L.a16u,
+                                                   s_brow + (16 * t + lrow) * ldp, sb, lane >> 4);
                 } else {
                     const uint16_t* b16 = s_b16 + (16 * t + lrow) * rs16 + 8 * (lane >> 4);
                     acc = float4_t{0.f, 0.f, 0.f, 0.f};
