@@ -435,8 +435,8 @@ def main():
     to_host = {"points_per_s": round(n / (t_b + t_d2h), 1), "build_ms": round(t_b * 1e3, 2),
                "ids_d2h_ms": round(t_d2h * 1e3, 2), "ids_bytes": n * k * 4,
                "definition": "one more build from device-resident points, then its n*k ids "
-                             "copied to pageable host memory (hipMemcpy); value excludes the "
-                             "copy"}
+                             "copied to pageable host memory (knng_memcpy_d2h: through two pinned "
+                             "32 MB buffers, copied on by host threads); value excludes the copy"}
     log(f"[bench] to host: build {t_b * 1e3:.1f} ms + ids copy {t_d2h * 1e3:.1f} ms "
         f"-> {n / (t_b + t_d2h):.0f} points/s")
 

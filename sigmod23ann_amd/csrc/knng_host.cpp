@@ -1477,6 +1477,63 @@ int upload_state(Builder& b, const uint32_t* ids, const float* dists, bool empty
     return KNNG_OK;
 }
 
+// Large device-to-host copies for callers without a builder (knng_memcpy_d2h): two pinned 32 MB
+// buffers on a stream of their own, each block copied on to the destination by host threads while
+// the next block arrives (as Builder::download does). Serialised by a mutex; kept for the process.
+struct HostStager {
+    std::mutex mu;
+    knng::PinBuf<unsigned char> stage[2];
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipStream_t st = nullptr;
+};
+
+HostStager& host_stager() {
+    static HostStager s;
+    return s;
+}
+
+int staged_d2h(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kBlk = Builder::kStageBytes;
+    HostStager& s = host_stager();
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (!s.st) {
+        KNNG_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+        for (auto& e : s.ev) KNNG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    for (int b = 0; b < 2; ++b) KNNG_TRY(s.stage[b].ensure(kBlk));
+    // the source may have been written on any stream: the blocking hipMemcpy this replaces waited
+    // for the device as well
+    KNNG_HIP(hipDeviceSynchronize());
+    const unsigned char* s8 = static_cast<const unsigned char*>(src);
+    unsigned char* d8 = static_cast<unsigned char*>(dst);
+    size_t pend_off[2] = {0, 0}, pend_len[2] = {0, 0};
+    auto drain = [&](int b) -> int {
+        if (!pend_len[b]) return KNNG_OK;
+        KNNG_HIP(hipEventSynchronize(s.ev[b]));
+        const unsigned char* from = s.stage[b].p;
+        unsigned char* to = d8 + pend_off[b];
+        const size_t len = pend_len[b];
+        Builder::parallel_rows(len / 4096 + 1, [=](size_t lo, size_t hi) {
+            const size_t a0 = std::min(lo * 4096, len), a1 = std::min(hi * 4096, len);
+            if (a1 > a0) std::memcpy(to + a0, from + a0, a1 - a0);
+        });
+        pend_len[b] = 0;
+        return KNNG_OK;
+    };
+    for (size_t o = 0, blk = 0; o < bytes; o += kBlk, ++blk) {
+        const int b = (int)(blk & 1);
+        KNNG_TRY(drain(b));
+        const size_t m = std::min(kBlk, bytes - o);
+        KNNG_HIP(hipMemcpyAsync(s.stage[b].p, s8 + o, m, hipMemcpyDeviceToHost, s.st));
+        KNNG_HIP(hipEventRecord(s.ev[b], s.st));
+        pend_off[b] = o;
+        pend_len[b] = m;
+    }
+    KNNG_TRY(drain(0));
+    KNNG_TRY(drain(1));
+    return KNNG_OK;
+}
+
 int download_state(Builder& b, uint32_t* ids, float* dists) {
     const size_t tot = (size_t)b.n * b.kl();
     KNNG_HIP(hipMemcpyAsync(ids, b.knn_ids.p, tot * 4, hipMemcpyDeviceToHost, b.st));
@@ -1944,6 +2001,7 @@ int knng_memcpy_h2d(void* dst, const void* src, size_t bytes) {
     return KNNG_OK;
 }
 int knng_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    if (bytes >= (64u << 20)) return staged_d2h(dst, src, bytes);
     KNNG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return KNNG_OK;
 }

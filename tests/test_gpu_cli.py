@@ -41,3 +41,15 @@ def test_cli_build_truth_and_grade(tmp_path):
                          check=True, capture_output=True, text=True).stdout
     avg = float(out.strip().splitlines()[-1].split(":")[1])
     assert abs(avg - 100 * K.recall(got, truth)) < 1e-3
+
+
+def test_large_device_to_host_copy_through_pinned_stages():
+    # knng_memcpy_d2h sends copies of 64 MB and more through two pinned 32 MB buffers: a size that
+    # is neither a multiple of the buffer nor of the host copy's 4096-byte pieces, written on the
+    # device by a copy of a host array, must come back unchanged, and so must a small copy
+    n = (96 << 20) // 4 + 1237
+    a = np.random.default_rng(3).integers(0, 2**32, size=n, dtype=np.uint32)
+    d = K.DeviceArray(a.nbytes)
+    d.upload(a)
+    np.testing.assert_array_equal(d.download(a.shape, np.uint32), a)
+    np.testing.assert_array_equal(d.download((1000,), np.uint32), a[:1000])
