@@ -309,7 +309,7 @@ struct Builder {
     DevBuf<uint32_t> ovf_flag;
     DevBuf<uint32_t> rcnt, rfill, rsl, ncnt, nbuf_id;
     DevBuf<uint32_t> snap_prev, rnew, rfill_old;
-    DevBuf<uint32_t> rslot;  // full passes only (incremental ones hold snap_prev instead)
+    DevBuf<unsigned char> rs_ws;  // the reverse-list sort's buffers when the candidate set is too small
     bool have_prev = false;  // snap_prev holds the previous pass's snapshot of the current lists
     uint32_t pass_no = 0;    // neighbours-of-neighbours passes so far in this build
     DevBuf<uint64_t> roff, scan_tmp;
@@ -1126,16 +1126,44 @@ struct Builder {
                                   : (prm.flags & KNNG_FLAG_NON_FP32_GRAM) ? 0u : 3u;
         const bool reverse = ld <= 128 && non_reverse_lds(k, ld, d, screen16) <= 160 * 1024 &&
                              !(prm.flags & KNNG_FLAG_DIRECT_NON);
-        if (reverse) {  // the snapshot kernel also counts the reverse lists (and, in full
-                        // passes, gives each entry its slot)
+        // Full passes: the snapshot kernel writes an (id, p) pair per entry that goes to a reverse
+        // list, and a stable radix sort by id turns them into the lists (in p order); incremental
+        // passes count and fill the lists with atomics, new entries first.
+        const bool sorted_rs = reverse && !incremental;
+        uint32_t *ka = nullptr, *kb = nullptr, *va = nullptr;
+        void* stmp = nullptr;
+        size_t stb = 0;
+        const uint32_t sbits = 32u - (uint32_t)__builtin_clz(n | 1u);  // 2^bits - 1 >= n
+        if (reverse) {
             KNNG_TRY(rcnt.ensure(n));
             KNNG_TRY(rnew.ensure(n));
-            if (!incremental) KNNG_TRY(rslot.ensure(nk));
+            KNNG_TRY(rsl.ensure(nk));
+            KNNG_TRY(nbuf_d.ensure((size_t)n * non_cap()));
+            KNNG_TRY(nbuf_id.ensure((size_t)n * non_cap()));
+        }
+        if (sorted_rs) {
+            // The sort's buffers lie in the first candidate set, idle until the pass's first
+            // screen (every merge of the previous pass is done): keys a and b in nbuf_d, values a
+            // and the temporary storage in nbuf_id. Buffers too small for them (the tests' 16-slot
+            // candidate buffers) give way to a workspace of their own.
+            stb = rsort_temp_bytes(nk, sbits);
+            if (stb == 0) return fail(KNNG_ERR_HIP, "radix sort: no temporary storage size");
+            const size_t cap_b = (size_t)n * non_cap() * 4;
+            if (2 * nk * 4 <= cap_b && nk * 4 + stb + 256 <= cap_b) {
+                ka = reinterpret_cast<uint32_t*>(nbuf_d.p);
+                kb = ka + nk;
+                va = nbuf_id.p;
+            } else {
+                KNNG_TRY(rs_ws.ensure(3 * nk * 4 + stb + 256));
+                ka = reinterpret_cast<uint32_t*>(rs_ws.p);
+                kb = ka + nk;
+                va = kb + nk;
+            }
+            stmp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(va + nk) + 255) & ~(uintptr_t)255);
         }
         launch_non_snapshot(knn_ids.p, snap.p, n, k, incremental ? snap_prev.p : nullptr,
-                            prm.non_rank_lo, reverse ? rcnt.p : nullptr,
-                            reverse && incremental ? rnew.p : nullptr,
-                            reverse && !incremental ? rslot.p : nullptr, st);
+                            prm.non_rank_lo, reverse && incremental ? rcnt.p : nullptr,
+                            reverse && incremental ? rnew.p : nullptr, ka, va, st);
         NonArgs da{};
         da.X = X;
         da.ld = ld;
@@ -1152,16 +1180,11 @@ struct Builder {
             have_prev = false;
             return KNNG_OK;
         }
-        KNNG_TRY(rcnt.ensure(n));
         KNNG_TRY(rfill.ensure(n));
-        KNNG_TRY(rnew.ensure(n));
         KNNG_TRY(rfill_old.ensure(n));
         KNNG_TRY(roff.ensure((size_t)n + 1));
         KNNG_TRY(scan_tmp.ensure(scan_scratch_elems(n)));
-        KNNG_TRY(rsl.ensure(nk));
         KNNG_TRY(ncnt.ensure(n));
-        KNNG_TRY(nbuf_d.ensure((size_t)n * non_cap()));
-        KNNG_TRY(nbuf_id.ensure((size_t)n * non_cap()));
         // The second candidate set (the merge of one sub-pass beside the next screen; 13 ms per
         // 10M build) is taken only while it leaves half the free memory: at 10M points its 41 GB
         // fit, and a build near 40M points on one GPU still runs, on one set with each merge
@@ -1182,12 +1205,21 @@ struct Builder {
         KNNG_TRY(ovf_flag.ensure(n));
         KNNG_TRY(perm.ensure(n));
         if (!perm_is_tree) launch_iota(perm.p, n, st);
-        launch_reverse_lists(snap.p, n, k, rcnt.p, incremental ? rnew.p : nullptr, roff.p, scan_tmp.p,
-                             rfill.p, rfill_old.p, incremental ? nullptr : rslot.p, rsl.p, st);
+        if (sorted_rs) {
+            bool in_b = false;
+            KNNG_HIP(rsort_pairs(stmp, stb, ka, kb, va, rsl.p, nk, sbits, st, &in_b));
+            if (!in_b) KNNG_HIP(hipMemcpyAsync(rsl.p, va, nk * 4, hipMemcpyDeviceToDevice, st));
+            launch_roff_search(in_b ? kb : ka, nk, n, roff.p, st);
+        } else {
+            launch_reverse_lists(snap.p, n, k, rcnt.p, rnew.p, roff.p, scan_tmp.p, rfill.p, rfill_old.p,
+                                 rsl.p, st);
+        }
         if (verbose) {
-            std::vector<uint32_t> rc(n);
-            KNNG_HIP(hipMemcpyAsync(rc.data(), rcnt.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+            std::vector<uint64_t> ro((size_t)n + 1);
+            KNNG_HIP(hipMemcpyAsync(ro.data(), roff.p, ((size_t)n + 1) * 8, hipMemcpyDeviceToHost, st));
             KNNG_HIP(hipStreamSynchronize(st));
+            std::vector<uint32_t> rc(n);
+            for (uint32_t q = 0; q < n; ++q) rc[q] = (uint32_t)(ro[q + 1] - ro[q]);
             std::sort(rc.begin(), rc.end());
             uint64_t tot = 0, big = 0;
             for (uint32_t v : rc) { tot += v; if (v > 1024) big += v; }
@@ -2150,12 +2182,13 @@ uint64_t knng_job_footprint(uint32_t n, uint32_t d, const knng_params* params) {
     const uint64_t ld = (d + 3) & ~3u, k = std::max(params->k, params->search_k);
     const bool incremental = (params->flags & KNNG_FLAG_INCREMENTAL_NON) != 0;
     const bool kd = (params->flags & KNNG_FLAG_KD_SPLIT) != 0;
-    // Per point: the points copy; top-k ids/dists; outputs; the snapshot, its reverse-list slots
-    // and the reverse lists (plus the previous snapshot in incremental passes); two sets of NoN
+    // Per point: the points copy; top-k ids/dists; outputs; the snapshot and the reverse lists
+    // (plus the previous snapshot in incremental passes; the full passes' sort of the reverse lists
+    // works in the first candidate set, or beside it for 16-slot buffers); two sets of NoN
     // candidate buffers; three permutations, side flags, bounds, norms, seven per-point counters,
     // overflow list, reverse offsets; two sets of leaf descriptors (at most one leaf per point);
     // projected rows for the KD splitter.
-    uint64_t per_point = 4 * ld + 8 * k + 8 * k + (incremental ? 16 : 12) * k +
+    uint64_t per_point = 4 * ld + 8 * k + 8 * k + (incremental ? 12 : 8) * k +
                          16 * (uint64_t)Builder::kNonCap + 12 + 1 + 4 + 4 + 28 + 4 + 8 + 16;
     if (kd) per_point += 4 * (uint64_t)knng::kKdStride;
     else if (!(params->flags & KNNG_FLAG_SIDE_FP32)) per_point += 2 * (uint64_t)knng::split_ld16(d);  // fp16 rows

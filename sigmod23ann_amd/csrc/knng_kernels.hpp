@@ -186,20 +186,29 @@ void launch_norm_range(const float* norms, uint32_t n, uint32_t* out, hipStream_
 // kExpBit. rank_lo = 0: unfilled slots -> id 0, every entry expanded; rank_lo > 0: unfilled
 // slots -> the point's own id, filled entries at ranks >= rank_lo expanded.
 // rcnt (nullable): also count the reverse lists (their sizes for launch_reverse_lists), and rnew
-// (nullable) their new entries; rslot (nullable, n*k, used without rnew): each counted entry's
-// slot in its reverse list, for launch_reverse_lists to place it without an atomic.
+// (nullable) their new entries.
+// pkey/pval (nullable, used without rcnt): n*k (id, p) pairs of the entries that go to a reverse
+// list, 0xFFFFFFFF keys for the other slots; sorted by key (rsort_pairs) their values are the
+// reverse lists, and launch_roff_search gives their offsets.
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
                          const uint32_t* prev, uint32_t rank_lo, uint32_t* rcnt, uint32_t* rnew,
-                         uint32_t* rslot, hipStream_t st);
+                         uint32_t* pkey, uint32_t* pval, hipStream_t st);
+void launch_roff_search(const uint32_t* keys, uint64_t count, uint32_t n, uint64_t* roff, hipStream_t st);
+// Stable radix sort of u32 (key, value) pairs by the keys' low `bits` bits (knng_rsort.hip): the
+// pairs start in (keys_a, vals_a) and end in the a or the b buffers (*in_b).
+size_t rsort_temp_bytes(size_t count, uint32_t bits);
+hipError_t rsort_pairs(void* temp, size_t temp_bytes, uint32_t* keys_a, uint32_t* keys_b, uint32_t* vals_a,
+                       uint32_t* vals_b, size_t count, uint32_t bits, hipStream_t st, bool* in_b);
 // plist == nullptr: points 0..np-1.
 void launch_non_direct(const NonArgs& a, const uint32_t* plist, uint32_t np, hipStream_t st);
 size_t scan_scratch_elems(uint32_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint64_t* out, uint64_t* scratch,
                            hipStream_t st);
-// rcnt (and rnew) hold the counts launch_non_snapshot made: offsets, then the lists
+// rcnt (and rnew) hold the counts launch_non_snapshot made: offsets, then the lists (entries
+// placed by atomics; rnew nullable: no new-first order)
 void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t* rcnt,
                           uint32_t* rnew, uint64_t* roff, uint64_t* scratch, uint32_t* rfill,
-                          uint32_t* rfill_old, const uint32_t* rslot, uint32_t* rs, hipStream_t st);
+                          uint32_t* rfill_old, uint32_t* rs, hipStream_t st);
 size_t non_reverse_lds(uint32_t k, uint32_t ld, uint32_t d, uint32_t screen16);
 // The reverse kernel's 16-bit centred screens cover d with 4 ceil(d / 4) in {64, 100, 128}.
 bool non_screen16_supported(uint32_t d);

@@ -36,8 +36,9 @@ namespace knng {
 // ---- snapshot sorted by id -------------------------------------------------------------------
 // rcnt (nullable): each entry that goes to a reverse list (expanded, and not a repeat of an id
 // that sorts before it, i.e. the repeated id 0 of an unfilled list) is counted for RS[id], the
-// same entries k_rs_fill places; rnew (nullable) counts the new ones; rslot (nullable, not with
-// rnew) receives each such entry's slot in RS[id] (the count before it), at its snapshot slot.
+// same entries k_rs_fill places; rnew (nullable) counts the new ones. pkey/pval (nullable,
+// without rcnt): the same entries as (id, p) pairs at the entry's snapshot slot, every other slot
+// as (0xFFFFFFFF, p), for a sort by id that yields the reverse lists.
 __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __restrict__ ids,
                                                              uint32_t* __restrict__ snap,
                                                              uint32_t n, uint32_t k,
@@ -45,7 +46,8 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
                                                              uint32_t rank_lo,
                                                              uint32_t* __restrict__ rcnt,
                                                              uint32_t* __restrict__ rnew,
-                                                             uint32_t* __restrict__ rslot) {
+                                                             uint32_t* __restrict__ pkey,
+                                                             uint32_t* __restrict__ pval) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (p >= n) return;
@@ -98,29 +100,38 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
     uint32_t* dst = snap + (size_t)p * k;
     if (lane < k) dst[r0] = v0 | f0 | x0;
     if (lane + 64 < k) dst[r1] = v1 | f1 | x1;
+    if (pkey) {
+        if (lane < k) {
+            pkey[(size_t)p * k + r0] = x0 && !rep0 ? v0 : 0xFFFFFFFFu;
+            pval[(size_t)p * k + r0] = p;
+        }
+        if (lane + 64 < k) {
+            pkey[(size_t)p * k + r1] = x1 && !rep1 ? v1 : 0xFFFFFFFFu;
+            pval[(size_t)p * k + r1] = p;
+        }
+    }
     if (rcnt) {
         if (lane < k && x0 && !rep0) {
-            const uint32_t at = atomicAdd(&rcnt[v0], 1u);
-            if (rslot) rslot[(size_t)p * k + r0] = at;
+            atomicAdd(&rcnt[v0], 1u);
             if (rnew && f0) atomicAdd(&rnew[v0], 1u);
         }
         if (lane + 64 < k && x1 && !rep1) {
-            const uint32_t at = atomicAdd(&rcnt[v1], 1u);
-            if (rslot) rslot[(size_t)p * k + r1] = at;
+            atomicAdd(&rcnt[v1], 1u);
             if (rnew && f1) atomicAdd(&rnew[v1], 1u);
         }
     }
 }
 
 // ---- reverse lists: RS[q] = { p : q in S[p] } ----------------------------------------------------
-// RS[q] holds the p with a new entry q first (rnew[q] of them), then the rest.
+// Incremental passes (full passes sort (id, p) pairs instead): RS[q] holds the p with a new entry q
+// first (rnew[q] of them), then the rest.
 // One wave per snapshot row (slots lane, lane + 64; k <= 128), four rows per workgroup. An entry
 // is placed when it is expanded and its id differs from the previous slot's (a repeated id 0 of
 // an unfilled list is placed once), the entries launch_non_snapshot counted.
 __global__ __launch_bounds__(256) void k_rs_fill(const uint32_t* __restrict__ snap, uint32_t n, uint32_t k,
                                                  const uint64_t* __restrict__ roff, const uint32_t* __restrict__ rcnt,
                                                  uint32_t* __restrict__ rfill, uint32_t* __restrict__ rfill_old,
-                                                 const uint32_t* __restrict__ rslot, uint32_t* __restrict__ rs) {
+                                                 uint32_t* __restrict__ rs) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (p >= n) return;
@@ -131,15 +142,13 @@ __global__ __launch_bounds__(256) void k_rs_fill(const uint32_t* __restrict__ sn
     const uint32_t up0 = __shfl_up(q0, 1), up1 = __shfl_up(q1, 1), last0 = __shfl(q0, 63);
     const bool ok0 = lane < k && (v0 & kExpBit) && !(lane > 0 && up0 == q0);
     const bool ok1 = lane + 64 < k && (v1 & kExpBit) && (lane == 0 ? last0 : up1) != q1;
-    // rslot: the slot the snapshot kernel's count gave the entry; else one taken here
-    auto place = [&](uint32_t v, uint32_t q, uint32_t e) {
-        const uint32_t pos = rslot ? rslot[(size_t)p * k + e]
-                             : (!rfill_old || (v & kNewBit)) ? atomicAdd(&rfill[q], 1u)
-                                                             : rcnt[q] - 1u - atomicAdd(&rfill_old[q], 1u);
+    auto place = [&](uint32_t v, uint32_t q) {
+        const uint32_t pos = (!rfill_old || (v & kNewBit)) ? atomicAdd(&rfill[q], 1u)
+                                                           : rcnt[q] - 1u - atomicAdd(&rfill_old[q], 1u);
         rs[roff[q] + pos] = p;
     };
-    if (ok0) place(v0, q0, lane);
-    if (ok1) place(v1, q1, lane + 64);
+    if (ok0) place(v0, q0);
+    if (ok1) place(v1, q1);
 }
 
 // ---- exclusive scan of n u32 counts into n+1 u64 offsets ----------------------------------------
@@ -1351,13 +1360,32 @@ void launch_norm_range(const float* norms, uint32_t n, uint32_t* out, hipStream_
 // ---- launchers ----------------------------------------------------------------------------------------
 void launch_non_snapshot(const uint32_t* ids, uint32_t* snap, uint32_t n, uint32_t k,
                          const uint32_t* prev, uint32_t rank_lo, uint32_t* rcnt, uint32_t* rnew,
-                         uint32_t* rslot, hipStream_t st) {
+                         uint32_t* pkey, uint32_t* pval, hipStream_t st) {
     if (!n) return;
     if (rcnt) (void)hipMemsetAsync(rcnt, 0, (size_t)n * 4, st);
     if (rcnt && rnew) (void)hipMemsetAsync(rnew, 0, (size_t)n * 4, st);
     k_non_snapshot_sorted<<<(n + 3) / 4, 256, 0, st>>>(ids, snap, n, k, prev, rank_lo, rcnt,
                                                        rcnt ? rnew : nullptr,
-                                                       rcnt && !rnew ? rslot : nullptr);
+                                                       rcnt ? nullptr : pkey, rcnt ? nullptr : pval);
+}
+
+// roff[q] = the first position of a key >= q in the sorted keys (q <= n): the reverse list of q
+// spans [roff[q], roff[q+1]); keys past the last list are 0xFFFFFFFF.
+__global__ __launch_bounds__(256) void k_roff_search(const uint32_t* __restrict__ keys, uint64_t count,
+                                                     uint32_t n, uint64_t* __restrict__ roff) {
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q > n) return;
+    uint64_t lo = 0, hi = count;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < q) lo = mid + 1;
+        else hi = mid;
+    }
+    roff[q] = lo;
+}
+
+void launch_roff_search(const uint32_t* keys, uint64_t count, uint32_t n, uint64_t* roff, hipStream_t st) {
+    k_roff_search<<<(n + 1 + 255) / 256, 256, 0, st>>>(keys, count, n, roff);
 }
 
 void launch_non_direct(const NonArgs& a, const uint32_t* plist, uint32_t np, hipStream_t st) {
@@ -1381,15 +1409,14 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint64_t* out, uint64
 
 void launch_reverse_lists(const uint32_t* snap, uint32_t n, uint32_t k, uint32_t* rcnt,
                           uint32_t* rnew, uint64_t* roff, uint64_t* scratch, uint32_t* rfill,
-                          uint32_t* rfill_old, const uint32_t* rslot, uint32_t* rs, hipStream_t st) {
+                          uint32_t* rfill_old, uint32_t* rs, hipStream_t st) {
     if (!n || !k) return;
     const unsigned g = (n + 3) / 4;
     // rnew == nullptr: no new/old split (every entry filled from the front)
-    if (rnew) rslot = nullptr;
-    if (!rslot) (void)hipMemsetAsync(rfill, 0, (size_t)n * 4, st);
+    (void)hipMemsetAsync(rfill, 0, (size_t)n * 4, st);
     if (rnew) (void)hipMemsetAsync(rfill_old, 0, (size_t)n * 4, st);
     launch_exclusive_scan(rcnt, n, roff, scratch, st);
-    k_rs_fill<<<g, 256, 0, st>>>(snap, n, k, roff, rcnt, rfill, rnew ? rfill_old : nullptr, rslot, rs);
+    k_rs_fill<<<g, 256, 0, st>>>(snap, n, k, roff, rcnt, rfill, rnew ? rfill_old : nullptr, rs);
 }
 
 void launch_non_reverse(const NonRevArgs& a, hipStream_t st) {

@@ -60,9 +60,9 @@ def test_server_admission_and_validation_without_gpu():
     f3 = K.job_footprint(1_000_000, 100, K.params(k=10))
     assert f2 > f1 > 0 and f3 < f2
     # per point: points (4 ld) and their fp16 copy for the side test (128 halves) + lists, outputs
-    # (8k each), the snapshot, its reverse-list slots and the reverse lists (12k) + two sets of NoN
-    # buffers, then per-point scalars and split tables (a few hundred bytes at most)
-    core = 4 * 100 + 2 * 128 + 28 * 100 + 16 * 512
+    # (8k each), the snapshot and the reverse lists (8k; their sort works in the candidate buffers)
+    # + two sets of NoN buffers, then per-point scalars and split tables (a few hundred bytes at most)
+    core = 4 * 100 + 2 * 128 + 24 * 100 + 16 * 512
     assert 900_000 * core < f2 - f1 < 900_000 * (core + 200)
     assert f2 - K.job_footprint(1_000_000, 100, K.params(k=100, flags=K.FLAG_SIDE_FP32)) == 256 * 1_000_000
     # the KD splitter's projected rows (32 floats) in place of the fp16 rows, and the incremental
