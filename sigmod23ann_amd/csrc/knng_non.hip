@@ -34,6 +34,44 @@
 namespace knng {
 
 // ---- snapshot sorted by id -------------------------------------------------------------------
+// Bitonic sort of 128 keys held two per lane (element lane in k0, element lane + 64 in k1),
+// ascending. Exchange partners lane ^ j: DPP quad permutes for j = 1, 2, swizzles within 32 lanes
+// for j < 32, a permute for 32 and an in-lane exchange for 64.
+template <uint32_t J>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+    if constexpr (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+    else if constexpr (J < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (int)(0x1F | (J << 10)));
+    else return (uint32_t)__shfl_xor((int)v, (int)J);
+}
+
+// merge step j of stage s: sequences of s elements, ascending where element & s is 0 (for
+// s = 64: k0's half ascending, k1's descending; s = 128: all ascending)
+template <uint32_t S, uint32_t J>
+__device__ __forceinline__ void bitonic_sub(uint32_t& k0, uint32_t& k1, uint32_t lane) {
+    const bool lower = (lane & J) == 0;
+    const bool asc0 = S >= 64 || (lane & S) == 0;
+    const bool asc1 = S >= 128 || (S < 64 && (lane & S) == 0);
+    const uint32_t o0 = xor_lane<J>(k0), o1 = xor_lane<J>(k1);
+    k0 = lower == asc0 ? min(k0, o0) : max(k0, o0);
+    k1 = lower == asc1 ? min(k1, o1) : max(k1, o1);
+    if constexpr (J > 1) bitonic_sub<S, J / 2>(k0, k1, lane);
+}
+
+template <uint32_t S>
+__device__ __forceinline__ void bitonic_stage(uint32_t& k0, uint32_t& k1, uint32_t lane) {
+    if constexpr (S == 128) {
+        const uint32_t lo = min(k0, k1);
+        k1 = max(k0, k1);
+        k0 = lo;
+        bitonic_sub<128, 32>(k0, k1, lane);
+    } else {
+        bitonic_sub<S, S / 2>(k0, k1, lane);
+        bitonic_stage<S * 2>(k0, k1, lane);
+    }
+}
+
+// ---- snapshot ---------------------------------------------------------------------------------
 // rcnt (nullable): each entry that goes to a reverse list (expanded, and not a repeat of an id
 // that sorts before it, i.e. the repeated id 0 of an unfilled list) is counted for RS[id], the
 // same entries k_rs_fill places; rnew (nullable) counts the new ones. pkey/pval (nullable,
@@ -58,29 +96,22 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
     if (un0) v0 = rank_lo ? p : 0u;
     if (un1) v1 = rank_lo ? p : 0u;
     // slots are in (dist, id) order, so a slot is its rank
-    const uint32_t x0 = (!rank_lo || (!un0 && lane >= rank_lo)) ? kExpBit : 0u;
-    const uint32_t x1 = (!rank_lo || (!un1 && lane + 64 >= rank_lo)) ? kExpBit : 0u;
-    // rank = number of (value, slot) pairs before mine; rep: an equal value sorts before mine
-    uint32_t r0 = 0, r1 = 0;
-    bool rep0 = false, rep1 = false;
-    const uint32_t e0 = lane, e1 = lane + 64;
-    // slot j's values broadcast by readlane (j is uniform): no LDS round trip per step
-    for (uint32_t j = 0; j < 64; ++j) {
-        const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v0, (int)j);
-        const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v1, (int)j);
-        if (j < k) {
-            r0 += (a < v0 || (a == v0 && j < e0));
-            r1 += (a < v1 || (a == v1 && j < e1));
-            rep0 |= a == v0 && j < e0;
-            rep1 |= a == v1 && j < e1;
-        }
-        if (j + 64 < k) {
-            r0 += (b < v0 || (b == v0 && j + 64 < e0));
-            r1 += (b < v1 || (b == v1 && j + 64 < e1));
-            rep0 |= b == v0 && j + 64 < e0;
-            rep1 |= b == v1 && j + 64 < e1;
-        }
-    }
+    const bool x0 = !rank_lo || (!un0 && lane >= rank_lo);
+    const bool x1 = !rank_lo || (!un1 && lane + 64 >= rank_lo);
+    // sort key id << 1 | expanded (ids < 2^30); slots past k sort last. Equal ids (the repeated
+    // fill id of an unfilled list) carry equal flags, so their order among themselves is moot.
+    uint32_t k0 = lane < k ? v0 << 1 | (uint32_t)x0 : 0xFFFFFFFFu;
+    uint32_t k1 = lane + 64 < k ? v1 << 1 | (uint32_t)x1 : 0xFFFFFFFFu;
+    bitonic_stage<2>(k0, k1, lane);
+    v0 = k0 >> 1;
+    v1 = k1 >> 1;
+    const uint32_t xb0 = k0 & 1u ? kExpBit : 0u, xb1 = k1 & 1u ? kExpBit : 0u;
+    // rep: the previous sorted slot holds the same id (slot lane's predecessor is lane - 1's,
+    // slot 64's is slot 63)
+    const uint32_t pr0 = (uint32_t)__shfl((int)v0, (int)((lane + 63) & 63));
+    const uint32_t pr1 = (uint32_t)__shfl((int)v1, (int)((lane + 63) & 63));
+    const bool rep0 = lane > 0 && pr0 == v0;
+    const bool rep1 = (lane > 0 ? pr1 : pr0) == v1;
     // new: not in the previous snapshot of this point (all new without one)
     uint32_t f0 = kNewBit, f1 = kNewBit;
     if (prev) {
@@ -98,24 +129,24 @@ __global__ __launch_bounds__(256) void k_non_snapshot_sorted(const uint32_t* __r
         f1 = h1 ? 0u : kNewBit;
     }
     uint32_t* dst = snap + (size_t)p * k;
-    if (lane < k) dst[r0] = v0 | f0 | x0;
-    if (lane + 64 < k) dst[r1] = v1 | f1 | x1;
+    if (lane < k) dst[lane] = v0 | f0 | xb0;
+    if (lane + 64 < k) dst[lane + 64] = v1 | f1 | xb1;
     if (pkey) {
         if (lane < k) {
-            pkey[(size_t)p * k + r0] = x0 && !rep0 ? v0 : 0xFFFFFFFFu;
-            pval[(size_t)p * k + r0] = p;
+            pkey[(size_t)p * k + lane] = xb0 && !rep0 ? v0 : 0xFFFFFFFFu;
+            pval[(size_t)p * k + lane] = p;
         }
         if (lane + 64 < k) {
-            pkey[(size_t)p * k + r1] = x1 && !rep1 ? v1 : 0xFFFFFFFFu;
-            pval[(size_t)p * k + r1] = p;
+            pkey[(size_t)p * k + lane + 64] = xb1 && !rep1 ? v1 : 0xFFFFFFFFu;
+            pval[(size_t)p * k + lane + 64] = p;
         }
     }
     if (rcnt) {
-        if (lane < k && x0 && !rep0) {
+        if (lane < k && xb0 && !rep0) {
             atomicAdd(&rcnt[v0], 1u);
             if (rnew && f0) atomicAdd(&rnew[v0], 1u);
         }
-        if (lane + 64 < k && x1 && !rep1) {
+        if (lane + 64 < k && xb1 && !rep1) {
             atomicAdd(&rcnt[v1], 1u);
             if (rnew && f1) atomicAdd(&rnew[v1], 1u);
         }
